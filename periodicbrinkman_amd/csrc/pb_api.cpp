@@ -1,0 +1,518 @@
+// pb_api.cpp -- the C ABI (include/periodicbrinkman.h): contexts, argument
+// checks turned into status codes, and the host orchestration of the kernels.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "pb_internal.h"
+
+// ---- context helpers --------------------------------------------------------
+int pb_fail(pb_context *c, int status, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return status;
+}
+
+int pb_hip_check(pb_context *c, hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return PB_OK;
+    return pb_fail(c, e == hipErrorOutOfMemory ? PB_ERR_NOMEM : PB_ERR_HIP,
+                   std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int pb_reserve(pb_context *c, pb_buffer *b, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    if (b->bytes >= bytes) return PB_OK;
+    if (b->ptr) {
+        hipError_t e = hipFree(b->ptr);
+        b->ptr = nullptr;
+        b->bytes = 0;
+        if (e != hipSuccess) return pb_hip_check(c, e, "hipFree");
+    }
+    size_t want = bytes + bytes / 8; // headroom for slowly growing sizes
+    hipError_t e = hipMalloc(&b->ptr, want);
+    if (e != hipSuccess) {
+        b->ptr = nullptr;
+        return pb_hip_check(c, e, "hipMalloc");
+    }
+    b->bytes = want;
+    return PB_OK;
+}
+
+void pb_mark(pb_context *c, hipStream_t s, int i)
+{
+    if (c && c->timing && c->ev_ready && i >= 0 && i <= PB_NPHASE)
+        (void)hipEventRecord(c->ev[c->cur][i], s);
+}
+
+namespace {
+
+// Fold a finished slot's phase durations into the running sums.
+int timing_collect(pb_context *c, int slot)
+{
+    if (!c->slot_used[slot]) return PB_OK;
+    if (hipEventSynchronize(c->ev[slot][PB_NPHASE]) != hipSuccess) return PB_ERR_HIP;
+    for (int i = 0; i < PB_NPHASE; i++) {
+        float t = 0;
+        if (hipEventElapsedTime(&t, c->ev[slot][i], c->ev[slot][i + 1]) != hipSuccess) return PB_ERR_HIP;
+        c->tsum[i] += t;
+    }
+    c->slot_used[slot] = 0;
+    c->nsummed++;
+    return PB_OK;
+}
+
+// Start the timing slot of a new call: every boundary is recorded once here so
+// phases that do not run keep a zero-length interval.
+int timing_begin(pb_context *c, hipStream_t s)
+{
+    if (!c->timing) return PB_OK;
+    int slot = (int)(c->ncalls % PB_TSLOTS);
+    int st = timing_collect(c, slot);
+    if (st) return st;
+    c->cur = slot;
+    c->slot_used[slot] = 1;
+    c->ncalls++;
+    for (int i = 0; i <= PB_NPHASE; i++) pb_mark(c, s, i);
+    return PB_OK;
+}
+
+} // namespace
+
+namespace {
+
+bool finite_pos(double v) { return std::isfinite(v) && v > 0; }
+
+int check_params(pb_context *c, const pb_params *p)
+{
+    if (!p) return pb_fail(c, PB_ERR_INVALID, "null params");
+    for (int i = 0; i < 3; i++)
+        if (!finite_pos(p->L[i])) return pb_fail(c, PB_ERR_INVALID, "box lengths must be finite and > 0");
+    if (!finite_pos(p->alpha)) return pb_fail(c, PB_ERR_INVALID, "alpha must be finite and > 0");
+    if (!(std::isfinite(p->eps) && p->eps >= 0)) return pb_fail(c, PB_ERR_INVALID, "eps must be >= 0");
+    if (!finite_pos(p->d)) return pb_fail(c, PB_ERR_INVALID, "split d must be > 0 (run pb_tune)");
+    if (!finite_pos(p->rc)) return pb_fail(c, PB_ERR_INVALID, "cutoff rc must be > 0 (run pb_tune)");
+    if (p->precision != PB_PREC_FP64 && p->precision != PB_PREC_MIXED)
+        return pb_fail(c, PB_ERR_INVALID, "precision must be PB_PREC_FP64 or PB_PREC_MIXED");
+    return PB_OK;
+}
+
+int make_cells(pb_context *c, const pb_params *p, pb_cells *cells)
+{
+    std::memset(cells, 0, sizeof(*cells));
+    long ncell = 1;
+    for (int i = 0; i < 3; i++) {
+        int n = (int)std::floor(p->L[i] / p->rc);
+        if (n < 1) n = 1;
+        if (n > 96) n = 96;
+        cells->nc[i] = n;
+        cells->h[i] = p->L[i] / n;
+        cells->inv_h[i] = n / p->L[i];
+        cells->st[i] = (int)std::ceil(p->rc / cells->h[i]);
+        if (cells->st[i] > 3)
+            return pb_fail(c, PB_ERR_UNSUPPORTED, "cutoff rc larger than 3 box lengths is not supported");
+        ncell *= n;
+    }
+    cells->ncell = (int)ncell;
+    cells->rc2 = p->rc * p->rc;
+    return PB_OK;
+}
+
+int make_kgrid(pb_context *c, const pb_params *p, pb_kgrid *g)
+{
+    std::memset(g, 0, sizeof(*g));
+    g->ng = 1;
+    for (int i = 0; i < 3; i++) {
+        if (p->grid[i] < 2) return pb_fail(c, PB_ERR_INVALID, "grid sizes must be >= 2 (run pb_tune)");
+        g->n[i] = p->grid[i];
+        g->P[i] = p->P;
+        g->h[i] = p->L[i] / p->grid[i];
+        g->inv_h[i] = (double)p->grid[i] / p->L[i];
+        g->ng *= p->grid[i];
+    }
+    double m = p->m > 0 ? p->m : std::sqrt(M_PI * p->P);
+    double eta = p->P * g->h[0] / m; // fftBrinkman.h:113-114
+    g->eta = eta * eta / 2;
+    double sc = 1.0 / (g->eta * M_PI);
+    g->scale = sc * std::sqrt(sc);
+    g->vol = g->h[0] * g->h[1] * g->h[2];
+    return PB_OK;
+}
+
+struct kparams_cache {
+    double a = -1, d = -1, e = -1, rfar = -1;
+    pb_kparams kp;
+};
+
+const pb_kparams &cached_kparams(double a, double d, double e, double rfar)
+{
+    static thread_local kparams_cache cache[2];
+    static thread_local int next = 0;
+    for (auto &k : cache)
+        if (k.a == a && k.d == d && k.e == e && k.rfar == rfar) return k.kp;
+    kparams_cache &k = cache[next];
+    next ^= 1;
+    pb_fill_kparams(&k.kp, a, d, e, rfar);
+    k.a = a;
+    k.d = d;
+    k.e = e;
+    k.rfar = rfar;
+    return k.kp;
+}
+
+int velocity_impl(pb_context *c, const pb_params *p, int ns, const double *xn, const double *f,
+                  int nt, const double *y, double *u, void *stream, bool do_real, bool do_k)
+{
+    if (!c) return PB_ERR_INVALID;
+    int st;
+    if ((st = check_params(c, p))) return st;
+    if (ns < 0 || nt < 0) return pb_fail(c, PB_ERR_INVALID, "negative point count");
+    if ((ns > 0 && (!xn || !f)) || (nt > 0 && (!y || !u)))
+        return pb_fail(c, PB_ERR_INVALID, "null point array");
+    if ((st = pb_hip_check(c, hipSetDevice(c->device), "hipSetDevice"))) return st;
+    hipStream_t s = (hipStream_t)stream;
+    pb_box box;
+    pb_fill_box(&box, p->L);
+    pb_cells cells;
+    if ((st = make_cells(c, p, &cells))) return st;
+    if ((st = timing_begin(c, s))) return pb_fail(c, st, "timing events");
+    if ((st = pb_bin_points(c, s, cells, box, ns, xn, f, true))) return st;
+    if ((st = pb_bin_points(c, s, cells, box, nt, y, nullptr, false))) return st;
+    for (int i = 1; i < PB_NPHASE; i++) pb_mark(c, s, i);
+    bool acc = false;
+    if (do_k) {
+        pb_kgrid g;
+        if ((st = make_kgrid(c, p, &g))) return st;
+        if ((st = pb_run_kspace(c, s, g, box, p->alpha, p->d, ns, nt, u, false, p->precision)))
+            return st;
+        acc = true;
+    }
+    if (do_real) {
+        double rfar = std::fmax(3 * p->d, 8 * p->eps);
+        const pb_kparams &kp = cached_kparams(p->alpha, p->d, p->eps, rfar);
+        if ((st = pb_run_real(c, s, cells, kp, box, ns, nt, u, acc, p->precision))) return st;
+    }
+    pb_mark(c, s, PB_NPHASE);
+    return PB_OK;
+}
+
+// A default context per host thread for the reference-signature entry points.
+pb_context *default_ctx()
+{
+    static thread_local pb_context *ctx = nullptr;
+    if (!ctx) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || pb_create(&ctx, dev) != PB_OK) ctx = nullptr;
+    }
+    return ctx;
+}
+
+// hipMemcpy that folds failures into a status
+int copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    if (bytes == 0) return PB_OK;
+    return hipMemcpy(dst, src, bytes, kind) == hipSuccess ? PB_OK : PB_ERR_HIP;
+}
+
+struct dev_tmp {
+    std::vector<void *> ptrs;
+    ~dev_tmp()
+    {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    double *alloc(size_t n)
+    {
+        void *p = nullptr;
+        if (hipMalloc(&p, (n ? n : 1) * sizeof(double)) != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return (double *)p;
+    }
+};
+
+int shells_impl(double *A, double *absA, int lda, const double *x, int nt, const double *x0,
+                int ns, const int *newM, const int *oldM, const double *L, double a, double d,
+                double e, bool fourier)
+{
+    if (!A || !absA || !x || !x0 || !newM || !oldM || !L) return PB_ERR_INVALID;
+    if (nt < 0 || ns < 0 || lda < 3 * nt) return PB_ERR_INVALID;
+    if (!(a > 0) || !(d > 0) || !(e >= 0)) return PB_ERR_INVALID;
+    for (int i = 0; i < 3; i++)
+        if (!(L[i] > 0) || newM[i] < 0 || oldM[i] < 0) return PB_ERR_INVALID;
+    bool home = newM[0] == oldM[0] && newM[1] == oldM[1];
+    if (fourier && home) return PB_OK; // periodicBrinkman.h:719-721
+    if (nt == 0 || ns == 0) return PB_OK;
+    dev_tmp t;
+    size_t na = (size_t)lda * 3 * ns;
+    double *dA = t.alloc(na), *dB = t.alloc(na), *dx = t.alloc(3 * (size_t)nt), *dx0 = t.alloc(3 * (size_t)ns);
+    if (!dA || !dB || !dx || !dx0) return PB_ERR_NOMEM;
+    int st = copy(dA, A, na * sizeof(double), hipMemcpyHostToDevice);
+    if (!st) st = copy(dB, absA, na * sizeof(double), hipMemcpyHostToDevice);
+    if (!st) st = copy(dx, x, 3 * (size_t)nt * sizeof(double), hipMemcpyHostToDevice);
+    if (!st) st = copy(dx0, x0, 3 * (size_t)ns * sizeof(double), hipMemcpyHostToDevice);
+    if (st) return st;
+    pb_box box;
+    pb_fill_box(&box, L);
+    const pb_kparams &kp = cached_kparams(a, d, e, 0.0);
+    if (fourier)
+        st = pb_run_dense(nullptr, dA, lda, dx, nt, dx0, ns, kp, box, nullptr, nullptr, 0, oldM, newM, 1, dB);
+    else
+        st = pb_run_dense(nullptr, dA, lda, dx, nt, dx0, ns, kp, box, oldM, newM, home ? 1 : 2, nullptr,
+                          nullptr, 0, dB);
+    if (st) return st;
+    if (hipDeviceSynchronize() != hipSuccess) return PB_ERR_HIP;
+    st = copy(A, dA, na * sizeof(double), hipMemcpyDeviceToHost);
+    if (!st) st = copy(absA, dB, na * sizeof(double), hipMemcpyDeviceToHost);
+    return st;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *pb_status_string(int status)
+{
+    switch (status) {
+    case PB_OK: return "ok";
+    case PB_ERR_INVALID: return "invalid argument";
+    case PB_ERR_HIP: return "HIP runtime error";
+    case PB_ERR_NOMEM: return "device out of memory";
+    case PB_ERR_UNSUPPORTED: return "unsupported configuration";
+    default: return "unknown status";
+    }
+}
+
+int pb_create(pb_context **ctx, int device)
+{
+    if (!ctx) return PB_ERR_INVALID;
+    *ctx = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PB_ERR_HIP;
+    if (device < 0 || device >= n) return PB_ERR_INVALID;
+    pb_context *c = new pb_context();
+    c->device = device;
+    *ctx = c;
+    return PB_OK;
+}
+
+int pb_destroy(pb_context *c)
+{
+    if (!c) return PB_OK;
+    (void)hipSetDevice(c->device);
+    pb_buffer *bufs[] = {&c->cell_count, &c->cell_start, &c->cell_of_s, &c->rank_s, &c->cell_of_t,
+                         &c->rank_t, &c->tcell_count, &c->tcell_start, &c->spos, &c->sfrc, &c->tpos,
+                         &c->tperm, &c->tiles, &c->tile_off, &c->ntiles, &c->counters, &c->grid,
+                         &c->twiddle, &c->axis_fac, &c->dense_tmp};
+    for (pb_buffer *b : bufs)
+        if (b->ptr) (void)hipFree(b->ptr);
+    if (c->ev_ready)
+        for (int k = 0; k < PB_TSLOTS; k++)
+            for (int i = 0; i <= PB_NPHASE; i++) (void)hipEventDestroy(c->ev[k][i]);
+    delete c;
+    return PB_OK;
+}
+
+const char *pb_last_error(const pb_context *c) { return c ? c->err.c_str() : "null context"; }
+
+int pb_set_timing(pb_context *c, int enable)
+{
+    if (!c) return PB_ERR_INVALID;
+    if (enable && !c->ev_ready) {
+        if (hipSetDevice(c->device) != hipSuccess) return PB_ERR_HIP;
+        for (int k = 0; k < PB_TSLOTS; k++)
+            for (int i = 0; i <= PB_NPHASE; i++)
+                if (hipEventCreate(&c->ev[k][i]) != hipSuccess) return PB_ERR_HIP;
+        c->ev_ready = 1;
+    }
+    // drain outstanding slots, then restart the averages
+    for (int k = 0; k < PB_TSLOTS; k++)
+        if (c->slot_used[k] && hipEventSynchronize(c->ev[k][PB_NPHASE]) != hipSuccess) return PB_ERR_HIP;
+    for (int k = 0; k < PB_TSLOTS; k++) c->slot_used[k] = 0;
+    for (int i = 0; i < PB_NPHASE; i++) c->tsum[i] = 0;
+    c->ncalls = c->nsummed = 0;
+    c->timing = enable ? 1 : 0;
+    return PB_OK;
+}
+
+int pb_last_timings(pb_context *c, float *ms, int n)
+{
+    if (!c || !ms || n < 0 || n > PB_NPHASE) return PB_ERR_INVALID;
+    if (!c->ev_ready) return pb_fail(c, PB_ERR_INVALID, "timing was never enabled");
+    for (int k = 0; k < PB_TSLOTS; k++) {
+        int st = timing_collect(c, k);
+        if (st) return pb_fail(c, st, "timing events");
+    }
+    for (int i = 0; i < n; i++) ms[i] = c->nsummed ? (float)(c->tsum[i] / c->nsummed) : 0.0f;
+    return PB_OK;
+}
+
+int pb_velocity(pb_context *ctx, const pb_params *p, int ns, const double *xn, const double *f,
+                int nt, const double *y, double *u, void *stream)
+{
+    return velocity_impl(ctx, p, ns, xn, f, nt, y, u, stream, true, true);
+}
+
+int pb_velocity_real(pb_context *ctx, const pb_params *p, int ns, const double *xn,
+                     const double *f, int nt, const double *y, double *u, void *stream)
+{
+    return velocity_impl(ctx, p, ns, xn, f, nt, y, u, stream, true, false);
+}
+
+int pb_velocity_kspace(pb_context *ctx, const pb_params *p, int ns, const double *xn,
+                       const double *f, int nt, const double *y, double *u, void *stream)
+{
+    return velocity_impl(ctx, p, ns, xn, f, nt, y, u, stream, false, true);
+}
+
+int pb_last_pair_counts(pb_context *c, long long *n64, long long *n32)
+{
+    if (!c || !n64 || !n32) return PB_ERR_INVALID;
+    *n64 = *n32 = 0;
+    if (!c->counters.ptr) return PB_OK;
+    unsigned long long h[2] = {0, 0};
+    int st = pb_hip_check(c, hipMemcpy(h, c->counters.ptr, sizeof(h), hipMemcpyDeviceToHost), "counters");
+    *n64 = (long long)h[0];
+    *n32 = (long long)h[1];
+    return st;
+}
+
+int periodicBrinkman(double *dA, int lda, const double *dx, int nt, const double *dx0, int ns,
+                     double a, double d, double e, int M, int N, const double *L, int maxBlockSize)
+{
+    (void)maxBlockSize;
+    if (!L || (nt > 0 && (!dA || !dx)) || (ns > 0 && !dx0)) return PB_ERR_INVALID;
+    if (nt < 0 || ns < 0 || lda < 3 * nt) return PB_ERR_INVALID; // periodicBrinkman.h:443-445
+    if (!(a > 0) || !(d > 0) || !(e >= 0)) return PB_ERR_INVALID;
+    for (int i = 0; i < 3; i++)
+        if (!(L[i] > 0)) return PB_ERR_INVALID;
+    // the reference's shell counts: sort L, real shells ceil(M Lmin/Li),
+    // Fourier shells ceil(N Lmax/Li) (periodicBrinkman.h:457-487)
+    int idL[3] = {0, 1, 2}, tmp;
+    if (L[idL[0]] > L[idL[1]]) { tmp = idL[0]; idL[0] = idL[1]; idL[1] = tmp; }
+    if (L[idL[2]] <= L[idL[0]]) {
+        tmp = idL[2]; idL[2] = idL[1]; idL[1] = idL[0]; idL[0] = tmp;
+    } else if (L[idL[2]] < L[idL[1]]) {
+        tmp = idL[1]; idL[1] = idL[2]; idL[2] = tmp;
+    }
+    int Mx[3] = {M, M, M}, Nx[3] = {N, N, N};
+    Mx[idL[2]] = (int)std::ceil(M * L[idL[0]] / L[idL[2]]);
+    Mx[idL[1]] = (int)std::ceil(M * L[idL[0]] / L[idL[1]]);
+    Nx[idL[0]] = (int)std::ceil(N * L[idL[2]] / L[idL[0]]);
+    Nx[idL[1]] = (int)std::ceil(N * L[idL[2]] / L[idL[1]]);
+    pb_box box;
+    pb_fill_box(&box, L);
+    const pb_kparams &kp = cached_kparams(a, d, e, 0.0);
+    int zero[3] = {0, 0, 0};
+    int st = PB_OK;
+    if (N > 0) st = pb_run_dense(nullptr, dA, lda, dx, nt, dx0, ns, kp, box, nullptr, nullptr, 0, zero, Nx, 1, nullptr);
+    if (st) return st;
+    if (M >= 0) st = pb_run_dense(nullptr, dA, lda, dx, nt, dx0, ns, kp, box, zero, Mx, 3, nullptr, nullptr, 0, nullptr);
+    return st;
+}
+
+int realShells(double *A, double *absA, int lda, const double *x, int nt, const double *x0, int ns,
+               const int *newM, const int *oldM, const double *L, double a, double d, double e)
+{
+    return shells_impl(A, absA, lda, x, nt, x0, ns, newM, oldM, L, a, d, e, false);
+}
+
+int fourierShells(double *A, double *absA, int lda, const double *x, int nt, const double *x0,
+                  int ns, const int *newM, const int *oldM, const double *L, double a, double d,
+                  double e)
+{
+    return shells_impl(A, absA, lda, x, nt, x0, ns, newM, oldM, L, a, d, e, true);
+}
+
+int pb_vR_batch(double *A, const double *xh, int n, double a, double d, double e, void *stream)
+{
+    if (n < 0 || (n > 0 && (!A || !xh)) || !(a > 0) || !(d > 0) || !(e >= 0)) return PB_ERR_INVALID;
+    return pb_run_vR_batch((hipStream_t)stream, A, xh, n, cached_kparams(a, d, e, 0.0));
+}
+
+int pb_vF_batch(double *A, const double *xh, const double *k, int n, double a, double d, void *stream)
+{
+    if (n < 0 || (n > 0 && (!A || !xh || !k)) || !(a > 0) || !(d > 0)) return PB_ERR_INVALID;
+    return pb_run_vF_batch((hipStream_t)stream, A, xh, k, n, a, d);
+}
+
+int vR(double *A, const double *xh, double a, double d, double e)
+{
+    if (!A || !xh) return PB_ERR_INVALID;
+    dev_tmp t;
+    double *dA = t.alloc(9), *dx = t.alloc(3);
+    if (!dA || !dx) return PB_ERR_NOMEM;
+    int st = copy(dx, xh, 3 * sizeof(double), hipMemcpyHostToDevice);
+    if (!st) st = pb_vR_batch(dA, dx, 1, a, d, e, nullptr);
+    if (!st) st = copy(A, dA, 9 * sizeof(double), hipMemcpyDeviceToHost);
+    return st;
+}
+
+int vF(double *A, const double *xh, double a, double d, const double *k)
+{
+    if (!A || !xh || !k) return PB_ERR_INVALID;
+    dev_tmp t;
+    double *dA = t.alloc(9), *dx = t.alloc(3), *dk = t.alloc(3);
+    if (!dA || !dx || !dk) return PB_ERR_NOMEM;
+    int st = copy(dx, xh, 3 * sizeof(double), hipMemcpyHostToDevice);
+    if (!st) st = copy(dk, k, 3 * sizeof(double), hipMemcpyHostToDevice);
+    if (!st) st = pb_vF_batch(dA, dx, dk, 1, a, d, nullptr);
+    if (!st) st = copy(A, dA, 9 * sizeof(double), hipMemcpyDeviceToHost);
+    return st;
+}
+
+int fftBrinkman(double *U, const double *y, int m, const double *f, const double *xn, int N,
+                const double *L, const int *M, const int *P, int msd, int C, double e, double tol,
+                double d, double alpha)
+{
+    (void)C;
+    (void)e;
+    (void)tol;
+    if (!L || !M || !P || m < 0 || N < 0 || (m > 0 && (!U || !y)) || (N > 0 && (!f || !xn)))
+        return PB_ERR_INVALID;
+    if (!(d > 0) || !(alpha > 0) || msd <= 0) return PB_ERR_INVALID;
+    pb_context *c = default_ctx();
+    if (!c) return PB_ERR_HIP;
+    if (hipSetDevice(c->device) != hipSuccess) return PB_ERR_HIP;
+    pb_params p;
+    pb_params_default(&p);
+    for (int i = 0; i < 3; i++) {
+        p.L[i] = L[i];
+        p.grid[i] = M[i];
+        if (!(L[i] > 0) || M[i] < 2 || P[i] < 1) return PB_ERR_INVALID;
+    }
+    p.alpha = alpha;
+    p.d = d;
+    p.m = msd;
+    p.P = P[0]; // eta follows axis 0 (fftBrinkman.h:113-114)
+    p.rc = std::fmin(L[0], std::fmin(L[1], L[2])); // binning only
+    dev_tmp t;
+    double *dy = t.alloc(3 * (size_t)m), *df = t.alloc(3 * (size_t)N), *dx = t.alloc(3 * (size_t)N),
+           *du = t.alloc(3 * (size_t)m);
+    if (!dy || !df || !dx || !du) return PB_ERR_NOMEM;
+    int st = copy(dy, y, 3 * (size_t)m * sizeof(double), hipMemcpyHostToDevice);
+    if (!st) st = copy(df, f, 3 * (size_t)N * sizeof(double), hipMemcpyHostToDevice);
+    if (!st) st = copy(dx, xn, 3 * (size_t)N * sizeof(double), hipMemcpyHostToDevice);
+    if (st) return st;
+    pb_box box;
+    pb_fill_box(&box, p.L);
+    pb_cells cells;
+    std::memset(&cells, 0, sizeof(cells));
+    for (int i = 0; i < 3; i++) {
+        cells.nc[i] = 1;
+        cells.h[i] = L[i];
+        cells.inv_h[i] = 1.0 / L[i];
+        cells.st[i] = 1;
+    }
+    cells.ncell = 1;
+    if ((st = pb_bin_points(c, nullptr, cells, box, N, dx, df, true))) return st;
+    if ((st = pb_bin_points(c, nullptr, cells, box, m, dy, nullptr, false))) return st;
+    pb_kgrid g;
+    if ((st = make_kgrid(c, &p, &g))) return st;
+    for (int i = 0; i < 3; i++) g.P[i] = P[i];
+    if ((st = pb_run_kspace(c, nullptr, g, box, alpha, d, N, m, du, false, PB_PREC_FP64))) return st;
+    if (hipDeviceSynchronize() != hipSuccess) return PB_ERR_HIP;
+    return copy(U, du, 3 * (size_t)m * sizeof(double), hipMemcpyDeviceToHost);
+}
+
+} // extern "C"

@@ -1,0 +1,145 @@
+// pb_internal.h -- declarations shared by the HIP translation units of libpbamd.
+//
+// Device data layout (HBM):
+//   sorted sources   spos[j] = (x, y, z, 0) wrapped into [0, L), double4
+//                    sfrc[j] = (fx, fy, fz, 0), double4
+//   sorted targets   tpos[t] = (x, y, z, 0), tperm[t] = original index
+//   cells            cell_start[c] .. cell_start[c+1] index the sorted arrays;
+//                    c = (c0 * nc1 + c1) * nc2 + c2
+//   target tiles     up to 64 targets of one cell: (cell, first, count)
+//   k-space grids    3 complex fields, element (i0, i1, i2) at (i0*n1 + i1)*n2 + i2
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/periodicbrinkman.h"
+
+#define PB_NSER 36  // terms of the (r/d)^2 series of the regularized functions
+#define PB_NSING 14 // terms of the (a r) series of the singular function
+
+// Timed phases of pb_velocity: binning, spread, forward FFT, multiply,
+// inverse FFT, gather, pair kernel.
+#define PB_NPHASE 7
+#define PB_TSLOTS 64
+
+// Radial-function constants for one split width (the Ewald split d or the blob e).
+struct pb_radial {
+    double a;      // Brinkman parameter
+    double d;      // split width
+    double inv_d;  // 1/d
+    double x;      // a d / 2
+    double ex2;    // e^{-x^2}
+    double k4pa2;  // 1/(4 pi a^2)
+    double kg;     // (1 - e^{-x^2}) / (2 a^2 d^3 pi^{3/2})
+    double zs;     // r/d below which the series is used
+    double c1[PB_NSER], c2[PB_NSER]; // regH = sum c1[m] z^{2m}, sum c2[m] z^{2m}, z = r/d
+};
+
+// FP32 constants for the far field (r >= 3d), split d only.
+struct pb_radial32 {
+    float a, inv_d, x, ex2, k4pa2, kg, d2;
+};
+
+// Everything the real-space kernels need about the kernel K_R.
+struct pb_kparams {
+    pb_radial pd;   // split d
+    pb_radial pe;   // blob e (valid when has_e)
+    pb_radial32 pf; // FP32 far field
+    int has_e;
+    double e8;      // 8 e: below it the blob term is evaluated
+    double d3;      // 3 d
+    double rfar2;   // (mixed precision) squared radius beyond which FP32 is used
+};
+
+struct pb_box {
+    double L[3];
+    double inv_L[3];
+};
+
+// Cell list geometry for the real-space pass.
+struct pb_cells {
+    int nc[3];      // cells per axis
+    int ncell;      // nc0 * nc1 * nc2
+    int st[3];      // stencil half-width per axis
+    double h[3];    // cell size
+    double inv_h[3];
+    double rc2;     // cutoff squared
+};
+
+struct pb_tile {
+    int cell, first, count, pad;
+};
+
+// k-space geometry.
+struct pb_kgrid {
+    int n[3];       // grid
+    int P[3];       // window points per axis
+    long long ng;   // n0 n1 n2
+    double h[3], inv_h[3];
+    double eta;     // Gaussian e^{-x^2/eta}
+    double scale;   // (pi eta)^{-3/2}
+    double vol;     // h0 h1 h2 (gather quadrature weight)
+};
+
+// ---- host helpers (pb_params.cpp) ----------------------------------------
+void pb_fill_radial(pb_radial *r, double a, double d);
+void pb_fill_kparams(pb_kparams *k, double a, double d, double e, double rfar);
+void pb_fill_box(pb_box *b, const double *L);
+
+// ---- context ----------------------------------------------------------------
+struct pb_buffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct pb_context {
+    int device = 0;
+    std::string err;
+    // real space
+    pb_buffer cell_count, cell_start, cell_of_s, rank_s, cell_of_t, rank_t, tcell_count,
+        tcell_start;
+    pb_buffer spos, sfrc, tpos, tperm, tiles, tile_off, ntiles, counters;
+    // k space
+    pb_buffer grid, twiddle, axis_fac;
+    int twiddle_n = 0;
+    int twiddle_prec = -1;
+    // dense / shells
+    pb_buffer dense_tmp;
+    // host copies of the last call's counters
+    long long last_fp64 = 0, last_fp32 = 0;
+    // phase timing (pb_set_timing): a ring of event sets, one per call, so a
+    // timed loop of up to PB_TSLOTS calls needs no host synchronization
+    int timing = 0;
+    int ev_ready = 0;
+    int cur = 0;
+    long long ncalls = 0, nsummed = 0;
+    hipEvent_t ev[PB_TSLOTS][PB_NPHASE + 1] = {};
+    int slot_used[PB_TSLOTS] = {};
+    double tsum[PB_NPHASE] = {};
+};
+
+// Record the boundary event of phase i (0 = start) when timing is on.
+void pb_mark(pb_context *c, hipStream_t s, int i);
+
+int pb_reserve(pb_context *c, pb_buffer *b, size_t bytes);
+int pb_fail(pb_context *c, int status, const std::string &msg);
+int pb_hip_check(pb_context *c, hipError_t e, const char *what);
+
+// ---- launchers (pb_bin.hip, pb_real.hip, pb_kspace.hip, pb_dense.hip) -------
+int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_box &box, int n,
+                  const double *pos, const double *frc, bool is_source);
+int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams &kp,
+                const pb_box &box, int ns, int nt, double *u, bool accumulate, int precision);
+int pb_run_kspace(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_box &box, double a,
+                  double d, int ns, int nt, double *u, bool accumulate, int precision);
+int pb_run_dense(hipStream_t s, double *dA, int lda, const double *y, int nt, const double *x,
+                 int ns, const pb_kparams &kp, const pb_box &box, const int *lo_real,
+                 const int *hi_real, int real_mode, const int *lo_k, const int *hi_k,
+                 int k_mode, double *absA);
+int pb_run_vR_batch(hipStream_t s, double *A, const double *xh, int n, const pb_kparams &kp);
+int pb_run_vF_batch(hipStream_t s, double *A, const double *xh, const double *k, int n,
+                    double a, double d);

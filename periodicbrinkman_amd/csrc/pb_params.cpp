@@ -1,0 +1,228 @@
+// pb_params.cpp -- host-side constants for the device kernels, the parameter
+// tuner and the deterministic input generator.
+#include <cmath>
+#include <cstring>
+
+#include "pb_internal.h"
+
+namespace {
+
+const long double kPi = 3.141592653589793238462643383279502884L;
+
+// a_n = A_n / (Gamma(n+1/2)/2), A_n(x) = int_0^inf t^{2n} e^{-t^2}/(t^2+x^2) dt.
+// Forward recursion a_{n+1} = (1 - x^2 a_n)/(n + 1/2) from a_0 = sqrt(pi) erfcx(x)/x
+// is stable while x^2 stays small against n; for x > 2 the integrals are
+// evaluated by 16-point Gauss-Legendre on panels of width 1/4.
+void scaled_moments(long double x, int count, long double *an)
+{
+    long double x2 = x * x;
+    if (x <= 2) {
+        an[0] = sqrtl(kPi) * expl(x2) * erfcl(x) / x;
+        for (int n = 0; n + 1 < count; n++) an[n + 1] = (1 - x2 * an[n]) / (n + 0.5L);
+        return;
+    }
+    static long double gx[16], gw[16];
+    static bool init = false;
+    if (!init) {
+        const int q = 16;
+        for (int i = 0; i < q; i++) {
+            long double t = cosl(kPi * (i + 0.75L) / (q + 0.5L));
+            for (int it = 0; it < 60; it++) {
+                long double p0 = 1, p1 = t;
+                for (int k = 2; k <= q; k++) {
+                    long double p2 = ((2 * k - 1) * t * p1 - (k - 1) * p0) / k;
+                    p0 = p1;
+                    p1 = p2;
+                }
+                long double dp = q * (t * p1 - p0) / (t * t - 1);
+                t -= p1 / dp;
+                if (it > 50) {
+                    gx[i] = t;
+                    gw[i] = 2 / ((1 - t * t) * dp * dp);
+                }
+            }
+        }
+        init = true;
+    }
+    long double G = sqrtl(kPi) / 2;
+    for (int n = 0; n < count; n++) {
+        long double T = sqrtl((long double)n + 1) + 10;
+        int panels = (int)(T * 4) + 1;
+        long double hp = T / panels, sum = 0;
+        for (int p = 0; p < panels; p++)
+            for (int i = 0; i < 16; i++) {
+                long double t = p * hp + 0.5L * hp * (gx[i] + 1), t2 = t * t;
+                sum += 0.5L * hp * gw[i] * powl(t2, n) * expl(-t2) / (t2 + x2);
+            }
+        an[n] = sum / G;
+        G *= (n + 0.5L);
+    }
+}
+
+} // namespace
+
+// Taylor coefficients of regH in z = r/d (H1 = sum c1 z^{2m}, H2 = sum c2 z^{2m}):
+// with I_n = int_0^inf phi_d(k) k^{2n}/(k^2 + a^2) dk,
+//   H1 = sum_m (-1)^m (2m+2)/((2m+3)(2m+1)!) I_{m+1} r^{2m}/(2 pi^2),
+//   H2 = sum_m (-1)^m 4(m+1)(m+2)/(2m+5)!  I_{m+2} r^{2m}/(2 pi^2),
+// and I_n = (d/2)^{1-2n} [A_n + gam A_{n+1}], gam = (1 - e^{-x^2})/x^2.
+void pb_fill_radial(pb_radial *r, double a, double d)
+{
+    std::memset(r, 0, sizeof(*r));
+    r->a = a;
+    r->d = d;
+    if (!(d > 0)) return;
+    r->inv_d = 1.0 / d;
+    r->x = a * d / 2;
+    r->ex2 = std::exp(-r->x * r->x);
+    double om = -std::expm1(-r->x * r->x);
+    r->k4pa2 = 1.0 / (4 * M_PI * a * a);
+    r->kg = om / (2 * a * a * d * d * d * M_PI * std::sqrt(M_PI));
+    double zs = 0.32 / std::sqrt(r->x);
+    r->zs = zs < 0.2 ? 0.2 : (zs > 2.0 ? 2.0 : zs);
+
+    const int NA = PB_NSER + 4;
+    long double x = (long double)a * d / 2, x2 = x * x;
+    long double gam = -expm1l(-x2) / x2;
+    long double an[PB_NSER + 4], Gn[PB_NSER + 4];
+    scaled_moments(x, NA, an);
+    Gn[0] = sqrtl(kPi) / 2;
+    for (int n = 1; n < NA; n++) Gn[n] = Gn[n - 1] * (n - 0.5L);
+    long double pi2 = 2 * kPi * kPi, fact = 1;
+    for (int m = 0; m < PB_NSER; m++) {
+        if (m > 0) fact *= (2 * m) * (2 * m + 1);
+        long double sgn = (m & 1) ? -1 : 1;
+        long double p2m1 = ldexpl(1, 2 * m + 1);
+        long double I1 = Gn[m + 1] * (an[m + 1] + gam * (m + 1.5L) * an[m + 2]);
+        r->c1[m] = (double)(sgn * (2 * m + 2) * p2m1 * I1 / ((2 * m + 3) * fact * pi2 * d));
+        long double f5 = fact * (2 * m + 2) * (2 * m + 3) * (2 * m + 4) * (2 * m + 5);
+        long double I2 = Gn[m + 2] * (an[m + 2] + gam * (m + 2.5L) * an[m + 3]);
+        r->c2[m] = (double)(sgn * 16 * (m + 1) * (m + 2) * p2m1 * I2 / (f5 * pi2 * d * d * d));
+    }
+}
+
+void pb_fill_kparams(pb_kparams *k, double a, double d, double e, double rfar)
+{
+    std::memset(k, 0, sizeof(*k));
+    pb_fill_radial(&k->pd, a, d);
+    k->has_e = e > 0;
+    if (k->has_e) pb_fill_radial(&k->pe, a, e);
+    k->e8 = 8 * e;
+    k->d3 = 3 * d;
+    k->rfar2 = rfar * rfar;
+    k->pf.a = (float)a;
+    k->pf.inv_d = (float)(1.0 / d);
+    k->pf.x = (float)k->pd.x;
+    k->pf.ex2 = (float)k->pd.ex2;
+    k->pf.k4pa2 = (float)k->pd.k4pa2;
+    k->pf.kg = (float)k->pd.kg;
+    k->pf.d2 = (float)(d * d);
+}
+
+void pb_fill_box(pb_box *b, const double *L)
+{
+    for (int i = 0; i < 3; i++) {
+        b->L[i] = L[i];
+        b->inv_L[i] = 1.0 / L[i];
+    }
+}
+
+extern "C" {
+
+void pb_params_default(pb_params *p)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->L[0] = p->L[1] = p->L[2] = 1.0;
+    p->alpha = 1.0;
+    p->P = 12;
+    p->precision = PB_PREC_FP64;
+    p->tol = 1e-6;
+}
+
+// Error model (relative).  Real space: truncation ~ e^{-(rc/d)^2}.  k space:
+// spreading and gathering each multiply mode k by e^{-eta k^2/4} and the
+// multiply divides both out, so the modes the grid drops carry
+// e^{-(d^2/4 - eta/2) k^2}; requiring this at the Nyquist wavenumber k_max
+// gives d^2 = 4 [ln(1/tol)/k_max^2 + eta/2].  The window half-width m balances
+// truncation of the Gaussian (e^{-m^2/2}) against aliasing (e^{-pi^2 P^2/(2 m^2)}):
+// m = sqrt(pi P).  The grid (when not given) gives about two grid points per
+// particle, rounded up to a power of two per axis.
+int pb_tune(pb_params *p, int ns, int nt)
+{
+    if (!p || !(p->alpha > 0) || !(p->L[0] > 0) || !(p->L[1] > 0) || !(p->L[2] > 0) || p->P < 2)
+        return PB_ERR_INVALID;
+    double tol = p->tol > 0 ? p->tol : 1e-6;
+    double lt = std::log(1.0 / tol);
+    double Lmin = std::fmin(p->L[0], std::fmin(p->L[1], p->L[2]));
+    if (p->grid[0] <= 0 || p->grid[1] <= 0 || p->grid[2] <= 0) {
+        double vol = p->L[0] * p->L[1] * p->L[2];
+        double npart = ns > nt ? ns : nt;
+        double hh = std::cbrt(vol / (2.0 * (npart > 1 ? npart : 1)));
+        for (int i = 0; i < 3; i++) {
+            int g = 16;
+            while (g < 512 && p->L[i] / g > hh) g *= 2;
+            p->grid[i] = g;
+        }
+    }
+    if (!(p->m > 0)) p->m = std::sqrt(M_PI * p->P);
+    if (!(p->d > 0)) {
+        double kmax = 1e300;
+        for (int i = 0; i < 3; i++) kmax = std::fmin(kmax, M_PI * p->grid[i] / p->L[i]);
+        double w = p->P * (p->L[0] / p->grid[0]) / p->m;
+        double eta = w * w / 2;
+        p->d = 2.0 * std::sqrt((lt + 2.0) / (kmax * kmax) + eta / 2);
+    }
+    if (!(p->rc > 0)) p->rc = p->d * std::sqrt(lt + 1.0);
+    if (p->rc > Lmin * 3) p->rc = Lmin * 3;
+    return PB_OK;
+}
+
+// ---- xoshiro256** seeded by SplitMix64 -------------------------------------
+static inline unsigned long long rotl(unsigned long long x, int k)
+{
+    return (x << k) | (x >> (64 - k));
+}
+
+void pb_rng_seed(unsigned long long *s, unsigned long long seed)
+{
+    unsigned long long z = seed;
+    for (int i = 0; i < 4; i++) {
+        z += 0x9E3779B97F4A7C15ULL;
+        unsigned long long t = z;
+        t = (t ^ (t >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        t = (t ^ (t >> 27)) * 0x94D049BB133111EBULL;
+        s[i] = t ^ (t >> 31);
+    }
+}
+
+static inline unsigned long long next64(unsigned long long *s)
+{
+    unsigned long long r = rotl(s[1] * 5, 7) * 9;
+    unsigned long long t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return r;
+}
+
+void pb_rng_uniform(unsigned long long *s, long n, double *out, double scale)
+{
+    for (long i = 0; i < n; i++) out[i] = (double)(next64(s) >> 11) * 0x1.0p-53 * scale;
+}
+
+void pb_rng_normal(unsigned long long *s, long n, double *out)
+{
+    for (long i = 0; i < n; i += 2) {
+        double u1 = (double)(next64(s) >> 11) * 0x1.0p-53;
+        double u2 = (double)(next64(s) >> 11) * 0x1.0p-53;
+        double r = std::sqrt(-2.0 * std::log(1.0 - u1));
+        double t = 2.0 * M_PI * u2;
+        out[i] = r * std::cos(t);
+        if (i + 1 < n) out[i + 1] = r * std::sin(t);
+    }
+}
+
+} // extern "C"
