@@ -108,6 +108,7 @@ def main():
 
     p = pb.Params(alpha=ALPHA, eps=EPS, grid=GRID, P=P, tol=TOL).tune(NS, NT)
     pm, p64 = p.copy(precision="mixed"), p.copy(precision="fp64")
+    pplain = p.copy(precision="fp64", kernel="direct")
 
     log(f"params {p}")
     ms_mixed, ph_mixed = timed_steps(ctx, pm, xs, fs, ys, u, args.steps, args.warmup)
@@ -118,6 +119,9 @@ def main():
     u_fp64 = u.cpu().numpy().copy()
     n64b, _ = ctx.last_pair_counts()
     log(f"fp64 form: {ms_fp64:.3f} ms/step {ph_fp64}")
+    ms_plain, ph_plain = timed_steps(ctx, pplain, xs, fs, ys, u, args.steps, args.warmup)
+    u_plain = u.cpu().numpy().copy()
+    log(f"plain fp64 (closed forms) form: {ms_plain:.3f} ms/step {ph_plain}")
 
     # FP64 reference with tighter parameters; a second split checks it
     lt = np.log(1e13)
@@ -125,12 +129,13 @@ def main():
     ref = []
     for dref in (2 * np.sqrt(lt + 2) / kmax, 2 * np.sqrt(lt + 4) / kmax):
         pr = pb.Params(alpha=ALPHA, eps=EPS, d=dref, rc=dref * np.sqrt(lt + 2), grid=GRID, P=20,
-                       precision="fp64", tol=1e-13).tune(NS, NT)
+                       precision="fp64", tol=1e-13, kernel="direct").tune(NS, NT)
         ref.append((pr, ctx.velocity(pr, xs, fs, ys).cpu().numpy()))
     torch.cuda.synchronize()
     pref, uref = ref[0]
     err_mixed = relL2(u_mixed, uref)
     err_fp64 = relL2(u_fp64, uref)
+    err_plain = relL2(u_plain, uref)
     ref_split = relL2(ref[1][1], uref)
     log(f"rel L2 mixed {err_mixed:.3e} fp64 {err_fp64:.3e} reference split check {ref_split:.3e}")
 
@@ -160,9 +165,15 @@ def main():
         "rel_l2_mixed": err_mixed,
         "rel_l2_fp64": err_fp64,
         "reference_split_check": ref_split,
+        "rel_l2_plain_fp64": err_plain,
         "fp64_form": {"value": NT / (ms_fp64 / 1e3), "ms_per_step": ms_fp64,
-                      "phases_ms": ph_fp64, "pairs_fp64": n64b},
+                      "phases_ms": ph_fp64, "pairs_fp64": n64b,
+                      "note": "every step FP64; real-space radial functions from the FP64 table"},
+        "plain_fp64_form": {"value": NT / (ms_plain / 1e3), "ms_per_step": ms_plain,
+                            "phases_ms": ph_plain,
+                            "note": "every step FP64; closed forms (exp, erfcx) evaluated per pair"},
         "speedup_mixed_vs_fp64": ms_fp64 / ms_mixed,
+        "speedup_mixed_vs_plain_fp64": ms_plain / ms_mixed,
         "phases_ms": ph_mixed,
         "pairs": {"fp64": n64, "fp32": n32},
         "roofline": {"bound": "valu", "kernel": "k_pair<mixed> (real-space pair kernel)",

@@ -37,6 +37,10 @@ extern "C" {
 #define PB_PREC_FP64 0  /* plain form: every step in FP64 */
 #define PB_PREC_MIXED 1 /* FP64 near field, FP32 far field and k-space */
 
+#define PB_KERNEL_TABLE 0  /* real-space radial functions from a piecewise-polynomial
+                              table built from the closed forms (default) */
+#define PB_KERNEL_DIRECT 1 /* closed forms (exp, erfcx) evaluated per pair */
+
 /* ---- entry points replacing the reference's functions -------------------- */
 
 /* Replaces periodicBrinkman(MatrixOnDevice dA, dx, dx0, a, d, e, M, N, L,
@@ -108,6 +112,7 @@ typedef struct pb_params {
     double m;      /* window half-width in standard deviations (<= 0: sqrt(pi P)) */
     int precision; /* PB_PREC_FP64 or PB_PREC_MIXED */
     double tol;    /* target relative error for pb_tune */
+    int kernel;    /* PB_KERNEL_TABLE or PB_KERNEL_DIRECT */
 } pb_params;
 
 /* Defaults: unit box, alpha 1, eps 0, P 12, FP64, tol 1e-6, the rest 0. */

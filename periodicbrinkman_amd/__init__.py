@@ -22,6 +22,7 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "periodicbrinkman.
 
 PB_OK, PB_ERR_INVALID, PB_ERR_HIP, PB_ERR_NOMEM, PB_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
+KERNEL_TABLE, KERNEL_DIRECT = 0, 1
 
 _lib = None
 _D = ctypes.POINTER(ctypes.c_double)
@@ -40,7 +41,7 @@ class _Params(ctypes.Structure):
     _fields_ = [("L", ctypes.c_double * 3), ("alpha", ctypes.c_double), ("eps", ctypes.c_double),
                 ("d", ctypes.c_double), ("rc", ctypes.c_double), ("grid", ctypes.c_int * 3),
                 ("P", ctypes.c_int), ("m", ctypes.c_double), ("precision", ctypes.c_int),
-                ("tol", ctypes.c_double)]
+                ("tol", ctypes.c_double), ("kernel", ctypes.c_int)]
 
 
 def build(jobs=8):
@@ -137,15 +138,18 @@ def _stream_ptr(stream):
 
 # ---- parameters ---------------------------------------------------------------
 class Params:
-    """Evaluator parameters (pb_params).  precision: 'fp64' or 'mixed'."""
+    """Evaluator parameters (pb_params).  precision: 'fp64' or 'mixed';
+    kernel: 'table' (radial functions from the piecewise-polynomial table) or
+    'direct' (closed forms per pair)."""
 
     def __init__(self, L=(1.0, 1.0, 1.0), alpha=1.0, eps=0.0, d=0.0, rc=0.0, grid=(0, 0, 0), P=12,
-                 m=0.0, precision="fp64", tol=1e-6):
+                 m=0.0, precision="fp64", tol=1e-6, kernel="table"):
         self.L = tuple(float(v) for v in L)
         self.alpha, self.eps, self.d, self.rc = float(alpha), float(eps), float(d), float(rc)
         self.grid = tuple(int(g) for g in grid)
         self.P, self.m, self.tol = int(P), float(m), float(tol)
         self.precision = precision
+        self.kernel = kernel
 
     def _struct(self):
         p = _Params()
@@ -155,6 +159,7 @@ class Params:
         p.alpha, p.eps, p.d, p.rc = self.alpha, self.eps, self.d, self.rc
         p.P, p.m, p.tol = self.P, self.m, self.tol
         p.precision = PREC_MIXED if self.precision == "mixed" else PREC_FP64
+        p.kernel = KERNEL_DIRECT if self.kernel == "direct" else KERNEL_TABLE
         return p
 
     def tune(self, ns, nt):
@@ -167,7 +172,7 @@ class Params:
 
     def copy(self, **kw):
         q = Params(self.L, self.alpha, self.eps, self.d, self.rc, self.grid, self.P, self.m,
-                   self.precision, self.tol)
+                   self.precision, self.tol, self.kernel)
         for k, v in kw.items():
             setattr(q, k, v)
         return q
@@ -175,7 +180,7 @@ class Params:
     def __repr__(self):
         return (f"Params(L={self.L}, alpha={self.alpha}, eps={self.eps}, d={self.d:.6g}, "
                 f"rc={self.rc:.6g}, grid={self.grid}, P={self.P}, m={self.m:.6g}, "
-                f"precision={self.precision!r}, tol={self.tol})")
+                f"precision={self.precision!r}, tol={self.tol}, kernel={self.kernel!r})")
 
 
 # ---- evaluator context ----------------------------------------------------------

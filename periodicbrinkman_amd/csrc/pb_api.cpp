@@ -96,6 +96,8 @@ int check_params(pb_context *c, const pb_params *p)
     if (!finite_pos(p->rc)) return pb_fail(c, PB_ERR_INVALID, "cutoff rc must be > 0 (run pb_tune)");
     if (p->precision != PB_PREC_FP64 && p->precision != PB_PREC_MIXED)
         return pb_fail(c, PB_ERR_INVALID, "precision must be PB_PREC_FP64 or PB_PREC_MIXED");
+    if (p->kernel != PB_KERNEL_TABLE && p->kernel != PB_KERNEL_DIRECT)
+        return pb_fail(c, PB_ERR_INVALID, "kernel must be PB_KERNEL_TABLE or PB_KERNEL_DIRECT");
     return PB_OK;
 }
 
@@ -162,6 +164,49 @@ const pb_kparams &cached_kparams(double a, double d, double e, double rfar)
     return k.kp;
 }
 
+// Device copy of the kernel constants (and, for PB_KERNEL_TABLE, the radial
+// table on [rcommon, rc]) for the given parameters, rebuilt on change.
+int device_kparams(pb_context *c, hipStream_t s, const pb_params *p, double rfar,
+                   const pb_kparams **out, size_t *table_doubles)
+{
+    int st;
+    const double key[6] = {p->alpha, p->d, p->eps, rfar, p->rc, (double)p->kernel};
+    if ((st = pb_reserve(c, &c->kparams, sizeof(pb_kparams)))) return st;
+    bool same = true;
+    for (int i = 0; i < 6; i++) same = same && c->kp_key[i] == key[i];
+    if (!same) {
+        pb_kparams kp = cached_kparams(p->alpha, p->d, p->eps, rfar);
+        kp.tn = 0;
+        c->table_doubles = 0;
+        if (p->kernel == PB_KERNEL_TABLE && p->rc > kp.rcommon) {
+            std::vector<double> coef;
+            pb_build_table(&kp, kp.rcommon / p->d, p->rc / p->d * (1 + 1e-9), coef);
+            if (kp.tn > 0) {
+                size_t n = coef.size();
+                std::vector<float> c32(coef.begin(), coef.end());
+                if ((st = pb_reserve(c, &c->table, n * (sizeof(double) + sizeof(float))))) return st;
+                kp.tab64 = (const double *)c->table.ptr;
+                kp.tab32 = (const float *)((char *)c->table.ptr + n * sizeof(double));
+                if ((st = pb_hip_check(c, hipMemcpyAsync(c->table.ptr, coef.data(), n * sizeof(double),
+                                                         hipMemcpyHostToDevice, s), "table upload")))
+                    return st;
+                if ((st = pb_hip_check(c, hipMemcpyAsync((void *)kp.tab32, c32.data(), n * sizeof(float),
+                                                         hipMemcpyHostToDevice, s), "table upload")))
+                    return st;
+                c->table_doubles = n;
+            }
+        }
+        if ((st = pb_hip_check(c, hipMemcpyAsync(c->kparams.ptr, &kp, sizeof(kp), hipMemcpyHostToDevice, s),
+                               "kparams upload")))
+            return st;
+        if ((st = pb_hip_check(c, hipStreamSynchronize(s), "kparams upload"))) return st;
+        for (int i = 0; i < 6; i++) c->kp_key[i] = key[i];
+    }
+    *out = (const pb_kparams *)c->kparams.ptr;
+    *table_doubles = c->table_doubles;
+    return PB_OK;
+}
+
 int velocity_impl(pb_context *c, const pb_params *p, int ns, const double *xn, const double *f,
                   int nt, const double *y, double *u, void *stream, bool do_real, bool do_k)
 {
@@ -185,14 +230,16 @@ int velocity_impl(pb_context *c, const pb_params *p, int ns, const double *xn, c
     if (do_k) {
         pb_kgrid g;
         if ((st = make_kgrid(c, p, &g))) return st;
-        if ((st = pb_run_kspace(c, s, g, box, p->alpha, p->d, ns, nt, u, false, p->precision)))
+        if ((st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u, false, p->precision)))
             return st;
         acc = true;
     }
     if (do_real) {
         double rfar = std::fmax(3 * p->d, 8 * p->eps);
-        const pb_kparams &kp = cached_kparams(p->alpha, p->d, p->eps, rfar);
-        if ((st = pb_run_real(c, s, cells, kp, box, ns, nt, u, acc, p->precision))) return st;
+        const pb_kparams *kp = nullptr;
+        size_t ntab = 0;
+        if ((st = device_kparams(c, s, p, rfar, &kp, &ntab))) return st;
+        if ((st = pb_run_real(c, s, cells, kp, ntab, box, ns, nt, u, acc, p->precision))) return st;
     }
     pb_mark(c, s, PB_NPHASE);
     return PB_OK;
@@ -303,7 +350,7 @@ int pb_destroy(pb_context *c)
     pb_buffer *bufs[] = {&c->cell_count, &c->cell_start, &c->cell_of_s, &c->rank_s, &c->cell_of_t,
                          &c->rank_t, &c->tcell_count, &c->tcell_start, &c->spos, &c->sfrc, &c->tpos,
                          &c->tperm, &c->tiles, &c->tile_off, &c->ntiles, &c->counters, &c->grid,
-                         &c->twiddle, &c->axis_fac, &c->dense_tmp};
+                         &c->twiddle, &c->axis_fac, &c->dense_tmp, &c->kparams, &c->table};
     for (pb_buffer *b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     if (c->ev_ready)
@@ -510,7 +557,7 @@ int fftBrinkman(double *U, const double *y, int m, const double *f, const double
     pb_kgrid g;
     if ((st = make_kgrid(c, &p, &g))) return st;
     for (int i = 0; i < 3; i++) g.P[i] = P[i];
-    if ((st = pb_run_kspace(c, nullptr, g, box, alpha, d, N, m, du, false, PB_PREC_FP64))) return st;
+    if ((st = pb_run_kspace(c, nullptr, g, cells, box, alpha, d, N, m, du, false, PB_PREC_FP64))) return st;
     if (hipDeviceSynchronize() != hipSuccess) return PB_ERR_HIP;
     return copy(U, du, 3 * (size_t)m * sizeof(double), hipMemcpyDeviceToHost);
 }

@@ -15,6 +15,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/periodicbrinkman.h"
 
@@ -53,7 +54,19 @@ struct pb_kparams {
     double e8;      // 8 e: below it the blob term is evaluated
     double d3;      // 3 d
     double rfar2;   // (mixed precision) squared radius beyond which FP32 is used
+    double rcommon; // r >= rcommon: only the closed form of diffH(d) is needed
+    // piecewise-polynomial table of (H1, H2) of diffH(d) on z = r/d in
+    // [tz0, tz0 + tn/tinv): interval i, t = 2((z - tz0) tinv - i) - 1 in [-1, 1],
+    // H = sum_k c[i][k][0|1] t^k; coefficients at tab64 / tab32 (device)
+    double tz0, tinv;
+    int tn, tdeg; // intervals, polynomial degree (tn = 0: no table)
+    const double *tab64;
+    const float *tab32;
 };
+
+#define PB_TAB_MAXDEG 15
+#define PB_TAB_DEG 12 // degree of the table polynomials
+#define PB_TAB_LDS_DOUBLES 2048 // LDS budget of the FP64 table (16 KB)
 
 struct pb_box {
     double L[3];
@@ -86,6 +99,10 @@ struct pb_kgrid {
 };
 
 // ---- host helpers (pb_params.cpp) ----------------------------------------
+// Build the table of diffH(d) on z in [z0, z1]: fills tz0, tinv, tn, tdeg and
+// the coefficients (layout [tn][deg+1][2]); returns the verified max error
+// relative to each interval's largest |H|.
+double pb_build_table(pb_kparams *k, double z0, double z1, std::vector<double> &coef);
 void pb_fill_radial(pb_radial *r, double a, double d);
 void pb_fill_kparams(pb_kparams *k, double a, double d, double e, double rfar);
 void pb_fill_box(pb_box *b, const double *L);
@@ -109,6 +126,11 @@ struct pb_context {
     int twiddle_prec = -1;
     // dense / shells
     pb_buffer dense_tmp;
+    // kernel constants in device memory (kernels take a pointer: a by-value
+    // copy of the series tables in kernel arguments costs registers)
+    pb_buffer kparams, table;
+    double kp_key[6] = {-1, -1, -1, -1, -1, -1};
+    size_t table_doubles = 0;
     // host copies of the last call's counters
     long long last_fp64 = 0, last_fp32 = 0;
     // phase timing (pb_set_timing): a ring of event sets, one per call, so a
@@ -132,10 +154,12 @@ int pb_hip_check(pb_context *c, hipError_t e, const char *what);
 // ---- launchers (pb_bin.hip, pb_real.hip, pb_kspace.hip, pb_dense.hip) -------
 int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_box &box, int n,
                   const double *pos, const double *frc, bool is_source);
-int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams &kp,
-                const pb_box &box, int ns, int nt, double *u, bool accumulate, int precision);
-int pb_run_kspace(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_box &box, double a,
-                  double d, int ns, int nt, double *u, bool accumulate, int precision);
+int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp_dev,
+                size_t table_doubles, const pb_box &box, int ns, int nt, double *u, bool accumulate,
+                int precision);
+int pb_run_kspace(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells,
+                  const pb_box &box, double a, double d, int ns, int nt, double *u, bool accumulate,
+                  int precision);
 int pb_run_dense(hipStream_t s, double *dA, int lda, const double *y, int nt, const double *x,
                  int ns, const pb_kparams &kp, const pb_box &box, const int *lo_real,
                  const int *hi_real, int real_mode, const int *lo_k, const int *hi_k,

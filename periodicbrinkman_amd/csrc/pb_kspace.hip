@@ -28,100 +28,278 @@ __device__ __forceinline__ int wrapi(int i, int n)
     return r < 0 ? r + n : r;
 }
 
+// ---- spreading ----------------------------------------------------------------
+// One wavefront owns a 4 x 4 x 4 block of grid points, one point per lane, and
+// gathers every source whose centred window (first index
+// f0 = floor(x/h - P/2) + 1, the oracle's formula) touches the block.
+// Candidates come from the real-space cell list: the cells covering the
+// x-interval [(4b - P/2) h, (4b + 3 + P/2) h) per axis, unwrapped so that a
+// source near the opposite face enters with its periodic image.  Lanes test 64
+// candidates at a time and append the hits to an LDS ring; every 64 hits each
+// lane computes the 4+4+4 block-local weights of one hit, then all lanes
+// accumulate their own point.  Every grid point is written exactly once, so
+// there are no atomics and the grid needs no clearing.
+constexpr int kSpreadWaves = 4;
+
+template <typename T> struct spread_lds {
+    uint2 q[128];      // (source index, image code)
+    T w[64][16];       // wx[4], wy[4], wz[4], scale * f[3]
+};
+
 template <typename T>
-__device__ __forceinline__ void window(double x, double h, double inv_h, int P, double eta,
-                                       int &first, T *w)
+__device__ __forceinline__ void spread_flush(spread_lds<T> &W, int qh, int cnt, int lane,
+                                             const double4 *__restrict__ spos,
+                                             const double4 *__restrict__ sfrc, const pb_box &box,
+                                             const pb_kgrid &g, const int *g0, const int *lidx,
+                                             T &a0, T &a1, T &a2)
 {
-    double sft = x * inv_h;
-    first = (int)floor(sft - 0.5 * P) + 1;
-    for (int p = 0; p < P; p++) {
-        double dx = (first + p) * h - x;
-        if (sizeof(T) == 8)
-            w[p] = (T)exp(-dx * dx / eta);
-        else
-            w[p] = (T)__expf(-(float)(dx * dx) * (float)(1.0 / eta));
+    if (lane < cnt) {
+        const uint2 e = W.q[(qh + lane) & 127];
+        const double4 x = spos[e.x];
+        const double4 f = sfrc[e.x];
+        const double xc[3] = {x.x, x.y, x.z};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const int l = (int)((e.y >> (3 * a)) & 7) - 4;
+            const int f0 = (int)floor(xc[a] * g.inv_h[a] - 0.5 * g.P[a]) + 1 + l * g.n[a];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int gq = g0[a] + q;
+                const int p = gq - f0;
+                const double dx = (double)(gq - l * g.n[a]) * g.h[a] - xc[a];
+                T w = (T)0;
+                if (p >= 0 && p < g.P[a]) {
+                    if (sizeof(T) == 8)
+                        w = (T)exp(-dx * dx / g.eta);
+                    else
+                        w = (T)__expf(-(float)(dx * dx) * (float)(1.0 / g.eta));
+                }
+                W.w[lane][4 * a + q] = w;
+            }
+        }
+        W.w[lane][12] = (T)(f.x * g.scale);
+        W.w[lane][13] = (T)(f.y * g.scale);
+        W.w[lane][14] = (T)(f.z * g.scale);
     }
+    __builtin_amdgcn_wave_barrier();
+    for (int k = 0; k < cnt; k++) {
+        const T t = W.w[k][lidx[0]] * W.w[k][4 + lidx[1]] * W.w[k][8 + lidx[2]];
+        a0 += t * W.w[k][12];
+        a1 += t * W.w[k][13];
+        a2 += t * W.w[k][14];
+    }
+    __builtin_amdgcn_wave_barrier();
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_spread(int ns, const double4 *__restrict__ spos,
-                                                const double4 *__restrict__ sfrc, pb_kgrid g,
-                                                T *__restrict__ grid)
+__global__ __launch_bounds__(64 * kSpreadWaves) void k_spread_blocks(
+    const int *__restrict__ cstart, const double4 *__restrict__ spos,
+    const double4 *__restrict__ sfrc, pb_cells cells, pb_box box, pb_kgrid g,
+    T *__restrict__ grid)
 {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ns) return;
-    double4 x = spos[j], f = sfrc[j];
-    int f0[3];
-    T w0[kPmax], w1[kPmax], w2[kPmax];
-    window<T>(x.x, g.h[0], g.inv_h[0], g.P[0], g.eta, f0[0], w0);
-    window<T>(x.y, g.h[1], g.inv_h[1], g.P[1], g.eta, f0[1], w1);
-    window<T>(x.z, g.h[2], g.inv_h[2], g.P[2], g.eta, f0[2], w2);
-    const size_t fs = 2 * (size_t)g.ng;
-    const T fx = (T)f.x * (T)g.scale, fy = (T)f.y * (T)g.scale, fz = (T)f.z * (T)g.scale;
-    for (int px = 0; px < g.P[0]; px++) {
-        const int jx = wrapi(f0[0] + px, g.n[0]);
-        for (int py = 0; py < g.P[1]; py++) {
-            const int jy = wrapi(f0[1] + py, g.n[1]);
-            const T wxy = w0[px] * w1[py];
-            const size_t row = ((size_t)jx * g.n[1] + jy) * g.n[2];
-            for (int pz = 0; pz < g.P[2]; pz++) {
-                const int jz = wrapi(f0[2] + pz, g.n[2]);
-                const T wt = wxy * w2[pz];
-                const size_t idx = 2 * (row + jz);
-                atomicAdd(&grid[idx], fx * wt);
-                atomicAdd(&grid[fs + idx], fy * wt);
-                atomicAdd(&grid[2 * fs + idx], fz * wt);
+    __shared__ spread_lds<T> lds[kSpreadWaves];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nb0 = g.n[0] >> 2, nb1 = g.n[1] >> 2, nb2 = g.n[2] >> 2;
+    const int bid = blockIdx.x * kSpreadWaves + wave;
+    if (bid >= nb0 * nb1 * nb2) return; // whole wavefront; no block barriers below
+    spread_lds<T> &W = lds[wave];
+    const int b[3] = {bid / (nb1 * nb2), (bid / nb2) % nb1, bid % nb2};
+    const int g0[3] = {4 * b[0], 4 * b[1], 4 * b[2]};
+    const int lidx[3] = {lane >> 4, (lane >> 2) & 3, lane & 3};
+    int ulo[3], uhi[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const double lo = (g0[a] - 0.5 * g.P[a]) * g.h[a], hi = (g0[a] + 3 + 0.5 * g.P[a]) * g.h[a];
+        ulo[a] = (int)floor(lo * cells.inv_h[a] - 1e-9);
+        uhi[a] = (int)floor(hi * cells.inv_h[a] + 1e-9);
+    }
+    T a0 = 0, a1 = 0, a2 = 0;
+    int qh = 0, qn = 0;
+    for (int u0 = ulo[0]; u0 <= uhi[0]; u0++) {
+        const int w0 = ((u0 % cells.nc[0]) + cells.nc[0]) % cells.nc[0], l0 = (u0 - w0) / cells.nc[0];
+        for (int u1 = ulo[1]; u1 <= uhi[1]; u1++) {
+            const int w1 = ((u1 % cells.nc[1]) + cells.nc[1]) % cells.nc[1], l1 = (u1 - w1) / cells.nc[1];
+            for (int u2 = ulo[2]; u2 <= uhi[2]; u2++) {
+                const int w2 = ((u2 % cells.nc[2]) + cells.nc[2]) % cells.nc[2], l2 = (u2 - w2) / cells.nc[2];
+                const int cell = (w0 * cells.nc[1] + w1) * cells.nc[2] + w2;
+                const int js = cstart[cell], je = cstart[cell + 1];
+                const unsigned code = (unsigned)(l0 + 4) | ((unsigned)(l1 + 4) << 3) | ((unsigned)(l2 + 4) << 6);
+                for (int j0 = js; j0 < je; j0 += 64) {
+                    const int j = j0 + lane;
+                    bool hit = false;
+                    if (j < je) {
+                        const double4 x = spos[j];
+                        const int f0x = (int)floor(x.x * g.inv_h[0] - 0.5 * g.P[0]) + 1 + l0 * g.n[0];
+                        const int f0y = (int)floor(x.y * g.inv_h[1] - 0.5 * g.P[1]) + 1 + l1 * g.n[1];
+                        const int f0z = (int)floor(x.z * g.inv_h[2] - 0.5 * g.P[2]) + 1 + l2 * g.n[2];
+                        hit = f0x > g0[0] - g.P[0] && f0x <= g0[0] + 3 && f0y > g0[1] - g.P[1] &&
+                              f0y <= g0[1] + 3 && f0z > g0[2] - g.P[2] && f0z <= g0[2] + 3;
+                    }
+                    const unsigned long long m = __ballot(hit);
+                    if (m == 0ull) continue;
+                    if (hit) {
+                        const unsigned pre = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                       __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                        W.q[(qh + qn + pre) & 127] = make_uint2((unsigned)j, code);
+                    }
+                    qn += __popcll(m);
+                    if (qn >= 64) {
+                        __builtin_amdgcn_wave_barrier();
+                        spread_flush<T>(W, qh, 64, lane, spos, sfrc, box, g, g0, lidx, a0, a1, a2);
+                        qh = (qh + 64) & 127;
+                        qn -= 64;
+                    }
+                }
             }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (qn > 0) spread_flush<T>(W, qh, qn, lane, spos, sfrc, box, g, g0, lidx, a0, a1, a2);
+    const size_t idx = 2 * (((size_t)(g0[0] + lidx[0]) * g.n[1] + (g0[1] + lidx[1])) * g.n[2] + (g0[2] + lidx[2]));
+    const size_t fs = 2 * (size_t)g.ng;
+    grid[idx] = a0;
+    grid[idx + 1] = (T)0;
+    grid[fs + idx] = a1;
+    grid[fs + idx + 1] = (T)0;
+    grid[2 * fs + idx] = a2;
+    grid[2 * fs + idx + 1] = (T)0;
+}
+
+// ---- interpolation (gather) --------------------------------------------------
+// u(y) = h^3 (pi eta)^{-3/2} sum_window e^{-|x_g - y|^2/eta} G(x_g).  One
+// wavefront per target: the 3P weights and wrapped indices are computed once
+// into LDS, then the lanes sweep the P^3 window with consecutive lanes on
+// consecutive z (contiguous memory) and the partial sums meet in a wave
+// reduction.
+constexpr int kInterpWaves = 4;
+
+template <typename T> __device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename T, int P>
+__global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, const double4 *__restrict__ tpos,
+                                                            const int *__restrict__ tperm, pb_kgrid g,
+                                                            const T *__restrict__ grid,
+                                                            double *__restrict__ u, int accumulate)
+{
+    __shared__ T wts[kInterpWaves][3][P];
+    __shared__ int ix[kInterpWaves][3][P];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * kInterpWaves + wave;
+    if (i >= nt) return; // whole wavefront; no block barriers below
+    const double4 y = tpos[i];
+    for (int k = lane; k < 3 * P; k += 64) {
+        const int a = k / P, p = k % P;
+        const double yc = a == 0 ? y.x : (a == 1 ? y.y : y.z);
+        const int f0 = (int)floor(yc * g.inv_h[a] - 0.5 * P) + 1;
+        const double dx = (double)(f0 + p) * g.h[a] - yc;
+        if (sizeof(T) == 8)
+            wts[wave][a][p] = (T)exp(-dx * dx / g.eta);
+        else
+            wts[wave][a][p] = (T)__expf(-(float)(dx * dx) * (float)(1.0 / g.eta));
+        int j = f0 + p;
+        j = j < 0 ? j + g.n[a] : (j >= g.n[a] ? j - g.n[a] : j);
+        ix[wave][a][p] = j;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const size_t fs = 2 * (size_t)g.ng;
+    T a0 = 0, a1 = 0, a2 = 0;
+    for (int k = lane; k < P * P * P; k += 64) {
+        const int pz = k % P, py = (k / P) % P, px = k / (P * P);
+        const size_t off = 2 * (((size_t)ix[wave][0][px] * g.n[1] + ix[wave][1][py]) * g.n[2] + ix[wave][2][pz]);
+        const T wt = wts[wave][0][px] * wts[wave][1][py] * wts[wave][2][pz];
+        a0 += wt * grid[off];
+        a1 += wt * grid[fs + off];
+        a2 += wt * grid[2 * fs + off];
+    }
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+    if (lane == 0) {
+        const double sc = g.scale * g.vol;
+        const size_t o = 3 * (size_t)tperm[i];
+        if (accumulate) {
+            u[o] += sc * (double)a0;
+            u[o + 1] += sc * (double)a1;
+            u[o + 2] += sc * (double)a2;
+        } else {
+            u[o] = sc * (double)a0;
+            u[o + 1] = sc * (double)a1;
+            u[o + 2] = sc * (double)a2;
         }
     }
 }
 
+// Per-axis window sizes that differ (fftBrinkman's P[3]) take this path.
 template <typename T>
-__global__ __launch_bounds__(256) void k_gather(int nt, const double4 *__restrict__ tpos,
-                                                const int *__restrict__ tperm, pb_kgrid g,
-                                                const T *__restrict__ grid,
-                                                double *__restrict__ u, int accumulate)
+__global__ __launch_bounds__(256) void k_interp_any(int nt, const double4 *__restrict__ tpos,
+                                                    const int *__restrict__ tperm, pb_kgrid g,
+                                                    const T *__restrict__ grid,
+                                                    double *__restrict__ u, int accumulate)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
-    double4 y = tpos[i];
+    const double4 y = tpos[i];
+    const double yc[3] = {y.x, y.y, y.z};
     int f0[3];
-    T w0[kPmax], w1[kPmax], w2[kPmax];
-    window<T>(y.x, g.h[0], g.inv_h[0], g.P[0], g.eta, f0[0], w0);
-    window<T>(y.y, g.h[1], g.inv_h[1], g.P[1], g.eta, f0[1], w1);
-    window<T>(y.z, g.h[2], g.inv_h[2], g.P[2], g.eta, f0[2], w2);
+    for (int a = 0; a < 3; a++) f0[a] = (int)floor(yc[a] * g.inv_h[a] - 0.5 * g.P[a]) + 1;
     const size_t fs = 2 * (size_t)g.ng;
-    T a0 = 0, a1 = 0, a2 = 0;
+    double a0 = 0, a1 = 0, a2 = 0;
     for (int px = 0; px < g.P[0]; px++) {
-        const int jx = wrapi(f0[0] + px, g.n[0]);
+        const double dxx = (double)(f0[0] + px) * g.h[0] - yc[0];
+        const double wx = exp(-dxx * dxx / g.eta);
+        const int jx = ((f0[0] + px) % g.n[0] + g.n[0]) % g.n[0];
         for (int py = 0; py < g.P[1]; py++) {
-            const int jy = wrapi(f0[1] + py, g.n[1]);
-            const size_t row = ((size_t)jx * g.n[1] + jy) * g.n[2];
-            T b0 = 0, b1 = 0, b2 = 0;
+            const double dxy = (double)(f0[1] + py) * g.h[1] - yc[1];
+            const double wxy = wx * exp(-dxy * dxy / g.eta);
+            const int jy = ((f0[1] + py) % g.n[1] + g.n[1]) % g.n[1];
             for (int pz = 0; pz < g.P[2]; pz++) {
-                const int jz = wrapi(f0[2] + pz, g.n[2]);
-                const size_t idx = 2 * (row + jz);
-                const T wz = w2[pz];
-                b0 += wz * grid[idx];
-                b1 += wz * grid[fs + idx];
-                b2 += wz * grid[2 * fs + idx];
+                const double dxz = (double)(f0[2] + pz) * g.h[2] - yc[2];
+                const double wt = wxy * exp(-dxz * dxz / g.eta);
+                const int jz = ((f0[2] + pz) % g.n[2] + g.n[2]) % g.n[2];
+                const size_t idx = 2 * (((size_t)jx * g.n[1] + jy) * g.n[2] + jz);
+                a0 += wt * (double)grid[idx];
+                a1 += wt * (double)grid[fs + idx];
+                a2 += wt * (double)grid[2 * fs + idx];
             }
-            const T wxy = w0[px] * w1[py];
-            a0 += wxy * b0;
-            a1 += wxy * b1;
-            a2 += wxy * b2;
         }
     }
     const double sc = g.scale * g.vol;
     const size_t o = 3 * (size_t)tperm[i];
     if (accumulate) {
-        u[o] += sc * (double)a0;
-        u[o + 1] += sc * (double)a1;
-        u[o + 2] += sc * (double)a2;
+        u[o] += sc * a0;
+        u[o + 1] += sc * a1;
+        u[o + 2] += sc * a2;
     } else {
-        u[o] = sc * (double)a0;
-        u[o + 1] = sc * (double)a1;
-        u[o + 2] = sc * (double)a2;
+        u[o] = sc * a0;
+        u[o + 1] = sc * a1;
+        u[o + 2] = sc * a2;
+    }
+}
+
+template <typename T>
+void launch_interp(hipStream_t s, int nt, const pb_kgrid &g, const void *tpos, const void *tperm,
+                   const T *grid, double *u, int acc)
+{
+    dim3 gd((nt + kInterpWaves - 1) / kInterpWaves), bd(64 * kInterpWaves);
+    dim3 gd1((nt + 255) / 256), bd1(256);
+    const double4 *tp = (const double4 *)tpos;
+    const int *pm = (const int *)tperm;
+    bool same = g.P[0] == g.P[1] && g.P[1] == g.P[2];
+    switch (same ? g.P[0] : -1) {
+    case 4: hipLaunchKernelGGL((k_interp<T, 4>), gd, bd, 0, s, nt, tp, pm, g, grid, u, acc); break;
+    case 6: hipLaunchKernelGGL((k_interp<T, 6>), gd, bd, 0, s, nt, tp, pm, g, grid, u, acc); break;
+    case 8: hipLaunchKernelGGL((k_interp<T, 8>), gd, bd, 0, s, nt, tp, pm, g, grid, u, acc); break;
+    case 10: hipLaunchKernelGGL((k_interp<T, 10>), gd, bd, 0, s, nt, tp, pm, g, grid, u, acc); break;
+    case 12: hipLaunchKernelGGL((k_interp<T, 12>), gd, bd, 0, s, nt, tp, pm, g, grid, u, acc); break;
+    case 16: hipLaunchKernelGGL((k_interp<T, 16>), gd, bd, 0, s, nt, tp, pm, g, grid, u, acc); break;
+    case 20: hipLaunchKernelGGL((k_interp<T, 20>), gd, bd, 0, s, nt, tp, pm, g, grid, u, acc); break;
+    case 24: hipLaunchKernelGGL((k_interp<T, 24>), gd, bd, 0, s, nt, tp, pm, g, grid, u, acc); break;
+    default: hipLaunchKernelGGL(k_interp_any<T>, gd1, bd1, 0, s, nt, tp, pm, g, grid, u, acc); break;
     }
 }
 
@@ -247,14 +425,6 @@ __global__ __launch_bounds__(256) void k_multiply(typename cplx<T>::type *__rest
     F2[q] = o2;
 }
 
-template <typename T>
-__global__ void k_zero(T *__restrict__ p, long long n)
-{
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long stride = (long long)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) p[i] = (T)0;
-}
-
 template <typename T, int N>
 void launch_fft(hipStream_t s, typename cplx<T>::type *grid, const pb_kgrid &g, int axis,
                 const typename cplx<T>::type *tw, int inverse)
@@ -327,15 +497,15 @@ int upload_twiddles(pb_context *c, hipStream_t s, const pb_kgrid &g)
 }
 
 template <typename T>
-int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_box &box, double a,
-                 double d, int ns, int nt, double *u, bool accumulate)
+int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells,
+                 const pb_box &box, double a, double d, int ns, int nt, double *u, bool accumulate)
 {
     using C = typename cplx<T>::type;
     int st;
     for (int ax = 0; ax < 3; ax++) {
         int n = g.n[ax];
-        if (n < 2 || (n & (n - 1)) || n > 512)
-            return pb_fail(c, PB_ERR_UNSUPPORTED, "FFT grid sizes must be powers of two in [2, 512]");
+        if (n < 4 || (n & (n - 1)) || n > 512)
+            return pb_fail(c, PB_ERR_UNSUPPORTED, "FFT grid sizes must be powers of two in [4, 512]");
         if (g.P[ax] < 1 || g.P[ax] > kPmax || g.P[ax] > n)
             return pb_fail(c, PB_ERR_INVALID, "window points P must be in [1, min(32, grid)]");
     }
@@ -358,12 +528,14 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_box &
                                              hipMemcpyHostToDevice, s), "factor upload")))
         return st;
     C *grid = (C *)c->grid.ptr;
-    long long nreal = 3 * 2 * g.ng;
-    hipLaunchKernelGGL(k_zero<T>, dim3(2048), dim3(256), 0, s, (T *)grid, nreal);
     pb_mark(c, s, 1);
-    if (ns > 0)
-        hipLaunchKernelGGL(k_spread<T>, dim3((ns + 255) / 256), dim3(256), 0, s, ns,
-                           (const double4 *)c->spos.ptr, (const double4 *)c->sfrc.ptr, g, (T *)grid);
+    {
+        long long nb = (long long)(g.n[0] / 4) * (g.n[1] / 4) * (g.n[2] / 4);
+        hipLaunchKernelGGL(k_spread_blocks<T>, dim3((unsigned)((nb + kSpreadWaves - 1) / kSpreadWaves)),
+                           dim3(64 * kSpreadWaves), 0, s, (const int *)c->cell_start.ptr,
+                           (const double4 *)c->spos.ptr, (const double4 *)c->sfrc.ptr, cells, box, g,
+                           (T *)grid);
+    }
     const C *tw = (const C *)c->twiddle.ptr;
     const C *tws[3] = {tw, tw + g.n[0] / 2, tw + g.n[0] / 2 + g.n[1] / 2};
     pb_mark(c, s, 2);
@@ -378,20 +550,18 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_box &
     for (int ax = 0; ax < 3; ax++)
         if ((st = fft_dispatch<T>(s, grid, g, ax, tws[ax], 1))) return pb_fail(c, st, "FFT size");
     pb_mark(c, s, 5);
-    if (nt > 0)
-        hipLaunchKernelGGL(k_gather<T>, dim3((nt + 255) / 256), dim3(256), 0, s, nt,
-                           (const double4 *)c->tpos.ptr, (const int *)c->tperm.ptr, g, (const T *)grid,
-                           u, accumulate ? 1 : 0);
+    if (nt > 0) launch_interp<T>(s, nt, g, c->tpos.ptr, c->tperm.ptr, (const T *)grid, u, accumulate ? 1 : 0);
     pb_mark(c, s, 6);
     return pb_hip_check(c, hipGetLastError(), "k-space launch");
 }
 
 } // namespace
 
-int pb_run_kspace(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_box &box, double a,
-                  double d, int ns, int nt, double *u, bool accumulate, int precision)
+int pb_run_kspace(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells,
+                  const pb_box &box, double a, double d, int ns, int nt, double *u, bool accumulate,
+                  int precision)
 {
     if (precision == PB_PREC_MIXED)
-        return run_kspace_t<float>(c, s, g, box, a, d, ns, nt, u, accumulate);
-    return run_kspace_t<double>(c, s, g, box, a, d, ns, nt, u, accumulate);
+        return run_kspace_t<float>(c, s, g, cells, box, a, d, ns, nt, u, accumulate);
+    return run_kspace_t<double>(c, s, g, cells, box, a, d, ns, nt, u, accumulate);
 }

@@ -101,6 +101,121 @@ void pb_fill_radial(pb_radial *r, double a, double d)
     }
 }
 
+// ---- radial table ------------------------------------------------------------
+namespace {
+
+// diffH(d) closed form (periodicBrinkman.h:95-104) in long double; used only
+// to build the table, on z >= z0 where the closed form loses at most a few
+// digits of the 64-bit mantissa.
+void diffH_ld(long double r, long double a, long double d, long double &H1, long double &H2)
+{
+    const long double pi = kPi, spi = sqrtl(kPi);
+    long double r2 = r * r, z = r / d, x = a * d / 2, s = a * r;
+    long double om = -expm1l(-x * x);
+    long double E = expl(-z * z) * om;
+    long double Erfc = erfcl(z);
+    long double tm = expl(-s) * erfcl(z - x);
+    long double tp = (z + x) < 27 ? expl(s) * erfcl(z + x) : 0.0L;
+    long double c3 = 1 / (8 * pi * a * a * r2 * r), c5 = c3 / r2;
+    H1 = -Erfc / (4 * pi * a * a * r2 * r) - (d * d + 2 * r2) / (2 * a * a * d * d * d * pi * spi * r2) * E +
+         c3 * (1 + s + s * s) * tm + c3 * (1 - s + s * s) * tp;
+    H2 = 3 * Erfc / (4 * pi * a * a * r2 * r2 * r) + (3 * d * d + 2 * r2) / (2 * a * a * d * d * d * pi * spi * r2 * r2) * E -
+         c5 * (3 + 3 * s + s * s) * tm - c5 * (3 - 3 * s + s * s) * tp;
+}
+
+// Chebyshev interpolation of f on [-1, 1] at the deg+1 Chebyshev points, then
+// conversion to monomial coefficients in t (well conditioned here: the
+// coefficients decay geometrically on the short intervals used).
+void cheb_to_monomial(const long double *cheb, int deg, long double *mono)
+{
+    long double Tprev[PB_TAB_MAXDEG + 1] = {0}, Tcur[PB_TAB_MAXDEG + 1] = {0}, Tnext[PB_TAB_MAXDEG + 1];
+    for (int k = 0; k <= deg; k++) mono[k] = 0;
+    Tprev[0] = 1; // T0
+    mono[0] += cheb[0];
+    if (deg >= 1) {
+        Tcur[1] = 1; // T1 = t
+        mono[1] += cheb[1];
+    }
+    for (int n = 2; n <= deg; n++) {
+        for (int k = 0; k <= deg; k++) Tnext[k] = -Tprev[k] + (k > 0 ? 2 * Tcur[k - 1] : 0);
+        for (int k = 0; k <= deg; k++) {
+            mono[k] += cheb[n] * Tnext[k];
+            Tprev[k] = Tcur[k];
+            Tcur[k] = Tnext[k];
+        }
+    }
+}
+
+} // namespace
+
+double pb_build_table(pb_kparams *k, double z0, double z1, std::vector<double> &coef)
+{
+    const long double a = k->pd.a, d = k->pd.d;
+    const int deg = PB_TAB_DEG;
+    double err = 1;
+    for (int per_unit = 8; per_unit <= 256; per_unit *= 2) {
+        int n = (int)std::ceil((z1 - z0) * per_unit);
+        if (n < 1) n = 1;
+        coef.assign((size_t)n * (deg + 1) * 2, 0.0);
+        double worst = 0;
+        for (int i = 0; i < n; i++) {
+            long double zl = z0 + (long double)i / per_unit, hw = 0.5L / per_unit, zc = zl + hw;
+            long double f1[PB_TAB_MAXDEG + 1], f2[PB_TAB_MAXDEG + 1];
+            for (int j = 0; j <= deg; j++) {
+                long double t = cosl(kPi * (j + 0.5L) / (deg + 1));
+                diffH_ld((zc + hw * t) * d, a, d, f1[j], f2[j]);
+            }
+            long double c1[PB_TAB_MAXDEG + 1], c2[PB_TAB_MAXDEG + 1], m1[PB_TAB_MAXDEG + 1], m2[PB_TAB_MAXDEG + 1];
+            for (int q = 0; q <= deg; q++) {
+                long double s1 = 0, s2 = 0;
+                for (int j = 0; j <= deg; j++) {
+                    long double cj = cosl(kPi * q * (j + 0.5L) / (deg + 1));
+                    s1 += f1[j] * cj;
+                    s2 += f2[j] * cj;
+                }
+                c1[q] = s1 * (q == 0 ? 1.0L : 2.0L) / (deg + 1);
+                c2[q] = s2 * (q == 0 ? 1.0L : 2.0L) / (deg + 1);
+            }
+            cheb_to_monomial(c1, deg, m1);
+            cheb_to_monomial(c2, deg, m2);
+            for (int q = 0; q <= deg; q++) {
+                coef[((size_t)i * (deg + 1) + q) * 2 + 0] = (double)m1[q];
+                coef[((size_t)i * (deg + 1) + q) * 2 + 1] = (double)m2[q];
+            }
+            // verify in double Horner (what the kernel does) against long double
+            long double mx1 = 0, mx2 = 0, e1 = 0, e2 = 0;
+            for (int j = 0; j <= 16; j++) {
+                double t = -1.0 + 2.0 * j / 16;
+                double h1 = 0, h2 = 0;
+                for (int q = deg; q >= 0; q--) {
+                    h1 = h1 * t + coef[((size_t)i * (deg + 1) + q) * 2 + 0];
+                    h2 = h2 * t + coef[((size_t)i * (deg + 1) + q) * 2 + 1];
+                }
+                long double g1, g2;
+                diffH_ld((zc + hw * t) * d, a, d, g1, g2);
+                mx1 = fmaxl(mx1, fabsl(g1));
+                mx2 = fmaxl(mx2, fabsl(g2));
+                e1 = fmaxl(e1, fabsl(h1 - g1));
+                e2 = fmaxl(e2, fabsl(h2 - g2));
+            }
+            double ei = (double)fmaxl(mx1 > 0 ? e1 / mx1 : 0, mx2 > 0 ? e2 / mx2 : 0);
+            worst = std::fmax(worst, ei);
+        }
+        err = worst;
+        if ((size_t)n * (deg + 1) * 2 > PB_TAB_LDS_DOUBLES) break; // too large: no table
+        if (worst <= 4e-15) {
+            k->tz0 = z0;
+            k->tinv = per_unit;
+            k->tn = n;
+            k->tdeg = deg;
+            return err;
+        }
+    }
+    k->tn = 0;
+    coef.clear();
+    return err;
+}
+
 void pb_fill_kparams(pb_kparams *k, double a, double d, double e, double rfar)
 {
     std::memset(k, 0, sizeof(*k));
@@ -110,6 +225,8 @@ void pb_fill_kparams(pb_kparams *k, double a, double d, double e, double rfar)
     k->e8 = 8 * e;
     k->d3 = 3 * d;
     k->rfar2 = rfar * rfar;
+    k->rcommon = k->pd.zs * d;
+    if (k->has_e && 8 * e > k->rcommon) k->rcommon = 8 * e;
     k->pf.a = (float)a;
     k->pf.inv_d = (float)(1.0 / d);
     k->pf.x = (float)k->pd.x;

@@ -62,7 +62,8 @@ __device__ __forceinline__ void sing_H(double r, double a, double k4pa2, double 
 __device__ __forceinline__ void reg_series(double z, const pb_radial &P, double &H1, double &H2)
 {
     double z2 = z * z, h1 = 0, h2 = 0;
-#pragma unroll
+    // kept rolled: unrolling batches all coefficient loads into registers
+#pragma unroll 2
     for (int m = PB_NSER - 1; m >= 0; m--) {
         h1 = fma(h1, z2, P.c1[m]);
         h2 = fma(h2, z2, P.c2[m]);
@@ -141,6 +142,10 @@ __device__ __forceinline__ void diff_H(double r, const pb_radial &P, double &H1,
 }
 
 // Radial part of K_R = diffH(d) - diffH(e) (periodicBrinkman.h:152-168).
+// Regions (the same as the oracle's):
+//   e > 0, r < 8e and r < 3d : K = regH(e) - regH(d)  (singular parts cancel)
+//   otherwise                : K = diffH(d) [- diffH(e) when e > 0 and r < 8e]
+// with regH from its series for r/w < zs and diffH = singH - regH-series there.
 __device__ __forceinline__ void vR_H(double r, const pb_kparams &K, double &H1, double &H2)
 {
     if (K.has_e && r < K.e8 && r < K.d3) {
@@ -158,6 +163,61 @@ __device__ __forceinline__ void vR_H(double r, const pb_kparams &K, double &H1, 
         H1 -= t1;
         H2 -= t2;
     }
+}
+
+// The far region r >= K.rcommon (the closed form of diffH(d) only) is the
+// common case; it skips the near-field region tests.
+__device__ __forceinline__ void vR_H_split(double r, const pb_kparams &K, double &H1, double &H2)
+{
+    if (r >= K.rcommon)
+        diff_direct(r, r * K.pd.inv_d, K.pd, H1, H2);
+    else
+        vR_H(r, K, H1, H2);
+}
+
+// All regions, not inlined: kernels whose common path is the table call this
+// for the rare near field so its library constants stay out of their loops.
+__device__ __noinline__ void vR_H_call(double r, const pb_kparams *K, double *H)
+{
+    vR_H(r, *K, H[0], H[1]);
+}
+
+// Table evaluation of diffH(d) for r >= K.rcommon: interval i of z = r/d and
+// Horner in t on the interleaved (H1, H2) coefficients held in LDS.
+__device__ __forceinline__ void table_H(double r, const pb_kparams &K, const double2 *__restrict__ tab,
+                                        double &H1, double &H2)
+{
+    const double u = (r * K.pd.inv_d - K.tz0) * K.tinv;
+    int i = (int)u;
+    i = i < K.tn ? i : K.tn - 1;
+    const double t = 2.0 * (u - (double)i) - 1.0;
+    const double2 *c = tab + i * (PB_TAB_DEG + 1);
+    double h1 = c[PB_TAB_DEG].x, h2 = c[PB_TAB_DEG].y;
+#pragma unroll
+    for (int k = PB_TAB_DEG - 1; k >= 0; k--) {
+        h1 = fma(h1, t, c[k].x);
+        h2 = fma(h2, t, c[k].y);
+    }
+    H1 = h1;
+    H2 = h2;
+}
+
+__device__ __forceinline__ void table_H32(float r, const pb_kparams &K, const float2 *__restrict__ tab,
+                                          float &H1, float &H2)
+{
+    const float u = (r * (float)K.pd.inv_d - (float)K.tz0) * (float)K.tinv;
+    int i = (int)u;
+    i = i < K.tn ? i : K.tn - 1;
+    const float t = 2.0f * (u - (float)i) - 1.0f;
+    const float2 *c = tab + i * (PB_TAB_DEG + 1);
+    float h1 = c[PB_TAB_DEG].x, h2 = c[PB_TAB_DEG].y;
+#pragma unroll
+    for (int k = PB_TAB_DEG - 1; k >= 0; k--) {
+        h1 = fmaf(h1, t, c[k].x);
+        h2 = fmaf(h2, t, c[k].y);
+    }
+    H1 = h1;
+    H2 = h2;
 }
 
 // FP32 far field, r >= 3d and r >= 8e: only diffH(d) contributes.

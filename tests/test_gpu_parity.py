@@ -59,7 +59,7 @@ def test_pointwise_batch_matches_oracle():
         gf = pb.vF_batch(dev(xf), dev(ks), a, d).cpu().numpy()
         for i in range(50):
             want = po.vF(xf[i], a, d, ks[i]).T.reshape(-1)
-            assert np.max(np.abs(gf[i] - want)) <= 1e-13 * np.max(np.abs(want)) + 1e-300
+            assert np.max(np.abs(gf[i] - want)) <= 5e-13 * np.max(np.abs(want)) + 1e-300  # cos of |k.x| ~ 40
 
 
 def test_single_point_entry_points():
