@@ -43,7 +43,7 @@ constexpr int kSpreadWaves = 4;
 
 template <typename T> struct spread_lds {
     uint2 q[128];      // (source index, image code)
-    T w[64][16];       // wx[4], wy[4], wz[4], scale * f[3]
+    T w[64][17];       // wx[4], wy[4], wz[4], scale * f[3]; odd stride: no bank conflicts
 };
 
 template <typename T>
@@ -208,13 +208,18 @@ __global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, const doub
     __builtin_amdgcn_wave_barrier();
     const size_t fs = 2 * (size_t)g.ng;
     T a0 = 0, a1 = 0, a2 = 0;
-    for (int k = lane; k < P * P * P; k += 64) {
-        const int pz = k % P, py = (k / P) % P, px = k / (P * P);
-        const size_t off = 2 * (((size_t)ix[wave][0][px] * g.n[1] + ix[wave][1][py]) * g.n[2] + ix[wave][2][pz]);
-        const T wt = wts[wave][0][px] * wts[wave][1][py] * wts[wave][2][pz];
-        a0 += wt * grid[off];
-        a1 += wt * grid[fs + off];
-        a2 += wt * grid[2 * fs + off];
+    constexpr int kIter = (P * P * P + 63) / 64;
+#pragma unroll 9
+    for (int it = 0; it < kIter; it++) {
+        const int k = it * 64 + lane;
+        if (P * P * P % 64 == 0 || k < P * P * P) {
+            const int pz = k % P, py = (k / P) % P, px = k / (P * P);
+            const size_t off = 2 * (((size_t)ix[wave][0][px] * g.n[1] + ix[wave][1][py]) * g.n[2] + ix[wave][2][pz]);
+            const T wt = wts[wave][0][px] * wts[wave][1][py] * wts[wave][2][pz];
+            a0 += wt * grid[off];
+            a1 += wt * grid[fs + off];
+            a2 += wt * grid[2 * fs + off];
+        }
     }
     a0 = wave_sum(a0);
     a1 = wave_sum(a1);

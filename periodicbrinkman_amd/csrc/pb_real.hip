@@ -29,6 +29,15 @@ struct wave_lds {
     uint2 q32[kQ];
 };
 
+// Broadcast lane k's double to the whole wavefront (k uniform): v_readlane.
+__device__ __forceinline__ double lane_bcast(double v, int k)
+{
+    const unsigned long long b = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, k);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), k);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __device__ __forceinline__ unsigned lane_prefix(unsigned long long m)
 {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -160,9 +169,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_pair(
     const int colbase = (w0 * nc1 + w1) * nc2;
     const double rc2 = cells.rc2;
 
-    // iterator over the cells of this column (o2) and their sources (j)
-    int o2 = -cells.st[2] - 1, j = 0, je = 0;
-    double yy2 = 0;
+    // iterator over the cells of this column (o2) and their sources: sources
+    // are fetched 64 at a time, one per lane (coalesced), and broadcast to the
+    // wavefront one by one with readlane for the 64-target test
+    int o2 = -cells.st[2] - 1, j = 0, je = 0, k = 0, nk = 0;
+    double yy2 = 0, cx = 0, cy = 0, cz = 0;
     unsigned code = 0;
     bool done = false;
     int qh = 0, qn = 0, fh = 0, fn = 0;
@@ -170,36 +181,47 @@ __global__ __launch_bounds__(64 * kWaves) void k_pair(
     for (;;) {
         // fill: test candidates until a ring holds a full batch or the column ends
         while (!done && qn < 64 && (!MIXED || fn < 64)) {
-            if (j >= je) {
-                if (++o2 > cells.st[2]) {
-                    done = true;
-                    break;
+            if (k >= nk) {
+                if (j >= je) {
+                    if (++o2 > cells.st[2]) {
+                        done = true;
+                        break;
+                    }
+                    const int uu = c2 + o2, w2 = ((uu % nc2) + nc2) % nc2, l2 = (uu - w2) / nc2;
+                    j = cstart[colbase + w2];
+                    je = cstart[colbase + w2 + 1];
+                    yy2 = y2 - l2 * box.L[2];
+                    code = code01 | ((unsigned)(l2 + 4) << 12);
+                    continue;
                 }
-                const int uu = c2 + o2, w2 = ((uu % nc2) + nc2) % nc2, l2 = (uu - w2) / nc2;
-                j = cstart[colbase + w2];
-                je = cstart[colbase + w2 + 1];
-                yy2 = y2 - l2 * box.L[2];
-                code = code01 | ((unsigned)(l2 + 4) << 12);
-                continue;
+                nk = min(64, je - j);
+                if (lane < nk) {
+                    const double4 xs = spos[j + lane];
+                    cx = xs.x;
+                    cy = xs.y;
+                    cz = xs.z;
+                }
+                k = 0;
             }
-            const double4 xs = spos[j];
-            const double d0 = yy0 - xs.x, d1 = yy1 - xs.y, d2 = yy2 - xs.z;
+            const double sx = lane_bcast(cx, k), sy = lane_bcast(cy, k), sz = lane_bcast(cz, k);
+            const double d0 = yy0 - sx, d1 = yy1 - sy, d2 = yy2 - sz;
             const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
             const bool in = active && r2 < rc2;
+            const unsigned jk = (unsigned)(j + k);
             if (!MIXED) {
                 const unsigned long long m = __ballot(in);
-                if (in) W.q64[(qh + qn + lane_prefix(m)) & (kQ - 1)] = make_uint2((unsigned)j, code);
+                if (in) W.q64[(qh + qn + lane_prefix(m)) & (kQ - 1)] = make_uint2(jk, code);
                 qn += __popcll(m);
             } else {
                 const bool far = in && r2 >= kp.rfar2;
                 const bool near = in && !far;
                 const unsigned long long mn = __ballot(near), mf = __ballot(far);
-                if (near) W.q64[(qh + qn + lane_prefix(mn)) & (kQ - 1)] = make_uint2((unsigned)j, code);
-                if (far) W.q32[(fh + fn + lane_prefix(mf)) & (kQ - 1)] = make_uint2((unsigned)j, code);
+                if (near) W.q64[(qh + qn + lane_prefix(mn)) & (kQ - 1)] = make_uint2(jk, code);
+                if (far) W.q32[(fh + fn + lane_prefix(mf)) & (kQ - 1)] = make_uint2(jk, code);
                 qn += __popcll(mn);
                 fn += __popcll(mf);
             }
-            j++;
+            if (++k >= nk) j += nk;
         }
         // evaluate: one batch per ring that is full (or, at the end, non-empty)
         const int cnt = qn >= 64 ? 64 : (done ? qn : 0);
