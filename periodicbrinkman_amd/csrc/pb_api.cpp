@@ -104,20 +104,24 @@ int check_params(pb_context *c, const pb_params *p)
 int make_cells(pb_context *c, const pb_params *p, pb_cells *cells)
 {
     std::memset(cells, 0, sizeof(*cells));
-    long ncell = 1;
+    long ncell = 1, nbrick = 1;
     for (int i = 0; i < 3; i++) {
-        int n = (int)std::floor(p->L[i] / p->rc);
+        int n = (int)std::floor(3.0 * p->L[i] / p->rc);
         if (n < 1) n = 1;
-        if (n > 96) n = 96;
+        if (n > 160) n = 160;
         cells->nc[i] = n;
+        cells->nb[i] = (n + 3) / 4;
         cells->h[i] = p->L[i] / n;
         cells->inv_h[i] = n / p->L[i];
         cells->st[i] = (int)std::ceil(p->rc / cells->h[i]);
-        if (cells->st[i] > 3)
+        if (p->rc > 3 * p->L[i])
             return pb_fail(c, PB_ERR_UNSUPPORTED, "cutoff rc larger than 3 box lengths is not supported");
         ncell *= n;
+        nbrick *= cells->nb[i];
     }
     cells->ncell = (int)ncell;
+    cells->nkey_t = (int)(nbrick * 64);
+    cells->rc = p->rc;
     cells->rc2 = p->rc * p->rc;
     return PB_OK;
 }
@@ -349,7 +353,7 @@ int pb_destroy(pb_context *c)
     (void)hipSetDevice(c->device);
     pb_buffer *bufs[] = {&c->cell_count, &c->cell_start, &c->cell_of_s, &c->rank_s, &c->cell_of_t,
                          &c->rank_t, &c->tcell_count, &c->tcell_start, &c->spos, &c->sfrc, &c->tpos,
-                         &c->tperm, &c->tiles, &c->tile_off, &c->ntiles, &c->counters, &c->grid,
+                         &c->tperm, &c->counters, &c->scan_tmp, &c->grid,
                          &c->twiddle, &c->axis_fac, &c->dense_tmp, &c->kparams, &c->table};
     for (pb_buffer *b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
@@ -552,6 +556,8 @@ int fftBrinkman(double *U, const double *y, int m, const double *f, const double
         cells.st[i] = 1;
     }
     cells.ncell = 1;
+    cells.nb[0] = cells.nb[1] = cells.nb[2] = 1;
+    cells.nkey_t = 64;
     if ((st = pb_bin_points(c, nullptr, cells, box, N, dx, df, true))) return st;
     if ((st = pb_bin_points(c, nullptr, cells, box, m, dy, nullptr, false))) return st;
     pb_kgrid g;

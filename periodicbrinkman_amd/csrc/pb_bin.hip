@@ -1,5 +1,10 @@
-// pb_bin.hip -- cell binning of sources and targets (counting sort) and the
-// target tiles of the real-space pass.
+// pb_bin.hip -- cell binning of sources and targets (counting sort).
+//
+// Both use the same grid of fine cells (side >= rc/3).  Sources are ordered
+// z-major, so the cells of one (x, y) column are one contiguous run of the
+// sorted array.  Targets are ordered by a brick-Morton key (bricks of 4x4x4
+// cells in z-major order, Morton order inside a brick), so every 64
+// consecutive sorted targets form a compact tile for the pair kernel.
 //
 // Positions are wrapped into [0, L) with the same operation sequence as the
 // oracle (v - floor(v/L)*L, no contraction) so both place a point on a cell
@@ -16,8 +21,14 @@ __device__ __forceinline__ double wrap_coord(double v, double L)
     return w;
 }
 
+__device__ __forceinline__ int morton3_2bit(int x, int y, int z)
+{
+    // interleave the low 2 bits of x, y, z: x1 y1 z1 x0 y0 z0
+    return ((x & 2) << 4) | ((y & 2) << 3) | ((z & 2) << 2) | ((x & 1) << 2) | ((y & 1) << 1) | (z & 1);
+}
+
 __global__ void k_bin_count(int n, const double *__restrict__ pos, pb_box box, pb_cells cells,
-                            int *__restrict__ count, int *__restrict__ cell_of,
+                            int brick_order, int *__restrict__ count, int *__restrict__ key_of,
                             int *__restrict__ rank)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -29,20 +40,26 @@ __global__ void k_bin_count(int n, const double *__restrict__ pos, pb_box box, p
         int k = (int)(w * cells.inv_h[c]);
         ci[c] = k >= cells.nc[c] ? cells.nc[c] - 1 : k;
     }
-    int cell = (ci[0] * cells.nc[1] + ci[1]) * cells.nc[2] + ci[2];
-    cell_of[i] = cell;
-    rank[i] = atomicAdd(&count[cell], 1);
+    int key;
+    if (brick_order) {
+        int brick = ((ci[0] >> 2) * cells.nb[1] + (ci[1] >> 2)) * cells.nb[2] + (ci[2] >> 2);
+        key = brick * 64 + morton3_2bit(ci[0], ci[1], ci[2]);
+    } else {
+        key = (ci[0] * cells.nc[1] + ci[1]) * cells.nc[2] + ci[2];
+    }
+    key_of[i] = key;
+    rank[i] = atomicAdd(&count[key], 1);
 }
 
 __global__ void k_bin_scatter(int n, const double *__restrict__ pos, const double *__restrict__ frc,
                               pb_box box, const int *__restrict__ start,
-                              const int *__restrict__ cell_of, const int *__restrict__ rank,
+                              const int *__restrict__ key_of, const int *__restrict__ rank,
                               double4 *__restrict__ opos, double4 *__restrict__ ofrc,
                               int *__restrict__ operm)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int dst = start[cell_of[i]] + rank[i];
+    int dst = start[key_of[i]] + rank[i];
     double w[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) w[c] = wrap_coord(pos[3 * (size_t)i + c], box.L[c]);
@@ -51,54 +68,77 @@ __global__ void k_bin_scatter(int n, const double *__restrict__ pos, const doubl
     if (operm) operm[dst] = i;
 }
 
-// Exclusive scan of n ints in one workgroup of 1024 threads; out[n] = total.
-__global__ __launch_bounds__(1024) void k_scan(int n, const int *__restrict__ in,
-                                                int *__restrict__ out)
+// Exclusive scan of n ints, out[n] = total, in three passes: per-block sums of
+// 1024 elements, a one-workgroup scan of those, then per-block scans with the
+// block offsets added.
+constexpr int kScanBlock = 1024;
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int *sh, int &total)
 {
-    __shared__ int part[1024];
-    int t = threadIdx.x;
-    int chunk = (n + 1023) / 1024;
-    int lo = t * chunk, hi = min(n, lo + chunk);
-    int s = 0;
-    for (int i = lo; i < hi; i++) s += in[i];
-    part[t] = s;
+    const int t = threadIdx.x;
+    sh[t] = v;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int v = t >= off ? part[t - off] : 0;
+    for (int off = 1; off < 256; off <<= 1) {
+        int w = t >= off ? sh[t - off] : 0;
         __syncthreads();
-        part[t] += v;
+        sh[t] += w;
         __syncthreads();
     }
-    int run = part[t] - s;
+    total = sh[255];
+    int ex = sh[t] - v;
+    __syncthreads();
+    return ex;
+}
+
+__global__ __launch_bounds__(256) void k_scan_sums(int n, const int *__restrict__ in, int *__restrict__ sums)
+{
+    __shared__ int sh[256];
+    const int base = blockIdx.x * kScanBlock + threadIdx.x * 4;
+    int v = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) v += base + q < n ? in[base + q] : 0;
+    int tot;
+    block_exclusive_scan(v, sh, tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_scan_top(int nb, int *__restrict__ sums)
+{
+    // exclusive scan of nb <= 256 * 16 block sums in place
+    __shared__ int sh[256];
+    const int per = (nb + 255) / 256, lo = threadIdx.x * per, hi = min(nb, lo + per);
+    int v = 0;
+    for (int i = lo; i < hi; i++) v += sums[i];
+    int tot;
+    int run = block_exclusive_scan(v, sh, tot);
     for (int i = lo; i < hi; i++) {
-        int v = in[i];
-        out[i] = run;
-        run += v;
+        int w = sums[i];
+        sums[i] = run;
+        run += w;
     }
-    if (t == 1023) out[n] = part[1023];
+    if (threadIdx.x == 0) sums[nb] = tot;
 }
 
-__global__ void k_tile_count(int ncell, const int *__restrict__ count, int *__restrict__ tcount)
+__global__ __launch_bounds__(256) void k_scan_apply(int n, const int *__restrict__ in,
+                                                    const int *__restrict__ sums, int nb,
+                                                    int *__restrict__ out)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < ncell) tcount[c] = (count[c] + 63) >> 6;
-}
-
-__global__ void k_make_tiles(int ncell, const int *__restrict__ count,
-                             const int *__restrict__ start, const int *__restrict__ toff,
-                             pb_tile *__restrict__ tiles)
-{
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    int n = count[c], first = start[c], o = toff[c];
-    for (int k = 0; k * 64 < n; k++) {
-        pb_tile t;
-        t.cell = c;
-        t.first = first + 64 * k;
-        t.count = min(64, n - 64 * k);
-        t.pad = 0;
-        tiles[o + k] = t;
+    __shared__ int sh[256];
+    const int base = blockIdx.x * kScanBlock + threadIdx.x * 4;
+    int v[4], s = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        v[q] = base + q < n ? in[base + q] : 0;
+        s += v[q];
     }
+    int tot;
+    int run = block_exclusive_scan(s, sh, tot) + sums[blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (base + q < n) out[base + q] = run;
+        run += v[q];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = sums[nb];
 }
 
 } // namespace
@@ -106,17 +146,17 @@ __global__ void k_make_tiles(int ncell, const int *__restrict__ count,
 int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_box &box, int n,
                   const double *pos, const double *frc, bool is_source)
 {
-    int nc = cells.ncell;
+    const int nkey = is_source ? cells.ncell : cells.nkey_t;
     pb_buffer &count = is_source ? c->cell_count : c->tcell_count;
     pb_buffer &start = is_source ? c->cell_start : c->tcell_start;
-    pb_buffer &cell_of = is_source ? c->cell_of_s : c->cell_of_t;
+    pb_buffer &key_of = is_source ? c->cell_of_s : c->cell_of_t;
     pb_buffer &rank = is_source ? c->rank_s : c->rank_t;
     pb_buffer &opos = is_source ? c->spos : c->tpos;
     int st;
     size_t nn = (size_t)(n > 0 ? n : 1);
-    if ((st = pb_reserve(c, &count, sizeof(int) * (nc + 1)))) return st;
-    if ((st = pb_reserve(c, &start, sizeof(int) * (nc + 1)))) return st;
-    if ((st = pb_reserve(c, &cell_of, sizeof(int) * nn))) return st;
+    if ((st = pb_reserve(c, &count, sizeof(int) * (nkey + 1)))) return st;
+    if ((st = pb_reserve(c, &start, sizeof(int) * (nkey + 1)))) return st;
+    if ((st = pb_reserve(c, &key_of, sizeof(int) * nn))) return st;
     if ((st = pb_reserve(c, &rank, sizeof(int) * nn))) return st;
     if ((st = pb_reserve(c, &opos, sizeof(double4) * nn))) return st;
     if (is_source) {
@@ -124,36 +164,30 @@ int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_
     } else {
         if ((st = pb_reserve(c, &c->tperm, sizeof(int) * nn))) return st;
     }
-    if ((st = pb_hip_check(c, hipMemsetAsync(count.ptr, 0, sizeof(int) * (nc + 1), s), "memset")))
+    if ((st = pb_hip_check(c, hipMemsetAsync(count.ptr, 0, sizeof(int) * (nkey + 1), s), "memset")))
         return st;
     if (n > 0) {
         int blocks = (n + 255) / 256;
         hipLaunchKernelGGL(k_bin_count, dim3(blocks), dim3(256), 0, s, n, pos, box, cells,
-                           (int *)count.ptr, (int *)cell_of.ptr, (int *)rank.ptr);
+                           is_source ? 0 : 1, (int *)count.ptr, (int *)key_of.ptr, (int *)rank.ptr);
     }
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, nc, (const int *)count.ptr,
-                       (int *)start.ptr);
+    {
+        const int nb = (nkey + kScanBlock - 1) / kScanBlock;
+        if (nb > 256 * 16) return pb_fail(c, PB_ERR_UNSUPPORTED, "too many cells for the scan");
+        if ((st = pb_reserve(c, &c->scan_tmp, sizeof(int) * (nb + 1)))) return st;
+        int *sums = (int *)c->scan_tmp.ptr;
+        hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(256), 0, s, nkey, (const int *)count.ptr, sums);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, nb, sums);
+        hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, nkey, (const int *)count.ptr,
+                           (const int *)sums, nb, (int *)start.ptr);
+    }
     if (n > 0) {
         int blocks = (n + 255) / 256;
         hipLaunchKernelGGL(k_bin_scatter, dim3(blocks), dim3(256), 0, s, n, pos, frc, box,
-                           (const int *)start.ptr, (const int *)cell_of.ptr,
+                           (const int *)start.ptr, (const int *)key_of.ptr,
                            (const int *)rank.ptr, (double4 *)opos.ptr,
                            is_source ? (double4 *)c->sfrc.ptr : nullptr,
                            is_source ? nullptr : (int *)c->tperm.ptr);
-    }
-    if (!is_source) {
-        // tiles of <= 64 targets per cell
-        if ((st = pb_reserve(c, &c->tile_off, sizeof(int) * (nc + 1)))) return st;
-        size_t max_tiles = (size_t)nc + nn / 64 + 1;
-        if ((st = pb_reserve(c, &c->tiles, sizeof(pb_tile) * max_tiles))) return st;
-        if ((st = pb_reserve(c, &c->ntiles, sizeof(int) * (nc + 1)))) return st;
-        hipLaunchKernelGGL(k_tile_count, dim3((nc + 255) / 256), dim3(256), 0, s, nc,
-                           (const int *)count.ptr, (int *)c->ntiles.ptr);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, nc, (const int *)c->ntiles.ptr,
-                           (int *)c->tile_off.ptr);
-        hipLaunchKernelGGL(k_make_tiles, dim3((nc + 255) / 256), dim3(256), 0, s, nc,
-                           (const int *)count.ptr, (const int *)start.ptr,
-                           (const int *)c->tile_off.ptr, (pb_tile *)c->tiles.ptr);
     }
     return pb_hip_check(c, hipGetLastError(), "binning launch");
 }

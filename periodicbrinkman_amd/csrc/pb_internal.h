@@ -3,10 +3,10 @@
 // Device data layout (HBM):
 //   sorted sources   spos[j] = (x, y, z, 0) wrapped into [0, L), double4
 //                    sfrc[j] = (fx, fy, fz, 0), double4
-//   sorted targets   tpos[t] = (x, y, z, 0), tperm[t] = original index
-//   cells            cell_start[c] .. cell_start[c+1] index the sorted arrays;
-//                    c = (c0 * nc1 + c1) * nc2 + c2
-//   target tiles     up to 64 targets of one cell: (cell, first, count)
+//                    cell_start[c] .. cell_start[c+1]: sources of cell
+//                    c = (c0 * nc1 + c1) * nc2 + c2 (z-major)
+//   sorted targets   tpos[t] = (x, y, z, 0), tperm[t] = original index, in
+//                    brick-Morton order: targets 64k .. 64k+63 are tile k
 //   k-space grids    3 complex fields, element (i0, i1, i2) at (i0*n1 + i1)*n2 + i2
 #pragma once
 
@@ -73,18 +73,18 @@ struct pb_box {
     double inv_L[3];
 };
 
-// Cell list geometry for the real-space pass.
+// Cell grid shared by the real-space pass and spreading: fine cells of side
+// >= rc/3.  Sources are sorted z-major (key = (c0*nc1 + c1)*nc2 + c2), targets
+// by brick-Morton key (brick of 4x4x4 cells, Morton order inside).
 struct pb_cells {
     int nc[3];      // cells per axis
-    int ncell;      // nc0 * nc1 * nc2
-    int st[3];      // stencil half-width per axis
+    int ncell;      // nc0 * nc1 * nc2 (source keys)
+    int nb[3];      // bricks per axis, ceil(nc/4)
+    int nkey_t;     // nb0 * nb1 * nb2 * 64 (target keys)
+    int st[3];      // cells per rc, rounded up (stencil half-width)
     double h[3];    // cell size
     double inv_h[3];
-    double rc2;     // cutoff squared
-};
-
-struct pb_tile {
-    int cell, first, count, pad;
+    double rc, rc2; // cutoff
 };
 
 // k-space geometry.
@@ -119,7 +119,7 @@ struct pb_context {
     // real space
     pb_buffer cell_count, cell_start, cell_of_s, rank_s, cell_of_t, rank_t, tcell_count,
         tcell_start;
-    pb_buffer spos, sfrc, tpos, tperm, tiles, tile_off, ntiles, counters;
+    pb_buffer spos, sfrc, tpos, tperm, counters, scan_tmp;
     // k space
     pb_buffer grid, twiddle, axis_fac;
     int twiddle_n = 0;

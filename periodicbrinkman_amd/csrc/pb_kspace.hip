@@ -22,6 +22,12 @@ template <typename T> struct cplx;
 template <> struct cplx<double> { using type = double2; };
 template <> struct cplx<float> { using type = float2; };
 
+__device__ __forceinline__ int floordiv(int a, int b)
+{
+    int q = a / b;
+    return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
+}
+
 __device__ __forceinline__ int wrapi(int i, int n)
 {
     int r = i % n;
@@ -116,13 +122,18 @@ __global__ __launch_bounds__(64 * kSpreadWaves) void k_spread_blocks(
     T a0 = 0, a1 = 0, a2 = 0;
     int qh = 0, qn = 0;
     for (int u0 = ulo[0]; u0 <= uhi[0]; u0++) {
-        const int w0 = ((u0 % cells.nc[0]) + cells.nc[0]) % cells.nc[0], l0 = (u0 - w0) / cells.nc[0];
+        const int l0 = floordiv(u0, cells.nc[0]), w0 = u0 - l0 * cells.nc[0];
         for (int u1 = ulo[1]; u1 <= uhi[1]; u1++) {
-            const int w1 = ((u1 % cells.nc[1]) + cells.nc[1]) % cells.nc[1], l1 = (u1 - w1) / cells.nc[1];
-            for (int u2 = ulo[2]; u2 <= uhi[2]; u2++) {
-                const int w2 = ((u2 % cells.nc[2]) + cells.nc[2]) % cells.nc[2], l2 = (u2 - w2) / cells.nc[2];
-                const int cell = (w0 * cells.nc[1] + w1) * cells.nc[2] + w2;
-                const int js = cstart[cell], je = cstart[cell + 1];
+            const int l1 = floordiv(u1, cells.nc[1]), w1 = u1 - l1 * cells.nc[1];
+            const int colbase = (w0 * cells.nc[1] + w1) * cells.nc[2];
+            // the z range splits into runs of one image shift, each a contiguous
+            // run of the z-major sorted sources
+            for (int zs = ulo[2]; zs <= uhi[2];) {
+                const int l2 = floordiv(zs, cells.nc[2]);
+                const int zend = min(uhi[2], (l2 + 1) * cells.nc[2] - 1);
+                const int js = cstart[colbase + zs - l2 * cells.nc[2]];
+                const int je = cstart[colbase + zend - l2 * cells.nc[2] + 1];
+                zs = zend + 1;
                 const unsigned code = (unsigned)(l0 + 4) | ((unsigned)(l1 + 4) << 3) | ((unsigned)(l2 + 4) << 6);
                 for (int j0 = js; j0 < je; j0 += 64) {
                     const int j = j0 + lane;

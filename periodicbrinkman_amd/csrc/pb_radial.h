@@ -89,9 +89,21 @@ __device__ __forceinline__ void reg_direct(double r, double z, const pb_radial &
          + k8 * ir3 * ir2 * ((3.0 - 3.0 * s + s2) * Ap - (3.0 + 3.0 * s + s2) * Am);
 }
 
-// diffH closed form (periodicBrinkman.h:95-104) in erfcx form.
-__device__ __forceinline__ void diff_direct(double r, double z, const pb_radial &P, double &H1,
-                                            double &H2)
+// The scalars of pb_radial without its series tables: kernels copy these into
+// registers once instead of re-reading the constant struct in their loops.
+struct radial_core {
+    double a, d, inv_d, x, ex2, k4pa2, kg;
+};
+
+__device__ __forceinline__ radial_core core_of(const pb_radial &P)
+{
+    return radial_core{P.a, P.d, P.inv_d, P.x, P.ex2, P.k4pa2, P.kg};
+}
+
+// diffH closed form (periodicBrinkman.h:95-104) in erfcx form.  R is
+// pb_radial or radial_core.
+template <class R>
+__device__ __forceinline__ void diff_direct(double r, double z, const R &P, double &H1, double &H2)
 {
     double s = P.a * r, s2 = s * s, x = P.x;
     double E = exp(-z * z);
@@ -165,16 +177,6 @@ __device__ __forceinline__ void vR_H(double r, const pb_kparams &K, double &H1, 
     }
 }
 
-// The far region r >= K.rcommon (the closed form of diffH(d) only) is the
-// common case; it skips the near-field region tests.
-__device__ __forceinline__ void vR_H_split(double r, const pb_kparams &K, double &H1, double &H2)
-{
-    if (r >= K.rcommon)
-        diff_direct(r, r * K.pd.inv_d, K.pd, H1, H2);
-    else
-        vR_H(r, K, H1, H2);
-}
-
 // All regions, not inlined: kernels whose common path is the table call this
 // for the rare near field so its library constants stay out of their loops.
 __device__ __noinline__ void vR_H_call(double r, const pb_kparams *K, double *H)
@@ -182,14 +184,25 @@ __device__ __noinline__ void vR_H_call(double r, const pb_kparams *K, double *H)
     vR_H(r, *K, H[0], H[1]);
 }
 
+// Where the table of diffH(d) lives: z = r/d in [tz0, tz0 + tn/tinv).
+struct tabview {
+    double inv_d, tz0, tinv;
+    int tn;
+};
+
+__device__ __forceinline__ tabview tabview_of(const pb_kparams &K)
+{
+    return tabview{K.pd.inv_d, K.tz0, K.tinv, K.tn};
+}
+
 // Table evaluation of diffH(d) for r >= K.rcommon: interval i of z = r/d and
 // Horner in t on the interleaved (H1, H2) coefficients held in LDS.
-__device__ __forceinline__ void table_H(double r, const pb_kparams &K, const double2 *__restrict__ tab,
+__device__ __forceinline__ void table_H(double r, const tabview &V, const double2 *__restrict__ tab,
                                         double &H1, double &H2)
 {
-    const double u = (r * K.pd.inv_d - K.tz0) * K.tinv;
+    const double u = (r * V.inv_d - V.tz0) * V.tinv;
     int i = (int)u;
-    i = i < K.tn ? i : K.tn - 1;
+    i = i < V.tn ? i : V.tn - 1;
     const double t = 2.0 * (u - (double)i) - 1.0;
     const double2 *c = tab + i * (PB_TAB_DEG + 1);
     double h1 = c[PB_TAB_DEG].x, h2 = c[PB_TAB_DEG].y;
@@ -202,12 +215,22 @@ __device__ __forceinline__ void table_H(double r, const pb_kparams &K, const dou
     H2 = h2;
 }
 
-__device__ __forceinline__ void table_H32(float r, const pb_kparams &K, const float2 *__restrict__ tab,
+struct tabview32 {
+    float inv_d, tz0, tinv;
+    int tn;
+};
+
+__device__ __forceinline__ tabview32 tabview32_of(const pb_kparams &K)
+{
+    return tabview32{(float)K.pd.inv_d, (float)K.tz0, (float)K.tinv, K.tn};
+}
+
+__device__ __forceinline__ void table_H32(float r, const tabview32 &V, const float2 *__restrict__ tab,
                                           float &H1, float &H2)
 {
-    const float u = (r * (float)K.pd.inv_d - (float)K.tz0) * (float)K.tinv;
+    const float u = (r * V.inv_d - V.tz0) * V.tinv;
     int i = (int)u;
-    i = i < K.tn ? i : K.tn - 1;
+    i = i < V.tn ? i : V.tn - 1;
     const float t = 2.0f * (u - (float)i) - 1.0f;
     const float2 *c = tab + i * (PB_TAB_DEG + 1);
     float h1 = c[PB_TAB_DEG].x, h2 = c[PB_TAB_DEG].y;

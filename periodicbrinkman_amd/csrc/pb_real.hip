@@ -5,44 +5,80 @@
 // (periodicBrinkman.h:152-176; the reference sums all pairs over image cubes,
 // realShells :564-693, kernelReal :219-314).
 //
-// One wavefront owns one target tile (<= 64 targets of one cell, one per
-// lane).  It walks the (2st+1)^3 neighbour cells; each source is loaded once
-// per wave (uniform address -> scalar loads) and tested against all 64
-// targets.  In-range (target, source, image) triples are compacted with a
-// ballot into a 128-entry LDS ring; whenever 64 are pending every lane
-// evaluates one of them, so the expensive exp/erfcx work runs with full
-// lanes regardless of how sparse the neighbourhood test is.  Results are
-// added into per-target FP64 accumulators in LDS (ds_add_f64).  In mixed
-// precision, pairs beyond rfar (>= 3d and >= 8e) go to a second ring and are
-// evaluated in FP32; their results are still accumulated in FP64.
+// Work item = (tile, split): a tile is 64 consecutive targets in brick-Morton
+// order (a compact cluster, one per lane); its bounding box B decides the
+// candidate sources: the (x, y) cell columns within rc of B, each trimmed in
+// z to sqrt(rc^2 - gap^2), which are contiguous runs of the z-major sorted
+// sources.  The columns are dealt round-robin to `nsplit` wavefronts, which
+// meet in u through FP64 atomics.  The lanes of a wavefront set up its columns
+// in parallel (culling, image shifts, first run of sources); the wavefront
+// then walks the runs one source at a time: the source is a uniform scalar
+// load and is tested against the 64 lane targets.
+//
+// In-range (target, source, image) triples are compacted with a ballot into
+// an LDS ring, one LDS write per tested source; each time 64 are pending
+// every lane evaluates one, so the evaluation runs with full lanes however
+// sparse the test is.  An evaluated batch moves its rare pairs with
+// r < rcommon (series and blob regions) to a third ring that is evaluated 64
+// at a time out of line.  Results go to per-target FP64 accumulators in LDS
+// (ds_add_f64).  In mixed precision, pairs beyond rfar (>= 3d and >= 8e) are
+// classified at the test into the second half of the ring and evaluated in
+// FP32, and still accumulated in FP64.
 #include "pb_radial.h"
 
 namespace {
 
-constexpr int kWaves = 4;  // wavefronts per workgroup, each independent
-constexpr int kQ = 128;    // ring capacity (power of two)
+constexpr int kWaves = 4; // wavefronts per workgroup, each independent
+constexpr int kQ = 128;   // ring capacity (power of two)
 
 struct wave_lds {
     double y[64][3];
     double acc[64][3];
-    uint2 q64[kQ];
-    uint2 q32[kQ];
+    double4 stage[64]; // candidate sources of the current run: image position, index
+    uint2 q[2 * kQ];   // [0, kQ): FP64 ring; [kQ, 2 kQ): FP32 far ring (mixed precision)
+    uint2 qr[kQ];      // r < rcommon, moved out of evaluated FP64 batches
 };
-
-// Broadcast lane k's double to the whole wavefront (k uniform): v_readlane.
-__device__ __forceinline__ double lane_bcast(double v, int k)
-{
-    const unsigned long long b = __double_as_longlong(v);
-    const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, k);
-    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), k);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
 
 __device__ __forceinline__ unsigned lane_prefix(unsigned long long m)
 {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
+// Uniform copy of a value every lane holds (kept in SGPRs).
+__device__ __forceinline__ double uniform(double v)
+{
+    const unsigned long long b = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ int lane_read(int v, int i)
+{
+    return __builtin_amdgcn_readlane(v, i);
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return uniform(v);
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return uniform(v);
+}
+
+__device__ __forceinline__ int floordiv(int a, int b)
+{
+    int q = a / b;
+    return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
+}
+
+// ring entry: x = source index, y = lane | (l0+4) << 6 | (l1+4) << 9 | (l2+4) << 12
 __device__ __forceinline__ void decode(const uint2 e, const pb_box &box, int &t, double &s0,
                                        double &s1, double &s2)
 {
@@ -52,79 +88,137 @@ __device__ __forceinline__ void decode(const uint2 e, const pb_box &box, int &t,
     s2 = (double)((int)((e.y >> 12) & 7) - 4) * box.L[2];
 }
 
-template <bool TABLE>
-__device__ __forceinline__ void eval64(wave_lds &W, const uint2 e, const double4 *__restrict__ spos,
-                                       const double4 *__restrict__ sfrc, const pb_box &box,
-                                       const pb_kparams &kp, const pb_kparams *kpp,
-                                       const double2 *__restrict__ tab)
+__device__ __forceinline__ void accumulate(wave_lds &W, int t, double H1, double H2, double d0,
+                                           double d1, double d2, const double4 &fs)
 {
-    int t;
-    double s0, s1, s2;
-    decode(e, box, t, s0, s1, s2);
-    double4 xs = spos[e.x];
-    double4 fs = sfrc[e.x];
-    double d0 = W.y[t][0] - (xs.x + s0);
-    double d1 = W.y[t][1] - (xs.y + s1);
-    double d2 = W.y[t][2] - (xs.z + s2);
-    double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-    double H1, H2;
-    if (TABLE) {
-        if (r >= kp.rcommon) {
-            pbdev::table_H(r, kp, tab, H1, H2);
-        } else {
-            double H[2];
-            pbdev::vR_H_call(r, kpp, H);
-            H1 = H[0];
-            H2 = H[1];
-        }
-    } else {
-        pbdev::vR_H_split(r, kp, H1, H2);
-    }
-    double g = H2 * (d0 * fs.x + d1 * fs.y + d2 * fs.z);
+    const double g = H2 * (d0 * fs.x + d1 * fs.y + d2 * fs.z);
     atomicAdd(&W.acc[t][0], H1 * fs.x + g * d0);
     atomicAdd(&W.acc[t][1], H1 * fs.y + g * d1);
     atomicAdd(&W.acc[t][2], H1 * fs.z + g * d2);
 }
 
+// FP64 batch of cnt entries from the ring at head.  Pairs with r < rcommon
+// go to the rare ring; the others take the common far region of K_R (only
+// diffH(d)), from the table or the closed form.
+// Kernel constants the batches use, copied into registers once.
+struct batch_consts {
+    pbdev::tabview tv;
+    pbdev::tabview32 tv32;
+    pbdev::radial_core core;
+    pb_radial32 pf;
+    double rcommon;
+};
+
 template <bool TABLE>
-__device__ __forceinline__ void eval32(wave_lds &W, const uint2 e, const double4 *__restrict__ spos,
-                                       const double4 *__restrict__ sfrc, const pb_box &box,
-                                       const pb_kparams &kp, const float2 *__restrict__ tab32)
+__device__ __forceinline__ void batch64(wave_lds &W, const int lane, const int cnt, const int head,
+                                        const int rh, int &rn, const double4 *__restrict__ spos,
+                                        const double4 *__restrict__ sfrc, const pb_box &box,
+                                        const batch_consts &K, const double2 *__restrict__ tab)
 {
-    int t;
-    double s0, s1, s2;
-    decode(e, box, t, s0, s1, s2);
-    double4 xs = spos[e.x];
-    double4 fs = sfrc[e.x];
-    float d0 = (float)(W.y[t][0] - (xs.x + s0));
-    float d1 = (float)(W.y[t][1] - (xs.y + s1));
-    float d2 = (float)(W.y[t][2] - (xs.z + s2));
-    float f0 = (float)fs.x, f1 = (float)fs.y, f2 = (float)fs.z;
-    float r = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
-    float H1, H2;
-    if (TABLE)
-        pbdev::table_H32(r, kp, tab32, H1, H2);
-    else
-        pbdev::diff_far32(r, kp.pf, H1, H2);
-    float g = H2 * (d0 * f0 + d1 * f1 + d2 * f2);
-    atomicAdd(&W.acc[t][0], (double)(H1 * f0 + g * d0));
-    atomicAdd(&W.acc[t][1], (double)(H1 * f1 + g * d1));
-    atomicAdd(&W.acc[t][2], (double)(H1 * f2 + g * d2));
+    __builtin_amdgcn_wave_barrier();
+    const bool on = lane < cnt;
+    uint2 e = make_uint2(0u, 0u);
+    int t = 0;
+    double d0 = 0, d1 = 0, d2 = 0, r = 0;
+    double4 fs = make_double4(0, 0, 0, 0);
+    if (on) {
+        e = W.q[(head + lane) & (kQ - 1)];
+        double s0, s1, s2;
+        decode(e, box, t, s0, s1, s2);
+        const double4 xs = spos[e.x];
+        fs = sfrc[e.x];
+        d0 = W.y[t][0] - (xs.x + s0);
+        d1 = W.y[t][1] - (xs.y + s1);
+        d2 = W.y[t][2] - (xs.z + s2);
+        r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    }
+    const bool rare = on && r < K.rcommon;
+    const unsigned long long mr = __ballot(rare);
+    if (mr) {
+        if (rare) W.qr[(rh + rn + lane_prefix(mr)) & (kQ - 1)] = e;
+        rn += __popcll(mr);
+    }
+    if (on && !rare) {
+        double H1, H2;
+        if (TABLE)
+            pbdev::table_H(r, K.tv, tab, H1, H2);
+        else
+            pbdev::diff_direct(r, r * K.core.inv_d, K.core, H1, H2);
+        accumulate(W, t, H1, H2, d0, d1, d2, fs);
+    }
+}
+
+// Rare batch: every region of K_R, through an out-of-line call so its library
+// constants stay out of the main loop.
+__device__ __forceinline__ void batch_rare(wave_lds &W, const int lane, const int cnt, const int head,
+                                           const double4 *__restrict__ spos,
+                                           const double4 *__restrict__ sfrc, const pb_box &box,
+                                           const pb_kparams *kpp)
+{
+    __builtin_amdgcn_wave_barrier();
+    if (lane < cnt) {
+        const uint2 e = W.qr[(head + lane) & (kQ - 1)];
+        int t;
+        double s0, s1, s2;
+        decode(e, box, t, s0, s1, s2);
+        const double4 xs = spos[e.x];
+        const double4 fs = sfrc[e.x];
+        const double d0 = W.y[t][0] - (xs.x + s0);
+        const double d1 = W.y[t][1] - (xs.y + s1);
+        const double d2 = W.y[t][2] - (xs.z + s2);
+        const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        double H[2];
+        pbdev::vR_H_call(r, kpp, H);
+        accumulate(W, t, H[0], H[1], d0, d1, d2, fs);
+    }
+}
+
+// FP32 batch of the far ring (r >= rfar: only diffH(d), in FP32).
+template <bool TABLE>
+__device__ __forceinline__ void batch32(wave_lds &W, const int lane, const int cnt, const int head,
+                                        const double4 *__restrict__ spos,
+                                        const double4 *__restrict__ sfrc, const pb_box &box,
+                                        const batch_consts &K, const float2 *__restrict__ tab32)
+{
+    __builtin_amdgcn_wave_barrier();
+    if (lane < cnt) {
+        const uint2 e = W.q[kQ + ((head + lane) & (kQ - 1))];
+        int t;
+        double s0, s1, s2;
+        decode(e, box, t, s0, s1, s2);
+        const double4 xs = spos[e.x];
+        const double4 fs = sfrc[e.x];
+        const float d0 = (float)(W.y[t][0] - (xs.x + s0));
+        const float d1 = (float)(W.y[t][1] - (xs.y + s1));
+        const float d2 = (float)(W.y[t][2] - (xs.z + s2));
+        const float f0 = (float)fs.x, f1 = (float)fs.y, f2 = (float)fs.z;
+        const float r = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+        float H1, H2;
+        if (TABLE)
+            pbdev::table_H32(r, K.tv32, tab32, H1, H2);
+        else
+            pbdev::diff_far32(r, K.pf, H1, H2);
+        const float g = H2 * (d0 * f0 + d1 * f1 + d2 * f2);
+        atomicAdd(&W.acc[t][0], (double)(H1 * f0 + g * d0));
+        atomicAdd(&W.acc[t][1], (double)(H1 * f1 + g * d1));
+        atomicAdd(&W.acc[t][2], (double)(H1 * f2 + g * d2));
+    }
 }
 
 template <bool MIXED, bool TABLE>
-__global__ __launch_bounds__(64 * kWaves) void k_pair(
-    const pb_tile *__restrict__ tiles, const int *__restrict__ ntiles_ptr,
+__device__ __forceinline__ void pair_body(
     const int *__restrict__ cstart, const double4 *__restrict__ spos,
     const double4 *__restrict__ sfrc, const double4 *__restrict__ tpos,
-    const int *__restrict__ tperm, pb_cells cells, pb_box box,
+    const int *__restrict__ tperm, int nt, int nsplit, pb_cells cells, pb_box box,
     const pb_kparams *__restrict__ kpp, double *__restrict__ u,
     unsigned long long *__restrict__ counters)
 {
     const pb_kparams &kp = *kpp;
+    const batch_consts K{pbdev::tabview_of(kp), pbdev::tabview32_of(kp), pbdev::core_of(kp.pd), kp.pf,
+                         kp.rcommon};
+    const double rfar2 = kp.rfar2;
     __shared__ wave_lds lds[kWaves];
     extern __shared__ double dyn_lds[]; // the radial table: FP64 then (mixed) FP32 copy
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ntab = TABLE ? kp.tn * (PB_TAB_DEG + 1) * 2 : 0;
     if (TABLE) { // every wavefront takes part before any may leave
         for (int k = threadIdx.x; k < ntab; k += blockDim.x) dyn_lds[k] = kp.tab64[k];
@@ -136,125 +230,193 @@ __global__ __launch_bounds__(64 * kWaves) void k_pair(
     }
     const double2 *tab = reinterpret_cast<const double2 *>(dyn_lds);
     const float2 *tab32 = reinterpret_cast<const float2 *>(dyn_lds + ntab);
-    // work item = (tile, column of the stencil): the (2st0+1)(2st1+1) columns
-    // of one tile run on different wavefronts and meet in u through atomics
-    const int ncol = (2 * cells.st[0] + 1) * (2 * cells.st[1] + 1);
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long long item = (long long)blockIdx.x * kWaves + wave;
-    const int tile_id = (int)(item / ncol), col = (int)(item % ncol);
-    if (tile_id >= *ntiles_ptr) return; // whole wavefront leaves; no block barriers below
+    const int tile = (int)(item / nsplit), split = (int)(item % nsplit);
+    const int t0 = tile * 64;
+    if (t0 >= nt) return; // whole wavefront leaves; no block barriers below
     wave_lds &W = lds[wave];
-    const pb_tile T = tiles[tile_id];
-    const bool active = lane < T.count;
-    double y0 = 0, y1 = 0, y2 = 0;
-    if (active) {
-        double4 p = tpos[T.first + lane];
-        y0 = p.x;
-        y1 = p.y;
-        y2 = p.z;
-    }
-    W.y[lane][0] = y0;
-    W.y[lane][1] = y1;
-    W.y[lane][2] = y2;
+    const bool active = t0 + lane < nt;
+    const double4 p = tpos[active ? t0 + lane : t0];
+    W.y[lane][0] = p.x;
+    W.y[lane][1] = p.y;
+    W.y[lane][2] = p.z;
     W.acc[lane][0] = W.acc[lane][1] = W.acc[lane][2] = 0.0;
+    // bounding box of the tile (inactive lanes repeat the first target)
+    const double bmin0 = wave_min(p.x), bmin1 = wave_min(p.y), bmin2 = wave_min(p.z);
+    const double bmax0 = wave_max(p.x), bmax1 = wave_max(p.y), bmax2 = wave_max(p.z);
+    // inactive lanes are parked far away, so the distance test never passes
+    const double y0 = active ? p.x : 1e30, y1 = active ? p.y : 1e30, y2 = active ? p.z : 1e30;
     __builtin_amdgcn_wave_barrier();
 
     const int nc0 = cells.nc[0], nc1 = cells.nc[1], nc2 = cells.nc[2];
-    const int c2 = T.cell % nc2, c1 = (T.cell / nc2) % nc1, c0 = T.cell / (nc1 * nc2);
-    const int o0 = col / (2 * cells.st[1] + 1) - cells.st[0];
-    const int o1 = col % (2 * cells.st[1] + 1) - cells.st[1];
-    const int u0 = c0 + o0, w0 = ((u0 % nc0) + nc0) % nc0, l0 = (u0 - w0) / nc0;
-    const int u1 = c1 + o1, w1 = ((u1 % nc1) + nc1) % nc1, l1 = (u1 - w1) / nc1;
-    const double yy0 = y0 - l0 * box.L[0], yy1 = y1 - l1 * box.L[1];
-    const unsigned code01 = (unsigned)lane | ((unsigned)(l0 + 4) << 6) | ((unsigned)(l1 + 4) << 9);
-    const int colbase = (w0 * nc1 + w1) * nc2;
+    const double rcx = cells.rc * (1.0 + 1e-12); // culling is conservative; the test is exact
+    const double rcx2 = rcx * rcx;
+    const int ulo0 = (int)floor((bmin0 - rcx) * cells.inv_h[0]), uhi0 = (int)floor((bmax0 + rcx) * cells.inv_h[0]);
+    const int ulo1 = (int)floor((bmin1 - rcx) * cells.inv_h[1]), uhi1 = (int)floor((bmax1 + rcx) * cells.inv_h[1]);
+    const int ny = uhi1 - ulo1 + 1;
+    const int ncols = (uhi0 - ulo0 + 1) * ny;
     const double rc2 = cells.rc2;
 
-    // iterator over the cells of this column (o2) and their sources: sources
-    // are fetched 64 at a time, one per lane (coalesced), and broadcast to the
-    // wavefront one by one with readlane for the 64-target test
-    int o2 = -cells.st[2] - 1, j = 0, je = 0, k = 0, nk = 0;
-    double yy2 = 0, cx = 0, cy = 0, cz = 0;
-    unsigned code = 0;
-    bool done = false;
-    int qh = 0, qn = 0, fh = 0, fn = 0;
+    int qh = 0, qn = 0, fh = 0, fn = 0, rh = 0, rn = 0;
     unsigned long long n64 = 0, n32 = 0;
-    for (;;) {
-        // fill: test candidates until a ring holds a full batch or the column ends
-        while (!done && qn < 64 && (!MIXED || fn < 64)) {
-            if (k >= nk) {
-                if (j >= je) {
-                    if (++o2 > cells.st[2]) {
-                        done = true;
-                        break;
-                    }
-                    const int uu = c2 + o2, w2 = ((uu % nc2) + nc2) % nc2, l2 = (uu - w2) / nc2;
-                    j = cstart[colbase + w2];
-                    je = cstart[colbase + w2 + 1];
-                    yy2 = y2 - l2 * box.L[2];
-                    code = code01 | ((unsigned)(l2 + 4) << 12);
-                    continue;
-                }
-                nk = min(64, je - j);
-                if (lane < nk) {
-                    const double4 xs = spos[j + lane];
-                    cx = xs.x;
-                    cy = xs.y;
-                    cz = xs.z;
-                }
-                k = 0;
-            }
-            const double sx = lane_bcast(cx, k), sy = lane_bcast(cy, k), sz = lane_bcast(cz, k);
-            const double d0 = yy0 - sx, d1 = yy1 - sy, d2 = yy2 - sz;
-            const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-            const bool in = active && r2 < rc2;
-            const unsigned jk = (unsigned)(j + k);
-            if (!MIXED) {
-                const unsigned long long m = __ballot(in);
-                if (in) W.q64[(qh + qn + lane_prefix(m)) & (kQ - 1)] = make_uint2(jk, code);
-                qn += __popcll(m);
-            } else {
-                const bool far = in && r2 >= kp.rfar2;
-                const bool near = in && !far;
-                const unsigned long long mn = __ballot(near), mf = __ballot(far);
-                if (near) W.q64[(qh + qn + lane_prefix(mn)) & (kQ - 1)] = make_uint2(jk, code);
-                if (far) W.q32[(fh + fn + lane_prefix(mf)) & (kQ - 1)] = make_uint2(jk, code);
-                qn += __popcll(mn);
-                fn += __popcll(mf);
-            }
-            if (++k >= nk) j += nk;
+    // evaluate every full ring (or, at the end, every non-empty one)
+    auto flush = [&](const bool last) {
+        while (qn >= 64 || (last && qn > 0)) {
+            const int c = qn < 64 ? qn : 64;
+            batch64<TABLE>(W, lane, c, qh, rh, rn, spos, sfrc, box, K, tab);
+            qh = (qh + c) & (kQ - 1);
+            qn -= c;
+            n64 += c;
         }
-        // evaluate: one batch per ring that is full (or, at the end, non-empty)
-        const int cnt = qn >= 64 ? 64 : (done ? qn : 0);
-        if (cnt > 0) {
-            __builtin_amdgcn_wave_barrier();
-            if (lane < cnt) eval64<TABLE>(W, W.q64[(qh + lane) & (kQ - 1)], spos, sfrc, box, kp, kpp, tab);
-            qh = (qh + cnt) & (kQ - 1);
-            qn -= cnt;
-            n64 += cnt;
+        while (rn >= 64 || (last && rn > 0)) {
+            const int c = rn < 64 ? rn : 64;
+            batch_rare(W, lane, c, rh, spos, sfrc, box, kpp);
+            rh = (rh + c) & (kQ - 1);
+            rn -= c;
+            n64 += c;
         }
         if (MIXED) {
-            const int fc = fn >= 64 ? 64 : (done ? fn : 0);
-            if (fc > 0) {
-                __builtin_amdgcn_wave_barrier();
-                if (lane < fc) eval32<TABLE>(W, W.q32[(fh + lane) & (kQ - 1)], spos, sfrc, box, kp, tab32);
-                fh = (fh + fc) & (kQ - 1);
-                fn -= fc;
-                n32 += fc;
+            while (fn >= 64 || (last && fn > 0)) {
+                const int c = fn < 64 ? fn : 64;
+                batch32<TABLE>(W, lane, c, fh, spos, sfrc, box, K, tab32);
+                fh = (fh + c) & (kQ - 1);
+                fn -= c;
+                n32 += c;
             }
         }
-        if (done && qn == 0 && (!MIXED || fn == 0)) break;
+    };
+
+    // this wavefront's columns are split, split + nsplit, ...; the lanes set
+    // up 64 of them at a time
+    for (int cb = split; cb < ncols; cb += 64 * nsplit) {
+        const int colv = cb + nsplit * lane;
+        int zs = 0, ze = -1, l0 = 0, l1 = 0, colbase = 0, l2 = 0, zend = -1, ja = 0, jea = 0;
+        bool ok = colv < ncols;
+        if (ok) {
+            const int u0 = ulo0 + colv / ny, u1 = ulo1 + colv % ny;
+            const double g0 = fmax(0.0, fmax(u0 * cells.h[0] - bmax0, bmin0 - (u0 + 1) * cells.h[0]));
+            const double g1 = fmax(0.0, fmax(u1 * cells.h[1] - bmax1, bmin1 - (u1 + 1) * cells.h[1]));
+            const double g2 = g0 * g0 + g1 * g1;
+            ok = g2 < rcx2;
+            if (ok) {
+                const double rem = sqrt(rcx2 - g2);
+                zs = (int)floor((bmin2 - rem) * cells.inv_h[2]);
+                ze = (int)floor((bmax2 + rem) * cells.inv_h[2]);
+                l0 = floordiv(u0, nc0);
+                l1 = floordiv(u1, nc1);
+                colbase = ((u0 - l0 * nc0) * nc1 + (u1 - l1 * nc1)) * nc2;
+                l2 = floordiv(zs, nc2);
+                zend = min(ze, (l2 + 1) * nc2 - 1);
+                ja = cstart[colbase + zs - l2 * nc2];
+                jea = cstart[colbase + zend - l2 * nc2 + 1];
+            }
+        }
+        unsigned long long cm = __ballot(ok);
+        while (cm) {
+            const int i = __builtin_ctzll(cm);
+            cm &= cm - 1;
+            // lane i's column, uniform from here
+            const int cl0 = lane_read(l0, i), cl1 = lane_read(l1, i), cze = lane_read(ze, i);
+            const int cbase = lane_read(colbase, i);
+            int cl2 = lane_read(l2, i), czend = lane_read(zend, i);
+            int j = lane_read(ja, i), je = lane_read(jea, i);
+            const double sh0 = cl0 * box.L[0], sh1 = cl1 * box.L[1];
+            const unsigned code01 = (unsigned)lane | ((unsigned)(cl0 + 4) << 6) | ((unsigned)(cl1 + 4) << 9);
+            for (;;) {
+                const double sh2 = cl2 * box.L[2];
+                const unsigned code = code01 | ((unsigned)(cl2 + 4) << 12);
+                for (; j < je; j += 64) {
+                    // stage the next 64 sources of the run at this image, keeping
+                    // those within rc of the tile box, compacted
+                    const int nk = min(64, je - j);
+                    bool keep = false;
+                    double sx = 0, sy = 0, sz = 0;
+                    if (lane < nk) {
+                        const double4 xs = spos[j + lane];
+                        sx = xs.x + sh0;
+                        sy = xs.y + sh1;
+                        sz = xs.z + sh2;
+                        const double g0 = fmax(0.0, fmax(bmin0 - sx, sx - bmax0));
+                        const double g1 = fmax(0.0, fmax(bmin1 - sy, sy - bmax1));
+                        const double g2 = fmax(0.0, fmax(bmin2 - sz, sz - bmax2));
+                        keep = g0 * g0 + g1 * g1 + g2 * g2 < rcx2;
+                    }
+                    const unsigned long long km = __ballot(keep);
+                    const int nkeep = __popcll(km);
+                    __builtin_amdgcn_wave_barrier();
+                    if (keep)
+                        W.stage[lane_prefix(km)] = make_double4(sx, sy, sz, __longlong_as_double((long long)(j + lane)));
+                    __builtin_amdgcn_wave_barrier();
+                    for (int k = 0; k < nkeep; k++) {
+                        const double4 sc = W.stage[k]; // same address in every lane: broadcast
+                        const unsigned jk = (unsigned)__double_as_longlong(sc.w);
+                        const double d0 = y0 - sc.x, d1 = y1 - sc.y, d2 = y2 - sc.z;
+                        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+                        const bool in = r2 < rc2;
+                        const unsigned long long m = __ballot(in);
+                        if (!MIXED) {
+                            if (in) W.q[(qh + qn + lane_prefix(m)) & (kQ - 1)] = make_uint2(jk, code);
+                            qn += __popcll(m);
+                            if (qn >= 64) flush(false);
+                        } else {
+                            const bool far = r2 >= rfar2;
+                            const unsigned long long mf = __ballot(in && far), mn = m & ~mf;
+                            if (in) {
+                                const unsigned v = (unsigned)(far ? fh + fn : qh + qn) + lane_prefix(far ? mf : mn);
+                                W.q[(v & (kQ - 1)) | (far ? kQ : 0)] = make_uint2(jk, code);
+                            }
+                            qn += __popcll(mn);
+                            fn += __popcll(mf);
+                            if (qn >= 64 || fn >= 64) flush(false);
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+                if (czend >= cze) break;
+                // the z range runs into the next image: next run of sources
+                const int zsn = czend + 1;
+                cl2 = floordiv(zsn, nc2);
+                czend = min(cze, (cl2 + 1) * nc2 - 1);
+                j = cstart[cbase + zsn - cl2 * nc2];
+                je = cstart[cbase + czend - cl2 * nc2 + 1];
+            }
+        }
     }
+    flush(true);
     __builtin_amdgcn_wave_barrier();
-    if (active) {
-        const size_t o = 3 * (size_t)tperm[T.first + lane];
+    if (active && (n64 | n32)) {
+        const size_t o = 3 * (size_t)tperm[t0 + lane];
         atomicAdd(&u[o + 0], W.acc[lane][0]);
         atomicAdd(&u[o + 1], W.acc[lane][1]);
         atomicAdd(&u[o + 2], W.acc[lane][2]);
     }
-    if (lane == 0 && counters) {
+    if (lane == 0 && counters && (n64 | n32)) {
         atomicAdd(&counters[0], n64);
         atomicAdd(&counters[1], n32);
     }
+}
+
+#define PB_PAIR_PARAMS                                                                                 \
+    const int *__restrict__ cstart, const double4 *__restrict__ spos, const double4 *__restrict__ sfrc,   \
+        const double4 *__restrict__ tpos, const int *__restrict__ tperm, int nt, int nsplit, pb_cells cells, \
+        pb_box box, const pb_kparams *__restrict__ kpp, double *__restrict__ u,                           \
+        unsigned long long *__restrict__ counters
+#define PB_PAIR_FWD cstart, spos, sfrc, tpos, tperm, nt, nsplit, cells, box, kpp, u, counters
+
+// The table path fits 4 wavefronts per SIMD; the closed forms need more
+// registers and run with the compiler's own choice.
+template <bool MIXED>
+__global__ __launch_bounds__(64 * kWaves, 4) void k_pair_table(PB_PAIR_PARAMS)
+{
+    pair_body<MIXED, true>(PB_PAIR_FWD);
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(64 * kWaves) void k_pair_direct(PB_PAIR_PARAMS)
+{
+    pair_body<MIXED, false>(PB_PAIR_FWD);
 }
 
 } // namespace
@@ -273,23 +435,22 @@ int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kp
     if (!accumulate &&
         (st = pb_hip_check(c, hipMemsetAsync(u, 0, 3 * (size_t)nt * sizeof(double), s), "memset u")))
         return st;
-    size_t max_tiles = (size_t)cells.ncell + (size_t)nt / 64 + 1;
-    size_t ncol = (size_t)(2 * cells.st[0] + 1) * (2 * cells.st[1] + 1);
-    unsigned blocks = (unsigned)((max_tiles * ncol + kWaves - 1) / kWaves);
-    const int *ntiles = (const int *)c->tile_off.ptr + cells.ncell;
+    const int ntiles = (nt + 63) / 64;
+    const int nsplit = 8;
+    const unsigned blocks = (unsigned)(((long long)ntiles * nsplit + kWaves - 1) / kWaves);
     pb_mark(c, s, PB_NPHASE - 1); // the pair-kernel interval covers only this launch
     const bool mixed = precision == PB_PREC_MIXED;
     const size_t ntab = table_doubles; // 0: closed forms per pair
     const size_t lds = ntab ? ntab * sizeof(double) + (mixed ? ntab * sizeof(float) : 0) : 0;
-#define PB_PAIR_ARGS                                                                                  \
-    (const pb_tile *)c->tiles.ptr, ntiles, (const int *)c->cell_start.ptr, (const double4 *)c->spos.ptr, \
-        (const double4 *)c->sfrc.ptr, (const double4 *)c->tpos.ptr, (const int *)c->tperm.ptr, cells, box, \
-        kp, u, (unsigned long long *)c->counters.ptr
+#define PB_PAIR_ARGS                                                                                   \
+    (const int *)c->cell_start.ptr, (const double4 *)c->spos.ptr, (const double4 *)c->sfrc.ptr,         \
+        (const double4 *)c->tpos.ptr, (const int *)c->tperm.ptr, nt, nsplit, cells, box, kp, u,         \
+        (unsigned long long *)c->counters.ptr
     dim3 gd(blocks), bd(64 * kWaves);
-    if (mixed && ntab) hipLaunchKernelGGL((k_pair<true, true>), gd, bd, lds, s, PB_PAIR_ARGS);
-    else if (mixed) hipLaunchKernelGGL((k_pair<true, false>), gd, bd, 0, s, PB_PAIR_ARGS);
-    else if (ntab) hipLaunchKernelGGL((k_pair<false, true>), gd, bd, lds, s, PB_PAIR_ARGS);
-    else hipLaunchKernelGGL((k_pair<false, false>), gd, bd, 0, s, PB_PAIR_ARGS);
+    if (mixed && ntab) hipLaunchKernelGGL((k_pair_table<true>), gd, bd, lds, s, PB_PAIR_ARGS);
+    else if (mixed) hipLaunchKernelGGL((k_pair_direct<true>), gd, bd, 0, s, PB_PAIR_ARGS);
+    else if (ntab) hipLaunchKernelGGL((k_pair_table<false>), gd, bd, lds, s, PB_PAIR_ARGS);
+    else hipLaunchKernelGGL((k_pair_direct<false>), gd, bd, 0, s, PB_PAIR_ARGS);
 #undef PB_PAIR_ARGS
     return pb_hip_check(c, hipGetLastError(), "pair kernel launch");
 }
