@@ -109,12 +109,16 @@ def main():
     p = pb.Params(alpha=ALPHA, eps=EPS, grid=GRID, P=P, tol=TOL).tune(NS, NT)
     pm, p64 = p.copy(precision="mixed"), p.copy(precision="fp64")
     pplain = p.copy(precision="fp64", kernel="direct")
+    pfar32 = p.copy(precision="mixed", kernel="table_far32")
 
     log(f"params {p}")
     ms_mixed, ph_mixed = timed_steps(ctx, pm, xs, fs, ys, u, args.steps, args.warmup)
     log(f"mixed form: {ms_mixed:.3f} ms/step {ph_mixed}")
     u_mixed = u.cpu().numpy().copy()
     n64, n32 = ctx.last_pair_counts()
+    stats = ctx.last_pair_stats()
+    log(f"pair work: fp64 pairs {stats[0]} fp32 pairs {stats[1]} candidates tested {stats[2]} "
+        f"read {stats[3]}")
     ms_fp64, ph_fp64 = timed_steps(ctx, p64, xs, fs, ys, u, args.steps, args.warmup)
     u_fp64 = u.cpu().numpy().copy()
     n64b, _ = ctx.last_pair_counts()
@@ -122,6 +126,9 @@ def main():
     ms_plain, ph_plain = timed_steps(ctx, pplain, xs, fs, ys, u, args.steps, args.warmup)
     u_plain = u.cpu().numpy().copy()
     log(f"plain fp64 (closed forms) form: {ms_plain:.3f} ms/step {ph_plain}")
+    ms_far32, ph_far32 = timed_steps(ctx, pfar32, xs, fs, ys, u, args.steps, args.warmup)
+    u_far32 = u.cpu().numpy().copy()
+    log(f"mixed form with FP32 far-field table: {ms_far32:.3f} ms/step {ph_far32}")
 
     # FP64 reference with tighter parameters; a second split checks it
     lt = np.log(1e13)
@@ -136,6 +143,7 @@ def main():
     err_mixed = relL2(u_mixed, uref)
     err_fp64 = relL2(u_fp64, uref)
     err_plain = relL2(u_plain, uref)
+    err_far32 = relL2(u_far32, uref)
     ref_split = relL2(ref[1][1], uref)
     log(f"rel L2 mixed {err_mixed:.3e} fp64 {err_fp64:.3e} reference split check {ref_split:.3e}")
 
@@ -169,6 +177,9 @@ def main():
         "fp64_form": {"value": NT / (ms_fp64 / 1e3), "ms_per_step": ms_fp64,
                       "phases_ms": ph_fp64, "pairs_fp64": n64b,
                       "note": "every step FP64; real-space radial functions from the FP64 table"},
+        "mixed_far32_form": {"value": NT / (ms_far32 / 1e3), "ms_per_step": ms_far32,
+                             "phases_ms": ph_far32, "rel_l2": err_far32,
+                             "note": "mixed form with pairs beyond max(3d, 8e) from an FP32 table"},
         "plain_fp64_form": {"value": NT / (ms_plain / 1e3), "ms_per_step": ms_plain,
                             "phases_ms": ph_plain,
                             "note": "every step FP64; closed forms (exp, erfcx) evaluated per pair"},

@@ -35,11 +35,15 @@ extern "C" {
 #define PB_ERR_UNSUPPORTED 4 /* valid but not supported (e.g. FFT length) */
 
 #define PB_PREC_FP64 0  /* plain form: every step in FP64 */
-#define PB_PREC_MIXED 1 /* FP64 near field, FP32 far field and k-space */
+#define PB_PREC_MIXED 1 /* FP32 k-space (spread, FFT, multiply, interpolation);
+                           real space as chosen by the kernel field */
 
 #define PB_KERNEL_TABLE 0  /* real-space radial functions from a piecewise-polynomial
-                              table built from the closed forms (default) */
-#define PB_KERNEL_DIRECT 1 /* closed forms (exp, erfcx) evaluated per pair */
+                              table built from the closed forms, FP64 (default) */
+#define PB_KERNEL_DIRECT 1 /* closed forms (exp, erfcx) evaluated per pair; with
+                              PB_PREC_MIXED, pairs beyond max(3d, 8e) in FP32 */
+#define PB_KERNEL_TABLE_FAR32 2 /* table; with PB_PREC_MIXED, pairs beyond
+                              max(3d, 8e) from an FP32 copy of the table */
 
 /* ---- entry points replacing the reference's functions -------------------- */
 
@@ -112,7 +116,7 @@ typedef struct pb_params {
     double m;      /* window half-width in standard deviations (<= 0: sqrt(pi P)) */
     int precision; /* PB_PREC_FP64 or PB_PREC_MIXED */
     double tol;    /* target relative error for pb_tune */
-    int kernel;    /* PB_KERNEL_TABLE or PB_KERNEL_DIRECT */
+    int kernel;    /* PB_KERNEL_TABLE, PB_KERNEL_DIRECT or PB_KERNEL_TABLE_FAR32 */
 } pb_params;
 
 /* Defaults: unit box, alpha 1, eps 0, P 12, FP64, tol 1e-6, the rest 0. */
@@ -145,6 +149,12 @@ int pb_velocity_kspace(pb_context *ctx, const pb_params *p, int ns, const double
 /* Statistics of the last pb_velocity call on ctx (device work counters read
  * back synchronously): pairs evaluated in FP64 and FP32. */
 int pb_last_pair_counts(pb_context *ctx, long long *n_fp64, long long *n_fp32);
+
+/* Work statistics of the last real-space pass (no counterpart in the
+ * reference; for tuning): out[0] pairs evaluated in FP64, out[1] in FP32,
+ * out[2] candidate sources broadcast to a target tile, out[3] candidate
+ * sources read from the cell runs before the tile-box filter. */
+int pb_last_pair_stats(pb_context *ctx, long long out[4]);
 
 /* Phase timing: while enabled, every pb_velocity call records HIP events on
  * its stream around its phases (no host synchronization for up to 64 calls).

@@ -22,7 +22,8 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "periodicbrinkman.
 
 PB_OK, PB_ERR_INVALID, PB_ERR_HIP, PB_ERR_NOMEM, PB_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
-KERNEL_TABLE, KERNEL_DIRECT = 0, 1
+KERNEL_TABLE, KERNEL_DIRECT, KERNEL_TABLE_FAR32 = 0, 1, 2
+_KERNELS = {"table": KERNEL_TABLE, "direct": KERNEL_DIRECT, "table_far32": KERNEL_TABLE_FAR32}
 
 _lib = None
 _D = ctypes.POINTER(ctypes.c_double)
@@ -86,6 +87,7 @@ def _declare(L):
     L.pb_velocity.argtypes = vel
     L.pb_velocity_real.argtypes = vel
     L.pb_velocity_kspace.argtypes = vel
+    L.pb_last_pair_stats.argtypes = [_VP, ctypes.POINTER(ctypes.c_longlong)]
     L.pb_last_pair_counts.argtypes = [_VP, ctypes.POINTER(ctypes.c_longlong),
                                       ctypes.POINTER(ctypes.c_longlong)]
     L.pb_set_timing.argtypes = [_VP, ii]
@@ -138,9 +140,11 @@ def _stream_ptr(stream):
 
 # ---- parameters ---------------------------------------------------------------
 class Params:
-    """Evaluator parameters (pb_params).  precision: 'fp64' or 'mixed';
-    kernel: 'table' (radial functions from the piecewise-polynomial table) or
-    'direct' (closed forms per pair)."""
+    """Evaluator parameters (pb_params).  precision: 'fp64' or 'mixed' (FP32
+    k-space); kernel: 'table' (real-space radial functions from the FP64
+    piecewise-polynomial table), 'direct' (closed forms per pair; FP32 beyond
+    max(3d, 8e) in mixed precision) or 'table_far32' (table; FP32 copy beyond
+    max(3d, 8e) in mixed precision)."""
 
     def __init__(self, L=(1.0, 1.0, 1.0), alpha=1.0, eps=0.0, d=0.0, rc=0.0, grid=(0, 0, 0), P=12,
                  m=0.0, precision="fp64", tol=1e-6, kernel="table"):
@@ -159,7 +163,9 @@ class Params:
         p.alpha, p.eps, p.d, p.rc = self.alpha, self.eps, self.d, self.rc
         p.P, p.m, p.tol = self.P, self.m, self.tol
         p.precision = PREC_MIXED if self.precision == "mixed" else PREC_FP64
-        p.kernel = KERNEL_DIRECT if self.kernel == "direct" else KERNEL_TABLE
+        if self.kernel not in _KERNELS:
+            raise PBError(PB_ERR_INVALID, f"kernel must be one of {sorted(_KERNELS)}")
+        p.kernel = _KERNELS[self.kernel]
         return p
 
     def tune(self, ns, nt):
@@ -242,6 +248,13 @@ class Context:
         a, b = ctypes.c_longlong(), ctypes.c_longlong()
         _check(lib().pb_last_pair_counts(self.handle, ctypes.byref(a), ctypes.byref(b)), self)
         return a.value, b.value
+
+    def last_pair_stats(self):
+        """(FP64 pairs, FP32 pairs, candidates tested, candidates read) of the
+        last real-space pass."""
+        out = (ctypes.c_longlong * 4)()
+        _check(lib().pb_last_pair_stats(self.handle, out), self)
+        return tuple(int(v) for v in out)
 
 
 # ---- reference-signature entry points -----------------------------------------

@@ -96,8 +96,9 @@ int check_params(pb_context *c, const pb_params *p)
     if (!finite_pos(p->rc)) return pb_fail(c, PB_ERR_INVALID, "cutoff rc must be > 0 (run pb_tune)");
     if (p->precision != PB_PREC_FP64 && p->precision != PB_PREC_MIXED)
         return pb_fail(c, PB_ERR_INVALID, "precision must be PB_PREC_FP64 or PB_PREC_MIXED");
-    if (p->kernel != PB_KERNEL_TABLE && p->kernel != PB_KERNEL_DIRECT)
-        return pb_fail(c, PB_ERR_INVALID, "kernel must be PB_KERNEL_TABLE or PB_KERNEL_DIRECT");
+    if (p->kernel != PB_KERNEL_TABLE && p->kernel != PB_KERNEL_DIRECT && p->kernel != PB_KERNEL_TABLE_FAR32)
+        return pb_fail(c, PB_ERR_INVALID,
+                       "kernel must be PB_KERNEL_TABLE, PB_KERNEL_DIRECT or PB_KERNEL_TABLE_FAR32");
     return PB_OK;
 }
 
@@ -182,7 +183,7 @@ int device_kparams(pb_context *c, hipStream_t s, const pb_params *p, double rfar
         pb_kparams kp = cached_kparams(p->alpha, p->d, p->eps, rfar);
         kp.tn = 0;
         c->table_doubles = 0;
-        if (p->kernel == PB_KERNEL_TABLE && p->rc > kp.rcommon) {
+        if (p->kernel != PB_KERNEL_DIRECT && p->rc > kp.rcommon) {
             std::vector<double> coef;
             pb_build_table(&kp, kp.rcommon / p->d, p->rc / p->d * (1 + 1e-9), coef);
             if (kp.tn > 0) {
@@ -243,7 +244,12 @@ int velocity_impl(pb_context *c, const pb_params *p, int ns, const double *xn, c
         const pb_kparams *kp = nullptr;
         size_t ntab = 0;
         if ((st = device_kparams(c, s, p, rfar, &kp, &ntab))) return st;
-        if ((st = pb_run_real(c, s, cells, kp, ntab, box, ns, nt, u, acc, p->precision))) return st;
+        // FP32 far-field pairs: the closed-form kernel in mixed precision, or
+        // the table on request (on gfx950 FP64 FMA runs at the non-packed
+        // FP32 rate, so the FP64 table is the faster default)
+        const bool far32 = p->precision == PB_PREC_MIXED &&
+                           (p->kernel == PB_KERNEL_DIRECT || (p->kernel == PB_KERNEL_TABLE_FAR32 && ntab));
+        if ((st = pb_run_real(c, s, cells, kp, ntab, box, ns, nt, u, acc, far32))) return st;
     }
     pb_mark(c, s, PB_NPHASE);
     return PB_OK;
@@ -343,6 +349,9 @@ int pb_create(pb_context **ctx, int device)
     if (device < 0 || device >= n) return PB_ERR_INVALID;
     pb_context *c = new pb_context();
     c->device = device;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        c->cu_count = cus;
     *ctx = c;
     return PB_OK;
 }
@@ -353,7 +362,7 @@ int pb_destroy(pb_context *c)
     (void)hipSetDevice(c->device);
     pb_buffer *bufs[] = {&c->cell_count, &c->cell_start, &c->cell_of_s, &c->rank_s, &c->cell_of_t,
                          &c->rank_t, &c->tcell_count, &c->tcell_start, &c->spos, &c->sfrc, &c->tpos,
-                         &c->tperm, &c->counters, &c->scan_tmp, &c->grid,
+                         &c->tperm, &c->counters, &c->scan_tmp, &c->qcount, &c->qoff, &c->tiles, &c->grid,
                          &c->twiddle, &c->axis_fac, &c->dense_tmp, &c->kparams, &c->table};
     for (pb_buffer *b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
@@ -416,15 +425,25 @@ int pb_velocity_kspace(pb_context *ctx, const pb_params *p, int ns, const double
     return velocity_impl(ctx, p, ns, xn, f, nt, y, u, stream, false, true);
 }
 
+int pb_last_pair_stats(pb_context *c, long long out[4])
+{
+    if (!c || !out) return PB_ERR_INVALID;
+    out[0] = out[1] = out[2] = out[3] = 0;
+    if (!c->counters.ptr || c->stat_items <= 0) return PB_OK;
+    std::vector<unsigned> h(4 * (size_t)c->stat_items);
+    int st = pb_hip_check(c, hipMemcpy(h.data(), c->counters.ptr, h.size() * sizeof(unsigned),
+                                       hipMemcpyDeviceToHost), "counters");
+    for (size_t i = 0; i < h.size(); i++) out[i & 3] += h[i];
+    return st;
+}
+
 int pb_last_pair_counts(pb_context *c, long long *n64, long long *n32)
 {
     if (!c || !n64 || !n32) return PB_ERR_INVALID;
-    *n64 = *n32 = 0;
-    if (!c->counters.ptr) return PB_OK;
-    unsigned long long h[2] = {0, 0};
-    int st = pb_hip_check(c, hipMemcpy(h, c->counters.ptr, sizeof(h), hipMemcpyDeviceToHost), "counters");
-    *n64 = (long long)h[0];
-    *n32 = (long long)h[1];
+    long long s[4];
+    int st = pb_last_pair_stats(c, s);
+    *n64 = s[0];
+    *n32 = s[1];
     return st;
 }
 

@@ -3,8 +3,9 @@
 // Both use the same grid of fine cells (side >= rc/3).  Sources are ordered
 // z-major, so the cells of one (x, y) column are one contiguous run of the
 // sorted array.  Targets are ordered by a brick-Morton key (bricks of 4x4x4
-// cells in z-major order, Morton order inside a brick), so every 64
-// consecutive sorted targets form a compact tile for the pair kernel.
+// cells in z-major order, Morton order inside a brick), so the targets of a
+// quarter brick (2x2x4 cells, 16 consecutive keys) are contiguous; the pair
+// kernel's tiles are runs of <= 64 of them (pb_build_tiles).
 //
 // Positions are wrapped into [0, L) with the same operation sequence as the
 // oracle (v - floor(v/L)*L, no contraction) so both place a point on a cell
@@ -141,7 +142,58 @@ __global__ __launch_bounds__(256) void k_scan_apply(int n, const int *__restrict
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = sums[nb];
 }
 
+__global__ void k_quarter_tiles(int nq, const int *__restrict__ tstart, int *__restrict__ qcount)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int cnt = tstart[16 * q + 16] - tstart[16 * q];
+    qcount[q] = (cnt + 63) >> 6;
+}
+
+__global__ void k_write_tiles(int nq, const int *__restrict__ tstart, const int *__restrict__ qoff,
+                              int2 *__restrict__ tiles)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int first = tstart[16 * q], cnt = tstart[16 * q + 16] - first;
+    int o = qoff[q];
+    for (int i = 0; 64 * i < cnt; i++) tiles[o + i] = make_int2(first + 64 * i, min(64, cnt - 64 * i));
+}
+
 } // namespace
+
+// Exclusive scan of n ints from in to out (out[n] = total) on stream s.
+static int run_scan(pb_context *c, hipStream_t s, int n, const int *in, int *out)
+{
+    const int nb = (n + kScanBlock - 1) / kScanBlock;
+    if (nb > 256 * 16) return pb_fail(c, PB_ERR_UNSUPPORTED, "too many cells for the scan");
+    int st;
+    if ((st = pb_reserve(c, &c->scan_tmp, sizeof(int) * (nb + 1)))) return st;
+    int *sums = (int *)c->scan_tmp.ptr;
+    hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(256), 0, s, n, in, sums);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, nb, sums);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, n, in, (const int *)sums, nb, out);
+    return PB_OK;
+}
+
+int pb_build_tiles(pb_context *c, hipStream_t s, const pb_cells &cells, int nt, int *max_tiles)
+{
+    // a quarter of a brick is 2x2x4 cells: 16 consecutive target keys
+    const int nq = cells.nkey_t / 16;
+    const size_t mt = (size_t)nq + (size_t)nt / 64 + 1;
+    int st;
+    if ((st = pb_reserve(c, &c->qcount, sizeof(int) * (nq + 1)))) return st;
+    if ((st = pb_reserve(c, &c->qoff, sizeof(int) * (nq + 1)))) return st;
+    if ((st = pb_reserve(c, &c->tiles, sizeof(int2) * mt))) return st;
+    const int blocks = (nq + 255) / 256;
+    hipLaunchKernelGGL(k_quarter_tiles, dim3(blocks), dim3(256), 0, s, nq, (const int *)c->tcell_start.ptr,
+                       (int *)c->qcount.ptr);
+    if ((st = run_scan(c, s, nq, (const int *)c->qcount.ptr, (int *)c->qoff.ptr))) return st;
+    hipLaunchKernelGGL(k_write_tiles, dim3(blocks), dim3(256), 0, s, nq, (const int *)c->tcell_start.ptr,
+                       (const int *)c->qoff.ptr, (int2 *)c->tiles.ptr);
+    *max_tiles = (int)mt;
+    return pb_hip_check(c, hipGetLastError(), "tile launch");
+}
 
 int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_box &box, int n,
                   const double *pos, const double *frc, bool is_source)
@@ -171,16 +223,7 @@ int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_
         hipLaunchKernelGGL(k_bin_count, dim3(blocks), dim3(256), 0, s, n, pos, box, cells,
                            is_source ? 0 : 1, (int *)count.ptr, (int *)key_of.ptr, (int *)rank.ptr);
     }
-    {
-        const int nb = (nkey + kScanBlock - 1) / kScanBlock;
-        if (nb > 256 * 16) return pb_fail(c, PB_ERR_UNSUPPORTED, "too many cells for the scan");
-        if ((st = pb_reserve(c, &c->scan_tmp, sizeof(int) * (nb + 1)))) return st;
-        int *sums = (int *)c->scan_tmp.ptr;
-        hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(256), 0, s, nkey, (const int *)count.ptr, sums);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, nb, sums);
-        hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, nkey, (const int *)count.ptr,
-                           (const int *)sums, nb, (int *)start.ptr);
-    }
+    if ((st = run_scan(c, s, nkey, (const int *)count.ptr, (int *)start.ptr))) return st;
     if (n > 0) {
         int blocks = (n + 255) / 256;
         hipLaunchKernelGGL(k_bin_scatter, dim3(blocks), dim3(256), 0, s, n, pos, frc, box,

@@ -6,12 +6,15 @@
 //                    cell_start[c] .. cell_start[c+1]: sources of cell
 //                    c = (c0 * nc1 + c1) * nc2 + c2 (z-major)
 //   sorted targets   tpos[t] = (x, y, z, 0), tperm[t] = original index, in
-//                    brick-Morton order: targets 64k .. 64k+63 are tile k
+//                    brick-Morton order; tiles[k] = (first, count <= 64)
+//                    within one quarter brick (2x2x4 cells)
 //   k-space grids    3 complex fields, element (i0, i1, i2) at (i0*n1 + i1)*n2 + i2
 #pragma once
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -115,11 +118,13 @@ struct pb_buffer {
 
 struct pb_context {
     int device = 0;
+    int cu_count = 256; // compute units of the device
     std::string err;
     // real space
     pb_buffer cell_count, cell_start, cell_of_s, rank_s, cell_of_t, rank_t, tcell_count,
         tcell_start;
     pb_buffer spos, sfrc, tpos, tperm, counters, scan_tmp;
+    pb_buffer qcount, qoff, tiles; // target tiles of the pair kernel: int2 (first, count)
     // k space
     pb_buffer grid, twiddle, axis_fac;
     int twiddle_n = 0;
@@ -131,8 +136,9 @@ struct pb_context {
     pb_buffer kparams, table;
     double kp_key[6] = {-1, -1, -1, -1, -1, -1};
     size_t table_doubles = 0;
-    // host copies of the last call's counters
-    long long last_fp64 = 0, last_fp32 = 0;
+    // per-work-item statistics of the last real-space pass (uint4 each in
+    // counters): FP64 pairs, FP32 pairs, candidates tested, candidates read
+    long long stat_items = 0;
     // phase timing (pb_set_timing): a ring of event sets, one per call, so a
     // timed loop of up to PB_TSLOTS calls needs no host synchronization
     int timing = 0;
@@ -154,9 +160,14 @@ int pb_hip_check(pb_context *c, hipError_t e, const char *what);
 // ---- launchers (pb_bin.hip, pb_real.hip, pb_kspace.hip, pb_dense.hip) -------
 int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_box &box, int n,
                   const double *pos, const double *frc, bool is_source);
+// Target tiles: runs of <= 64 sorted targets inside one quarter brick; the
+// tile count is left on the device, *max_tiles bounds it.
+int pb_build_tiles(pb_context *c, hipStream_t s, const pb_cells &cells, int nt, int *max_tiles);
+// far32: pairs beyond rfar in FP32 (table copy when table_doubles > 0, else
+// the FP32 closed form); otherwise every pair in FP64.
 int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp_dev,
                 size_t table_doubles, const pb_box &box, int ns, int nt, double *u, bool accumulate,
-                int precision);
+                bool far32);
 int pb_run_kspace(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells,
                   const pb_box &box, double a, double d, int ns, int nt, double *u, bool accumulate,
                   int precision);

@@ -5,8 +5,8 @@
 // (periodicBrinkman.h:152-176; the reference sums all pairs over image cubes,
 // realShells :564-693, kernelReal :219-314).
 //
-// Work item = (tile, split): a tile is 64 consecutive targets in brick-Morton
-// order (a compact cluster, one per lane); its bounding box B decides the
+// Work item = (tile, split): a tile is <= 64 targets of one quarter brick
+// (2x2x4 cells, one target per lane); its bounding box B decides the
 // candidate sources: the (x, y) cell columns within rc of B, each trimmed in
 // z to sqrt(rc^2 - gap^2), which are contiguous runs of the z-major sorted
 // sources.  The columns are dealt round-robin to `nsplit` wavefronts, which
@@ -21,9 +21,9 @@
 // sparse the test is.  An evaluated batch moves its rare pairs with
 // r < rcommon (series and blob regions) to a third ring that is evaluated 64
 // at a time out of line.  Results go to per-target FP64 accumulators in LDS
-// (ds_add_f64).  In mixed precision, pairs beyond rfar (>= 3d and >= 8e) are
-// classified at the test into the second half of the ring and evaluated in
-// FP32, and still accumulated in FP64.
+// (ds_add_f64).  With an FP32 far field (the MIXED template), pairs beyond
+// rfar (>= 3d and >= 8e) are classified at the test into the second half of
+// the ring and evaluated in FP32, and still accumulated in FP64.
 #include "pb_radial.h"
 
 namespace {
@@ -31,11 +31,13 @@ namespace {
 constexpr int kWaves = 4; // wavefronts per workgroup, each independent
 constexpr int kQ = 128;   // ring capacity (power of two)
 
+
+
 struct wave_lds {
     double y[64][3];
     double acc[64][3];
     double4 stage[64]; // candidate sources of the current run: image position, index
-    uint2 q[2 * kQ];   // [0, kQ): FP64 ring; [kQ, 2 kQ): FP32 far ring (mixed precision)
+    uint2 q[2 * kQ];   // [0, kQ): FP64 ring; [kQ, 2 kQ): FP32 far ring
     uint2 qr[kQ];      // r < rcommon, moved out of evaluated FP64 batches
 };
 
@@ -207,18 +209,18 @@ __device__ __forceinline__ void batch32(wave_lds &W, const int lane, const int c
 
 template <bool MIXED, bool TABLE>
 __device__ __forceinline__ void pair_body(
+    const int2 *__restrict__ tiles, const int *__restrict__ ntiles_ptr,
     const int *__restrict__ cstart, const double4 *__restrict__ spos,
     const double4 *__restrict__ sfrc, const double4 *__restrict__ tpos,
-    const int *__restrict__ tperm, int nt, int nsplit, pb_cells cells, pb_box box,
-    const pb_kparams *__restrict__ kpp, double *__restrict__ u,
-    unsigned long long *__restrict__ counters)
+    const int *__restrict__ tperm, int nsplit, pb_cells cells, pb_box box,
+    const pb_kparams *__restrict__ kpp, double *__restrict__ u, uint4 *__restrict__ stats)
 {
     const pb_kparams &kp = *kpp;
     const batch_consts K{pbdev::tabview_of(kp), pbdev::tabview32_of(kp), pbdev::core_of(kp.pd), kp.pf,
                          kp.rcommon};
     const double rfar2 = kp.rfar2;
     __shared__ wave_lds lds[kWaves];
-    extern __shared__ double dyn_lds[]; // the radial table: FP64 then (mixed) FP32 copy
+    extern __shared__ double dyn_lds[]; // the radial table: FP64 then (FP32 far field) FP32 copy
     const int ntab = TABLE ? kp.tn * (PB_TAB_DEG + 1) * 2 : 0;
     if (TABLE) { // every wavefront takes part before any may leave
         for (int k = threadIdx.x; k < ntab; k += blockDim.x) dyn_lds[k] = kp.tab64[k];
@@ -234,10 +236,14 @@ __device__ __forceinline__ void pair_body(
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long long item = (long long)blockIdx.x * kWaves + wave;
     const int tile = (int)(item / nsplit), split = (int)(item % nsplit);
-    const int t0 = tile * 64;
-    if (t0 >= nt) return; // whole wavefront leaves; no block barriers below
+    if (tile >= *ntiles_ptr) { // whole wavefront leaves; no block barriers below
+        if (lane == 0) stats[item] = make_uint4(0u, 0u, 0u, 0u);
+        return;
+    }
+    const int2 T = tiles[tile];
+    const int t0 = T.x;
     wave_lds &W = lds[wave];
-    const bool active = t0 + lane < nt;
+    const bool active = lane < T.y;
     const double4 p = tpos[active ? t0 + lane : t0];
     W.y[lane][0] = p.x;
     W.y[lane][1] = p.y;
@@ -260,7 +266,7 @@ __device__ __forceinline__ void pair_body(
     const double rc2 = cells.rc2;
 
     int qh = 0, qn = 0, fh = 0, fn = 0, rh = 0, rn = 0;
-    unsigned long long n64 = 0, n32 = 0;
+    unsigned n64 = 0, n32 = 0, ntest = 0, nread = 0;
     // evaluate every full ring (or, at the end, every non-empty one)
     auto flush = [&](const bool last) {
         while (qn >= 64 || (last && qn > 0)) {
@@ -285,6 +291,27 @@ __device__ __forceinline__ void pair_body(
                 fn -= c;
                 n32 += c;
             }
+        }
+    };
+
+    // append the in-range lanes of one tested candidate to the rings
+    auto append = [&](const unsigned jk, const double r2, const unsigned code) {
+        const bool in = r2 < rc2;
+        const unsigned long long m = __ballot(in);
+        if (!MIXED) {
+            if (in) W.q[(qh + qn + lane_prefix(m)) & (kQ - 1)] = make_uint2(jk, code);
+            qn += __popcll(m);
+            if (qn >= 64) flush(false);
+        } else {
+            const bool far = r2 >= rfar2;
+            const unsigned long long mf = __ballot(in && far), mn = m & ~mf;
+            if (in) {
+                const unsigned v = (unsigned)(far ? fh + fn : qh + qn) + lane_prefix(far ? mf : mn);
+                W.q[(v & (kQ - 1)) | (far ? kQ : 0)] = make_uint2(jk, code);
+            }
+            qn += __popcll(mn);
+            fn += __popcll(mf);
+            if (qn >= 64 || fn >= 64) flush(false);
         }
     };
 
@@ -345,32 +372,27 @@ __device__ __forceinline__ void pair_body(
                     }
                     const unsigned long long km = __ballot(keep);
                     const int nkeep = __popcll(km);
+                    nread += nk;
+                    ntest += nkeep;
                     __builtin_amdgcn_wave_barrier();
                     if (keep)
                         W.stage[lane_prefix(km)] = make_double4(sx, sy, sz, __longlong_as_double((long long)(j + lane)));
                     __builtin_amdgcn_wave_barrier();
-                    for (int k = 0; k < nkeep; k++) {
-                        const double4 sc = W.stage[k]; // same address in every lane: broadcast
-                        const unsigned jk = (unsigned)__double_as_longlong(sc.w);
-                        const double d0 = y0 - sc.x, d1 = y1 - sc.y, d2 = y2 - sc.z;
-                        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-                        const bool in = r2 < rc2;
-                        const unsigned long long m = __ballot(in);
-                        if (!MIXED) {
-                            if (in) W.q[(qh + qn + lane_prefix(m)) & (kQ - 1)] = make_uint2(jk, code);
-                            qn += __popcll(m);
-                            if (qn >= 64) flush(false);
-                        } else {
-                            const bool far = r2 >= rfar2;
-                            const unsigned long long mf = __ballot(in && far), mn = m & ~mf;
-                            if (in) {
-                                const unsigned v = (unsigned)(far ? fh + fn : qh + qn) + lane_prefix(far ? mf : mn);
-                                W.q[(v & (kQ - 1)) | (far ? kQ : 0)] = make_uint2(jk, code);
-                            }
-                            qn += __popcll(mn);
-                            fn += __popcll(mf);
-                            if (qn >= 64 || fn >= 64) flush(false);
-                        }
+                    // two candidates per step: independent dependency chains
+                    int k = 0;
+                    for (; k + 1 < nkeep; k += 2) {
+                        const double4 sa = W.stage[k], sb = W.stage[k + 1]; // broadcast reads
+                        const double a0 = y0 - sa.x, a1 = y1 - sa.y, a2 = y2 - sa.z;
+                        const double b0 = y0 - sb.x, b1 = y1 - sb.y, b2 = y2 - sb.z;
+                        const double ra2 = a0 * a0 + a1 * a1 + a2 * a2;
+                        const double rb2 = b0 * b0 + b1 * b1 + b2 * b2;
+                        append((unsigned)__double_as_longlong(sa.w), ra2, code);
+                        append((unsigned)__double_as_longlong(sb.w), rb2, code);
+                    }
+                    if (k < nkeep) {
+                        const double4 sa = W.stage[k];
+                        const double a0 = y0 - sa.x, a1 = y1 - sa.y, a2 = y2 - sa.z;
+                        append((unsigned)__double_as_longlong(sa.w), a0 * a0 + a1 * a1 + a2 * a2, code);
                     }
                     __builtin_amdgcn_wave_barrier();
                 }
@@ -392,18 +414,17 @@ __device__ __forceinline__ void pair_body(
         atomicAdd(&u[o + 1], W.acc[lane][1]);
         atomicAdd(&u[o + 2], W.acc[lane][2]);
     }
-    if (lane == 0 && counters && (n64 | n32)) {
-        atomicAdd(&counters[0], n64);
-        atomicAdd(&counters[1], n32);
-    }
+    // per-item statistics: one plain store each (same-address atomics from
+    // every wavefront would serialize in L2)
+    if (lane == 0) stats[item] = make_uint4(n64, n32, ntest, nread);
 }
 
 #define PB_PAIR_PARAMS                                                                                 \
-    const int *__restrict__ cstart, const double4 *__restrict__ spos, const double4 *__restrict__ sfrc,   \
-        const double4 *__restrict__ tpos, const int *__restrict__ tperm, int nt, int nsplit, pb_cells cells, \
-        pb_box box, const pb_kparams *__restrict__ kpp, double *__restrict__ u,                           \
-        unsigned long long *__restrict__ counters
-#define PB_PAIR_FWD cstart, spos, sfrc, tpos, tperm, nt, nsplit, cells, box, kpp, u, counters
+    const int2 *__restrict__ tiles, const int *__restrict__ ntiles_ptr, const int *__restrict__ cstart,  \
+        const double4 *__restrict__ spos, const double4 *__restrict__ sfrc,                              \
+        const double4 *__restrict__ tpos, const int *__restrict__ tperm, int nsplit, pb_cells cells,      \
+        pb_box box, const pb_kparams *__restrict__ kpp, double *__restrict__ u, uint4 *__restrict__ stats
+#define PB_PAIR_FWD tiles, ntiles_ptr, cstart, spos, sfrc, tpos, tperm, nsplit, cells, box, kpp, u, stats
 
 // The table path fits 4 wavefronts per SIMD; the closed forms need more
 // registers and run with the compiler's own choice.
@@ -423,29 +444,33 @@ __global__ __launch_bounds__(64 * kWaves) void k_pair_direct(PB_PAIR_PARAMS)
 
 int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp,
                 size_t table_doubles, const pb_box &box, int ns, int nt, double *u, bool accumulate,
-                int precision)
+                bool far32)
 {
     (void)ns;
     int st;
-    if ((st = pb_reserve(c, &c->counters, 4 * sizeof(unsigned long long)))) return st;
-    if ((st = pb_hip_check(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), s),
-                           "memset counters")))
-        return st;
+    c->stat_items = 0;
     if (nt <= 0) return PB_OK;
     if (!accumulate &&
         (st = pb_hip_check(c, hipMemsetAsync(u, 0, 3 * (size_t)nt * sizeof(double), s), "memset u")))
         return st;
-    const int ntiles = (nt + 63) / 64;
-    const int nsplit = 8;
-    const unsigned blocks = (unsigned)(((long long)ntiles * nsplit + kWaves - 1) / kWaves);
-    pb_mark(c, s, PB_NPHASE - 1); // the pair-kernel interval covers only this launch
-    const bool mixed = precision == PB_PREC_MIXED;
+    pb_mark(c, s, PB_NPHASE - 1); // the pair phase: tile list and pair kernel
+    int max_tiles = 0;
+    if ((st = pb_build_tiles(c, s, cells, nt, &max_tiles))) return st;
+    const int *ntiles = (const int *)c->qoff.ptr + cells.nkey_t / 16;
+    // wavefronts per tile: about two rounds of resident wavefronts (16 per CU)
+    // in total; each extra wavefront repeats the tile set-up
+    const double tiles_est = std::max(1.0, nt / 48.0);
+    const int nsplit = (int)std::min(16.0, std::max(1.0, std::round(2.25 * 16 * c->cu_count / tiles_est)));
+    const unsigned blocks = (unsigned)(((long long)max_tiles * nsplit + kWaves - 1) / kWaves);
+    if ((st = pb_reserve(c, &c->counters, sizeof(uint4) * (size_t)blocks * kWaves))) return st;
+    c->stat_items = (long long)blocks * kWaves;
+    const bool mixed = far32;
     const size_t ntab = table_doubles; // 0: closed forms per pair
     const size_t lds = ntab ? ntab * sizeof(double) + (mixed ? ntab * sizeof(float) : 0) : 0;
 #define PB_PAIR_ARGS                                                                                   \
-    (const int *)c->cell_start.ptr, (const double4 *)c->spos.ptr, (const double4 *)c->sfrc.ptr,         \
-        (const double4 *)c->tpos.ptr, (const int *)c->tperm.ptr, nt, nsplit, cells, box, kp, u,         \
-        (unsigned long long *)c->counters.ptr
+    (const int2 *)c->tiles.ptr, ntiles, (const int *)c->cell_start.ptr, (const double4 *)c->spos.ptr,   \
+        (const double4 *)c->sfrc.ptr, (const double4 *)c->tpos.ptr, (const int *)c->tperm.ptr, nsplit,    \
+        cells, box, kp, u, (uint4 *)c->counters.ptr
     dim3 gd(blocks), bd(64 * kWaves);
     if (mixed && ntab) hipLaunchKernelGGL((k_pair_table<true>), gd, bd, lds, s, PB_PAIR_ARGS);
     else if (mixed) hipLaunchKernelGGL((k_pair_direct<true>), gd, bd, 0, s, PB_PAIR_ARGS);

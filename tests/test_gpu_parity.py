@@ -148,18 +148,28 @@ def test_velocity_fp64_matches_oracle(ctx, case):
     assert relL2(ur, wr) < 1e-12
 
 
-def test_velocity_mixed_meets_tolerance(ctx):
+@pytest.mark.parametrize("kernel", ["table", "table_far32", "direct"])
+def test_velocity_mixed_meets_tolerance(ctx, kernel):
     import torch
     L = (1.0, 1.0, 1.0)
     x, f, y = _system(9, 4000, 3000, L)
     p = pb.Params(L=L, alpha=10.0, eps=1e-3, grid=(32, 32, 32), P=12, tol=1e-7).tune(4000, 3000)
     want = po.velocity_ewald(y, x, f, L, p.alpha, p.d, p.eps, p.rc, p.grid, (p.P,) * 3, p.m)
-    pm = p.copy(precision="mixed")
+    pm = p.copy(precision="mixed", kernel=kernel)
     u = ctx.velocity(pm, dev(x), dev(f), dev(y)).cpu().numpy()
     torch.cuda.synchronize()
     assert relL2(u, want) < 1e-6
     n64, n32 = ctx.last_pair_counts()
-    assert n32 > n64 > 0
+    n64s, n32s, tested, read = ctx.last_pair_stats()
+    assert (n64s, n32s) == (n64, n32) and read >= tested > 0
+    if kernel == "table":  # FP64 table for every pair
+        assert n64 > 0 and n32 == 0
+    else:  # pairs beyond max(3d, 8e) in FP32
+        assert n32 > n64 > 0
+    # every in-range pair is evaluated once whatever the precision split
+    ur = ctx.velocity_real(p, dev(x), dev(f), dev(y))
+    torch.cuda.synchronize()
+    assert sum(ctx.last_pair_counts()) == n64 + n32
 
 
 def test_velocity_edge_cases(ctx):
