@@ -362,7 +362,7 @@ int pb_destroy(pb_context *c)
     (void)hipSetDevice(c->device);
     pb_buffer *bufs[] = {&c->cell_count, &c->cell_start, &c->cell_of_s, &c->rank_s, &c->cell_of_t,
                          &c->rank_t, &c->tcell_count, &c->tcell_start, &c->spos, &c->sfrc, &c->tpos,
-                         &c->tperm, &c->counters, &c->scan_tmp, &c->qcount, &c->qoff, &c->tiles, &c->grid,
+                         &c->tperm, &c->counters, &c->scan_tmp, &c->qcount, &c->qoff, &c->tiles, &c->grid, &c->spec,
                          &c->twiddle, &c->axis_fac, &c->dense_tmp, &c->kparams, &c->table};
     for (pb_buffer *b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);

@@ -8,7 +8,10 @@
 //   sorted targets   tpos[t] = (x, y, z, 0), tperm[t] = original index, in
 //                    brick-Morton order; tiles[k] = (first, count <= 64)
 //                    within one quarter brick (2x2x4 cells)
-//   k-space grids    3 complex fields, element (i0, i1, i2) at (i0*n1 + i1)*n2 + i2
+//   k-space grids    real grid: (fx, fy, fz, 0) then (ux, uy, uz, 0) per point,
+//                    point (i0, i1, i2) at (i0*n1 + i1)*n2 + i2 (T4);
+//                    half spectrum: 3 fields of n0 x n1 x (n2/2 + 1) complex,
+//                    mode (k0, k1, k2) at (k0*n1 + k1)*(n2/2 + 1) + k2
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -126,7 +129,7 @@ struct pb_context {
     pb_buffer spos, sfrc, tpos, tperm, counters, scan_tmp;
     pb_buffer qcount, qoff, tiles; // target tiles of the pair kernel: int2 (first, count)
     // k space
-    pb_buffer grid, twiddle, axis_fac;
+    pb_buffer grid, spec, twiddle, axis_fac; // real grid (T4 per point), half spectrum
     int twiddle_n = 0;
     int twiddle_prec = -1;
     // dense / shells

@@ -19,8 +19,14 @@ namespace {
 constexpr int kPmax = 32;
 
 template <typename T> struct cplx;
-template <> struct cplx<double> { using type = double2; };
-template <> struct cplx<float> { using type = float2; };
+template <> struct cplx<double> {
+    using type = double2;
+    using type4 = double4;
+};
+template <> struct cplx<float> {
+    using type = float2;
+    using type4 = float4;
+};
 
 __device__ __forceinline__ int floordiv(int a, int b)
 {
@@ -101,7 +107,7 @@ template <typename T>
 __global__ __launch_bounds__(64 * kSpreadWaves) void k_spread_blocks(
     const int *__restrict__ cstart, const double4 *__restrict__ spos,
     const double4 *__restrict__ sfrc, pb_cells cells, pb_box box, pb_kgrid g,
-    T *__restrict__ grid)
+    typename cplx<T>::type4 *__restrict__ grid4)
 {
     __shared__ spread_lds<T> lds[kSpreadWaves];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -166,14 +172,13 @@ __global__ __launch_bounds__(64 * kSpreadWaves) void k_spread_blocks(
     }
     __builtin_amdgcn_wave_barrier();
     if (qn > 0) spread_flush<T>(W, qh, qn, lane, spos, sfrc, box, g, g0, lidx, a0, a1, a2);
-    const size_t idx = 2 * (((size_t)(g0[0] + lidx[0]) * g.n[1] + (g0[1] + lidx[1])) * g.n[2] + (g0[2] + lidx[2]));
-    const size_t fs = 2 * (size_t)g.ng;
-    grid[idx] = a0;
-    grid[idx + 1] = (T)0;
-    grid[fs + idx] = a1;
-    grid[fs + idx + 1] = (T)0;
-    grid[2 * fs + idx] = a2;
-    grid[2 * fs + idx + 1] = (T)0;
+    const size_t idx = ((size_t)(g0[0] + lidx[0]) * g.n[1] + (g0[1] + lidx[1])) * g.n[2] + (g0[2] + lidx[2]);
+    typename cplx<T>::type4 o;
+    o.x = a0;
+    o.y = a1;
+    o.z = a2;
+    o.w = (T)0;
+    grid4[idx] = o;
 }
 
 // ---- interpolation (gather) --------------------------------------------------
@@ -194,7 +199,7 @@ template <typename T> __device__ __forceinline__ T wave_sum(T v)
 template <typename T, int P>
 __global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, const double4 *__restrict__ tpos,
                                                             const int *__restrict__ tperm, pb_kgrid g,
-                                                            const T *__restrict__ grid,
+                                                            const typename cplx<T>::type4 *__restrict__ grid4,
                                                             double *__restrict__ u, int accumulate)
 {
     __shared__ T wts[kInterpWaves][3][P];
@@ -217,7 +222,6 @@ __global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, const doub
         ix[wave][a][p] = j;
     }
     __builtin_amdgcn_wave_barrier();
-    const size_t fs = 2 * (size_t)g.ng;
     T a0 = 0, a1 = 0, a2 = 0;
     constexpr int kIter = (P * P * P + 63) / 64;
 #pragma unroll 9
@@ -225,11 +229,12 @@ __global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, const doub
         const int k = it * 64 + lane;
         if (P * P * P % 64 == 0 || k < P * P * P) {
             const int pz = k % P, py = (k / P) % P, px = k / (P * P);
-            const size_t off = 2 * (((size_t)ix[wave][0][px] * g.n[1] + ix[wave][1][py]) * g.n[2] + ix[wave][2][pz]);
+            const size_t off = ((size_t)ix[wave][0][px] * g.n[1] + ix[wave][1][py]) * g.n[2] + ix[wave][2][pz];
             const T wt = wts[wave][0][px] * wts[wave][1][py] * wts[wave][2][pz];
-            a0 += wt * grid[off];
-            a1 += wt * grid[fs + off];
-            a2 += wt * grid[2 * fs + off];
+            const typename cplx<T>::type4 v = grid4[off];
+            a0 += wt * v.x;
+            a1 += wt * v.y;
+            a2 += wt * v.z;
         }
     }
     a0 = wave_sum(a0);
@@ -254,7 +259,7 @@ __global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, const doub
 template <typename T>
 __global__ __launch_bounds__(256) void k_interp_any(int nt, const double4 *__restrict__ tpos,
                                                     const int *__restrict__ tperm, pb_kgrid g,
-                                                    const T *__restrict__ grid,
+                                                    const typename cplx<T>::type4 *__restrict__ grid4,
                                                     double *__restrict__ u, int accumulate)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -263,7 +268,6 @@ __global__ __launch_bounds__(256) void k_interp_any(int nt, const double4 *__res
     const double yc[3] = {y.x, y.y, y.z};
     int f0[3];
     for (int a = 0; a < 3; a++) f0[a] = (int)floor(yc[a] * g.inv_h[a] - 0.5 * g.P[a]) + 1;
-    const size_t fs = 2 * (size_t)g.ng;
     double a0 = 0, a1 = 0, a2 = 0;
     for (int px = 0; px < g.P[0]; px++) {
         const double dxx = (double)(f0[0] + px) * g.h[0] - yc[0];
@@ -277,10 +281,10 @@ __global__ __launch_bounds__(256) void k_interp_any(int nt, const double4 *__res
                 const double dxz = (double)(f0[2] + pz) * g.h[2] - yc[2];
                 const double wt = wxy * exp(-dxz * dxz / g.eta);
                 const int jz = ((f0[2] + pz) % g.n[2] + g.n[2]) % g.n[2];
-                const size_t idx = 2 * (((size_t)jx * g.n[1] + jy) * g.n[2] + jz);
-                a0 += wt * (double)grid[idx];
-                a1 += wt * (double)grid[fs + idx];
-                a2 += wt * (double)grid[2 * fs + idx];
+                const typename cplx<T>::type4 v = grid4[((size_t)jx * g.n[1] + jy) * g.n[2] + jz];
+                a0 += wt * (double)v.x;
+                a1 += wt * (double)v.y;
+                a2 += wt * (double)v.z;
             }
         }
     }
@@ -299,7 +303,7 @@ __global__ __launch_bounds__(256) void k_interp_any(int nt, const double4 *__res
 
 template <typename T>
 void launch_interp(hipStream_t s, int nt, const pb_kgrid &g, const void *tpos, const void *tperm,
-                   const T *grid, double *u, int acc)
+                   const typename cplx<T>::type4 *grid, double *u, int acc)
 {
     dim3 gd((nt + kInterpWaves - 1) / kInterpWaves), bd(64 * kInterpWaves);
     dim3 gd1((nt + 255) / 256), bd1(256);
@@ -319,169 +323,473 @@ void launch_interp(hipStream_t s, int nt, const pb_kgrid &g, const void *tpos, c
     }
 }
 
-// Batched radix-2 Stockham FFT of length N along one axis, B pencils per
-// workgroup staged in LDS (ping-pong).  Forward uses e^{-2 pi i k/N}.
-template <typename T, int N>
-__global__ __launch_bounds__(256) void k_fft_axis(typename cplx<T>::type *__restrict__ data,
-                                                  int axis, int n0, int n1, int n2, int B,
-                                                  const typename cplx<T>::type *__restrict__ tw,
-                                                  int inverse, long long field_stride)
+// ---- FFT ---------------------------------------------------------------------
+// Pencils of length N held in LDS (pencil stride N + 1 against bank
+// conflicts) are transformed by Stockham stages of radix 8, 4, 2 (largest
+// first).  A stage reads all its butterfly inputs into registers, waits for
+// the workgroup, and writes the outputs in place, so one LDS buffer suffices.
+// Element budget per workgroup: E complex values.
+template <typename T> struct kbudget;
+template <> struct kbudget<float> { static constexpr int E = 4096; };
+template <> struct kbudget<double> { static constexpr int E = 2048; };
+
+template <typename C> __device__ __forceinline__ C cmul(C a, C b)
 {
-    using C = typename cplx<T>::type;
-    extern __shared__ unsigned char smem_raw[];
-    C *X = reinterpret_cast<C *>(smem_raw);
-    C *Y = X + (size_t)N * B;
-    C *base = data + (size_t)blockIdx.y * field_stride;
-    const long long pencil0 = (long long)blockIdx.x * B;
-    const int tid = threadIdx.x;
-    // addressing of element (p, b): pencil q = pencil0 + b
-    auto addr = [&](int p, int b) -> size_t {
-        long long q = pencil0 + b;
-        if (axis == 2) return (size_t)q * N + p;
-        if (axis == 1) {
-            long long i0 = q / n2, i2 = q % n2;
-            return (size_t)(i0 * (long long)n1 * n2 + (long long)p * n2 + i2);
-        }
-        return (size_t)((long long)p * n1 * n2 + q);
-    };
-    const int tot = N * B;
-    if (axis == 2) {
-        for (int idx = tid; idx < tot; idx += blockDim.x) {
-            int b = idx / N, p = idx % N;
-            X[b * N + p] = base[addr(p, b)];
-        }
+    C r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+template <typename C> __device__ __forceinline__ C cadd(C a, C b)
+{
+    C r;
+    r.x = a.x + b.x;
+    r.y = a.y + b.y;
+    return r;
+}
+template <typename C> __device__ __forceinline__ C csub(C a, C b)
+{
+    C r;
+    r.x = a.x - b.x;
+    r.y = a.y - b.y;
+    return r;
+}
+// multiply by -i (forward) or +i (inverse)
+template <typename C> __device__ __forceinline__ C mul_mi(C a, bool inv)
+{
+    C r;
+    if (inv) {
+        r.x = -a.y;
+        r.y = a.x;
     } else {
-        for (int idx = tid; idx < tot; idx += blockDim.x) {
-            int p = idx / B, b = idx % B;
-            X[b * N + p] = base[addr(p, b)];
+        r.x = a.y;
+        r.y = -a.x;
+    }
+    return r;
+}
+
+// In-register DFT of length R (2, 4, 8): V[q] = sum_m v[m] e^{-+2 pi i m q / R}.
+template <typename C, int R> __device__ __forceinline__ void dft(C *v, bool inv)
+{
+    using T = decltype(v[0].x);
+    if constexpr (R == 2) {
+        const C a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        const C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        const C s13 = cadd(v[1], v[3]), d13 = mul_mi(csub(v[1], v[3]), inv);
+        v[0] = cadd(s02, s13);
+        v[2] = csub(s02, s13);
+        v[1] = cadd(d02, d13);
+        v[3] = csub(d02, d13);
+    } else {
+        C e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        dft<C, 4>(e, inv);
+        dft<C, 4>(o, inv);
+        const T h = (T)0.70710678118654752440084436210484904;
+        C w1, w3;
+        // W8^1 = (1 -+ i)/sqrt2, W8^2 = -+i, W8^3 = (-1 -+ i)/sqrt2
+        w1.x = h * (o[1].x + (inv ? -o[1].y : o[1].y));
+        w1.y = h * (o[1].y + (inv ? o[1].x : -o[1].x));
+        const C w2 = mul_mi(o[2], inv);
+        w3.x = h * (-o[3].x + (inv ? -o[3].y : o[3].y));
+        w3.y = h * (-o[3].y + (inv ? o[3].x : -o[3].x));
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], w1);
+        v[5] = csub(e[1], w1);
+        v[2] = cadd(e[2], w2);
+        v[6] = csub(e[2], w2);
+        v[3] = cadd(e[3], w3);
+        v[7] = csub(e[3], w3);
+    }
+}
+
+// One Stockham stage of radix R after NS points are combined, for the E / N
+// pencils of the workgroup.  tw[q] = e^{-2 pi i q / N} (LDS).
+template <typename C, int N, int R, int NS, int E>
+__device__ __forceinline__ void fft_stage(C *buf, bool inv, const C *tw)
+{
+    constexpr int NP = N + 1;
+    constexpr int NB = E / R;
+    constexpr int KB = (NB + 255) / 256;
+    C v[KB][R];
+#pragma unroll
+    for (int kb = 0; kb < KB; kb++) {
+        const int b = threadIdx.x + 256 * kb;
+        if (NB % 256 == 0 || b < NB) {
+            const int p = b / (N / R), t = b % (N / R);
+#pragma unroll
+            for (int m = 0; m < R; m++) v[kb][m] = buf[p * NP + t + m * (N / R)];
         }
     }
     __syncthreads();
-    const int half = N / 2;
-    for (int Ns = 1; Ns < N; Ns <<= 1) {
-        for (int idx = tid; idx < B * half; idx += blockDim.x) {
-            int b = idx / half, k = idx % half;
-            C a = X[b * N + k], c = X[b * N + k + half];
-            int km = k & (Ns - 1);
-            C w = tw[km * (N / (2 * Ns))];
-            if (inverse) w.y = -w.y;
-            C cw;
-            cw.x = c.x * w.x - c.y * w.y;
-            cw.y = c.x * w.y + c.y * w.x;
-            int o = ((k - km) << 1) + km;
-            C s, d;
-            s.x = a.x + cw.x;
-            s.y = a.y + cw.y;
-            d.x = a.x - cw.x;
-            d.y = a.y - cw.y;
-            Y[b * N + o] = s;
-            Y[b * N + o + Ns] = d;
+#pragma unroll
+    for (int kb = 0; kb < KB; kb++) {
+        const int b = threadIdx.x + 256 * kb;
+        if (NB % 256 == 0 || b < NB) {
+            const int p = b / (N / R), t = b % (N / R);
+            const int j = t % NS;
+            if (NS > 1) {
+#pragma unroll
+                for (int m = 1; m < R; m++) {
+                    C w = tw[(m * j * (N / (NS * R))) % N];
+                    if (inv) w.y = -w.y;
+                    v[kb][m] = cmul(v[kb][m], w);
+                }
+            }
+            dft<C, R>(v[kb], inv);
+            const int o = p * NP + (t / NS) * NS * R + j;
+#pragma unroll
+            for (int q = 0; q < R; q++) buf[o + q * NS] = v[kb][q];
         }
-        __syncthreads();
-        C *t = X;
-        X = Y;
-        Y = t;
     }
-    if (axis == 2) {
-        for (int idx = tid; idx < tot; idx += blockDim.x) {
-            int b = idx / N, p = idx % N;
-            base[addr(p, b)] = X[b * N + p];
+    __syncthreads();
+}
+
+template <typename C, int N, int NS, int E>
+__device__ __forceinline__ void fft_stages(C *buf, bool inv, const C *tw)
+{
+    if constexpr (NS < N) {
+        constexpr int REM = N / NS;
+        constexpr int R = (REM % 8 == 0) ? 8 : ((REM % 4 == 0) ? 4 : 2);
+        fft_stage<C, N, R, NS, E>(buf, inv, tw);
+        fft_stages<C, N, NS * R, E>(buf, inv, tw);
+    }
+}
+
+template <typename C, int N>
+__device__ __forceinline__ void load_twiddles(C *tw, const C *__restrict__ gtw)
+{
+    for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = gtw[k];
+}
+
+// Forward z pass, real to complex: pencils a = (i0, i1) and b = (i0, i1 + 1)
+// of the real grid R (T4: fx, fy, fz) are transformed as the three complex
+// pencils fx_a + i fy_a, fx_b + i fy_b, fz_a + i fz_b; the half spectra
+// (kz = 0 .. N/2) of the six real pencils are written to S.
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_fft_z_r2c(const typename cplx<T>::type4 *__restrict__ R4,
+                                                   typename cplx<T>::type *__restrict__ S, int n0,
+                                                   int n1, const typename cplx<T>::type *__restrict__ gtw)
+{
+    using C = typename cplx<T>::type;
+    constexpr int NP = N + 1, H = N / 2 + 1;
+    constexpr int PP = kbudget<T>::E / (3 * N) > 0 ? kbudget<T>::E / (3 * N) : 1;
+    constexpr int E = 3 * PP * N;
+    __shared__ C buf[3 * PP * NP];
+    __shared__ C tw[N];
+    load_twiddles<C, N>(tw, gtw);
+    const int npair = n0 * (n1 / 2);
+    const int pair0 = blockIdx.x * PP;
+    for (int idx = threadIdx.x; idx < 2 * PP * N; idx += 256) {
+        const int pp = idx / (2 * N), r = idx % (2 * N), side = r / N, z = r % N;
+        const int pr = pair0 + pp;
+        typename cplx<T>::type4 f;
+        if (pr < npair) {
+            const size_t pos = (size_t)(2 * pr + side) * N + z; // pencil 2 pr + side = (i0, i1)
+            f = R4[pos];
+        } else {
+            f.x = f.y = f.z = f.w = (T)0;
         }
-    } else {
-        for (int idx = tid; idx < tot; idx += blockDim.x) {
-            int p = idx / B, b = idx % B;
-            base[addr(p, b)] = X[b * N + p];
+        C a;
+        a.x = f.x;
+        a.y = f.y;
+        buf[(3 * pp + side) * NP + z] = a;
+        if (side == 0)
+            buf[(3 * pp + 2) * NP + z].x = f.z;
+        else
+            buf[(3 * pp + 2) * NP + z].y = f.z;
+    }
+    __syncthreads();
+    fft_stages<C, N, 1, E>(buf, false, tw);
+    const size_t fs = (size_t)n0 * n1 * H;
+    for (int idx = threadIdx.x; idx < PP * 2 * H; idx += 256) {
+        const int pp = idx / (2 * H), r = idx % (2 * H), side = r / H, k = r % H;
+        const int pr = pair0 + pp;
+        if (pr >= npair) continue;
+        const int km = (N - k) & (N - 1);
+        const C z1 = buf[(3 * pp + side) * NP + k], z1m = buf[(3 * pp + side) * NP + km];
+        const C z3 = buf[(3 * pp + 2) * NP + k], z3m = buf[(3 * pp + 2) * NP + km];
+        // X = (Z(k) + conj Z(-k)) / 2, Y = (Z(k) - conj Z(-k)) / (2i)
+        C fx, fy, fz;
+        fx.x = (T)0.5 * (z1.x + z1m.x);
+        fx.y = (T)0.5 * (z1.y - z1m.y);
+        fy.x = (T)0.5 * (z1.y + z1m.y);
+        fy.y = (T)0.5 * (z1m.x - z1.x);
+        if (side == 0) {
+            fz.x = (T)0.5 * (z3.x + z3m.x);
+            fz.y = (T)0.5 * (z3.y - z3m.y);
+        } else {
+            fz.x = (T)0.5 * (z3.y + z3m.y);
+            fz.y = (T)0.5 * (z3m.x - z3.x);
+        }
+        const size_t o = (size_t)(2 * pr + side) * H + k;
+        S[o] = fx;
+        S[fs + o] = fy;
+        S[2 * fs + o] = fz;
+    }
+}
+
+// y pass (forward or inverse) on S: pencils along i1 of length N = n1 for
+// every (field, i0, kz), B consecutive (field, i0, kz) per workgroup.
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_fft_y(typename cplx<T>::type *__restrict__ S, int n0, int h2,
+                                               int inverse, const typename cplx<T>::type *__restrict__ gtw)
+{
+    using C = typename cplx<T>::type;
+    constexpr int NP = N + 1;
+    constexpr int B = kbudget<T>::E / N > 0 ? kbudget<T>::E / N : 1;
+    constexpr int E = B * N;
+    __shared__ C buf[B * NP];
+    __shared__ C tw[N];
+    load_twiddles<C, N>(tw, gtw);
+    const long long npen = 3LL * n0 * h2;
+    const long long q0 = (long long)blockIdx.x * B;
+    // pencil q = (f i0) * h2 + kz; element i1 at ((f i0) * N + i1) * h2 + kz
+    for (int idx = threadIdx.x; idx < E; idx += 256) {
+        const int i1 = idx / B, g = idx % B;
+        const long long q = q0 + g;
+        C v;
+        v.x = v.y = (T)0;
+        if (q < npen) {
+            const long long fi = q / h2, kz = q % h2;
+            v = S[((size_t)fi * N + i1) * h2 + kz];
+        }
+        buf[g * NP + i1] = v;
+    }
+    __syncthreads();
+    fft_stages<C, N, 1, E>(buf, inverse != 0, tw);
+    for (int idx = threadIdx.x; idx < E; idx += 256) {
+        const int i1 = idx / B, g = idx % B;
+        const long long q = q0 + g;
+        if (q < npen) {
+            const long long fi = q / h2, kz = q % h2;
+            S[((size_t)fi * N + i1) * h2 + kz] = buf[g * NP + i1];
         }
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_multiply(typename cplx<T>::type *__restrict__ grid,
-                                                  pb_kgrid g, double k0, double k1, double k2,
-                                                  double c, double a2, const double *__restrict__ fac)
+struct kmult {
+    double k[3];     // 2 pi / L per axis
+    double c;        // (1 - e^{-a^2 d^2/4}) / a^2
+    double a2;       // a^2
+    double inv_ng;   // 1 / (n0 n1 n2)
+    int n[3];
+};
+
+// x pass with the Brinkman projection between the forward and inverse
+// transforms: pencils along i0 of the three fields for B consecutive
+// (i1, kz).  Each mode gets
+//   U = b(k) (|k|^2 F - (k (k.F) + k' (k'.F)) / 2),
+//   b = (1 + c |k|^2) fac0 fac1 fac2 / (|k|^2 (|k|^2 + a^2) n0 n1 n2),
+// with Nyquist frequencies taken as -n/2 and k' = k with its Nyquist
+// components negated: averaging over their sign keeps the spectrum Hermitian,
+// which equals the real part of the full complex transform (the
+// reference's convention).  k = 0 is set to zero.
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_fft_x_mult(typename cplx<T>::type *__restrict__ S, int n1, int h2,
+                                                    kmult km, const double *__restrict__ fac,
+                                                    const typename cplx<T>::type *__restrict__ gtw)
 {
     using C = typename cplx<T>::type;
-    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= g.ng) return;
-    int q2 = (int)(q % g.n[2]);
-    int q1 = (int)((q / g.n[2]) % g.n[1]);
-    int q0 = (int)(q / ((long long)g.n[2] * g.n[1]));
-    // signed frequency, Nyquist taken as -n/2 (the reference's centred shift)
-    int f0 = (2 * q0 >= g.n[0]) ? q0 - g.n[0] : q0;
-    int f1 = (2 * q1 >= g.n[1]) ? q1 - g.n[1] : q1;
-    int f2 = (2 * q2 >= g.n[2]) ? q2 - g.n[2] : q2;
-    double kx = k0 * f0, ky = k1 * f1, kz = k2 * f2;
-    double kn = kx * kx + ky * ky + kz * kz;
-    C *F0 = grid, *F1 = grid + g.ng, *F2 = grid + 2 * g.ng;
-    if (kn == 0.0) {
-        C z;
-        z.x = 0;
-        z.y = 0;
-        F0[q] = z;
-        F1[q] = z;
-        F2[q] = z;
-        return;
+    constexpr int NP = N + 1;
+    constexpr int B = kbudget<T>::E / (3 * N) > 0 ? kbudget<T>::E / (3 * N) : 1;
+    constexpr int E = 3 * B * N;
+    __shared__ C buf[3 * B * NP];
+    __shared__ C tw[N];
+    load_twiddles<C, N>(tw, gtw);
+    const long long npen = (long long)n1 * h2;
+    const long long q0 = (long long)blockIdx.x * B;
+    const size_t fs = (size_t)N * npen;
+    // element (field f, i0) of pencil q = i1 * h2 + kz at (f * N + i0) * npen + q
+    for (int idx = threadIdx.x; idx < E; idx += 256) {
+        const int f = idx / (B * N), r = idx % (B * N), i0 = r / B, g = r % B;
+        const long long q = q0 + g;
+        C v;
+        v.x = v.y = (T)0;
+        if (q < npen) v = S[f * fs + (size_t)i0 * npen + q];
+        buf[(3 * g + f) * NP + i0] = v;
     }
-    // fac holds e^{-(d^2/4 - eta/2) k_a^2} per axis and frequency
-    double b = (1.0 + c * kn) * fac[q0] * fac[g.n[0] + q1] * fac[g.n[0] + g.n[1] + q2] /
-               (kn * (kn + a2) * (double)g.ng);
-    C v0 = F0[q], v1 = F1[q], v2 = F2[q];
-    T bk = (T)b, tkn = (T)kn, tx = (T)kx, ty = (T)ky, tz = (T)kz;
-    T kr = tx * v0.x + ty * v1.x + tz * v2.x;
-    T ki = tx * v0.y + ty * v1.y + tz * v2.y;
-    C o0, o1, o2;
-    o0.x = bk * (tkn * v0.x - tx * kr);
-    o0.y = bk * (tkn * v0.y - tx * ki);
-    o1.x = bk * (tkn * v1.x - ty * kr);
-    o1.y = bk * (tkn * v1.y - ty * ki);
-    o2.x = bk * (tkn * v2.x - tz * kr);
-    o2.y = bk * (tkn * v2.y - tz * ki);
-    F0[q] = o0;
-    F1[q] = o1;
-    F2[q] = o2;
+    __syncthreads();
+    fft_stages<C, N, 1, E>(buf, false, tw);
+    for (int idx = threadIdx.x; idx < B * N; idx += 256) {
+        const int g = idx / N, i0 = idx % N;
+        const long long q = q0 + g;
+        if (q >= npen) continue;
+        const int i1 = (int)(q / h2), kz = (int)(q % h2);
+        const int n2 = km.n[2];
+        const int f0 = (2 * i0 >= N) ? i0 - N : i0;
+        const int f1 = (2 * i1 >= n1) ? i1 - n1 : i1;
+        const int f2 = (2 * kz >= n2) ? kz - n2 : kz;
+        const double kx = km.k[0] * f0, ky = km.k[1] * f1, kz_ = km.k[2] * f2;
+        const double kn = kx * kx + ky * ky + kz_ * kz_;
+        C *F0 = &buf[(3 * g) * NP + i0], *F1 = F0 + NP, *F2 = F1 + NP;
+        if (kn == 0.0) {
+            C z;
+            z.x = z.y = (T)0;
+            *F0 = z;
+            *F1 = z;
+            *F2 = z;
+            continue;
+        }
+        const double b = (1.0 + km.c * kn) * fac[i0] * fac[N + i1] * fac[N + n1 + kz] * km.inv_ng /
+                         (kn * (kn + km.a2));
+        const T tx = (T)kx, ty = (T)ky, tz = (T)kz_;
+        const T sx = (2 * i0 == N) ? -tx : tx, sy = (2 * i1 == n1) ? -ty : ty, sz = (2 * kz == n2) ? -tz : tz;
+        const C v0 = *F0, v1 = *F1, v2 = *F2;
+        const T kr = tx * v0.x + ty * v1.x + tz * v2.x, ki = tx * v0.y + ty * v1.y + tz * v2.y;
+        const T sr = sx * v0.x + sy * v1.x + sz * v2.x, si = sx * v0.y + sy * v1.y + sz * v2.y;
+        const T bk = (T)b, tkn = (T)kn, hf = (T)0.5;
+        C o0, o1, o2;
+        o0.x = bk * (tkn * v0.x - hf * (tx * kr + sx * sr));
+        o0.y = bk * (tkn * v0.y - hf * (tx * ki + sx * si));
+        o1.x = bk * (tkn * v1.x - hf * (ty * kr + sy * sr));
+        o1.y = bk * (tkn * v1.y - hf * (ty * ki + sy * si));
+        o2.x = bk * (tkn * v2.x - hf * (tz * kr + sz * sr));
+        o2.y = bk * (tkn * v2.y - hf * (tz * ki + sz * si));
+        *F0 = o0;
+        *F1 = o1;
+        *F2 = o2;
+    }
+    __syncthreads();
+    fft_stages<C, N, 1, E>(buf, true, tw);
+    for (int idx = threadIdx.x; idx < E; idx += 256) {
+        const int f = idx / (B * N), r = idx % (B * N), i0 = r / B, g = r % B;
+        const long long q = q0 + g;
+        if (q < npen) S[f * fs + (size_t)i0 * npen + q] = buf[(3 * g + f) * NP + i0];
+    }
+}
+
+// Inverse z pass, complex to real: the half spectra of pencils a = (i0, i1)
+// and b = (i0, i1 + 1) are extended by U(N - k) = conj U(k) and transformed
+// as ux_a + i uy_a, ux_b + i uy_b, uz_a + i uz_b into the real grid (T4).
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_fft_z_c2r(const typename cplx<T>::type *__restrict__ S,
+                                                   typename cplx<T>::type4 *__restrict__ U4, int n0,
+                                                   int n1, const typename cplx<T>::type *__restrict__ gtw)
+{
+    using C = typename cplx<T>::type;
+    constexpr int NP = N + 1, H = N / 2 + 1;
+    constexpr int PP = kbudget<T>::E / (3 * N) > 0 ? kbudget<T>::E / (3 * N) : 1;
+    constexpr int E = 3 * PP * N;
+    __shared__ C buf[3 * PP * NP];
+    __shared__ C tw[N];
+    load_twiddles<C, N>(tw, gtw);
+    const int npair = n0 * (n1 / 2);
+    const int pair0 = blockIdx.x * PP;
+    const size_t fs = (size_t)n0 * n1 * H;
+    for (int idx = threadIdx.x; idx < PP * 2 * N; idx += 256) {
+        const int pp = idx / (2 * N), r = idx % (2 * N), side = r / N, k = r % N;
+        const int pr = pair0 + pp;
+        C ux, uy, uz;
+        ux.x = ux.y = uy.x = uy.y = uz.x = uz.y = (T)0;
+        if (pr < npair) {
+            const bool up = k >= H;
+            const int kk = up ? N - k : k;
+            const size_t o = (size_t)(2 * pr + side) * H + kk;
+            ux = S[o];
+            uy = S[fs + o];
+            uz = S[2 * fs + o];
+            if (up) {
+                ux.y = -ux.y;
+                uy.y = -uy.y;
+                uz.y = -uz.y;
+            }
+        }
+        C z;
+        z.x = ux.x - uy.y; // ux + i uy
+        z.y = ux.y + uy.x;
+        buf[(3 * pp + side) * NP + k] = z;
+        if (side == 0) { // uz_a enters the real part of the third pencil
+            buf[(3 * pp + 2) * NP + k].x = uz.x;
+            buf[(3 * pp + 2) * NP + k].y = uz.y;
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < PP * N; idx += 256) { // + i uz_b
+        const int pp = idx / N, k = idx % N;
+        const int pr = pair0 + pp;
+        if (pr >= npair) continue;
+        const bool up = k >= H;
+        const int kk = up ? N - k : k;
+        C uz = S[2 * fs + (size_t)(2 * pr + 1) * H + kk];
+        if (up) uz.y = -uz.y;
+        buf[(3 * pp + 2) * NP + k].x -= uz.y;
+        buf[(3 * pp + 2) * NP + k].y += uz.x;
+    }
+    __syncthreads();
+    fft_stages<C, N, 1, E>(buf, true, tw);
+    for (int idx = threadIdx.x; idx < 2 * PP * N; idx += 256) {
+        const int pp = idx / (2 * N), r = idx % (2 * N), side = r / N, z = r % N;
+        const int pr = pair0 + pp;
+        if (pr >= npair) continue;
+        const C a = buf[(3 * pp + side) * NP + z], c3 = buf[(3 * pp + 2) * NP + z];
+        typename cplx<T>::type4 o;
+        o.x = a.x;
+        o.y = a.y;
+        o.z = side == 0 ? c3.x : c3.y;
+        o.w = (T)0;
+        U4[(size_t)(2 * pr + side) * N + z] = o;
+    }
 }
 
 template <typename T, int N>
-void launch_fft(hipStream_t s, typename cplx<T>::type *grid, const pb_kgrid &g, int axis,
-                const typename cplx<T>::type *tw, int inverse)
+void launch_axis(hipStream_t s, int pass, const pb_kgrid &g, void *grid4, void *spec, const kmult &km,
+                 const double *fac, const void *tw)
 {
     using C = typename cplx<T>::type;
-    int B;
-    if (axis == 2) {
-        B = 8;
-        while (B > 1 && (size_t)2 * N * B * sizeof(C) > 65536) B >>= 1;
-    } else {
-        B = 16;
-        while (B > 1 && ((size_t)2 * N * B * sizeof(C) > 65536 || g.n[2] % B)) B >>= 1;
+    using C4 = typename cplx<T>::type4;
+    const int h2 = g.n[2] / 2 + 1;
+    const C *t = (const C *)tw;
+    if (pass == 0 || pass == 4) { // z
+        constexpr int PP = kbudget<T>::E / (3 * N) > 0 ? kbudget<T>::E / (3 * N) : 1;
+        const int npair = g.n[0] * (g.n[1] / 2);
+        dim3 gd((npair + PP - 1) / PP);
+        if (pass == 0)
+            hipLaunchKernelGGL((k_fft_z_r2c<T, N>), gd, dim3(256), 0, s, (const C4 *)grid4, (C *)spec,
+                               g.n[0], g.n[1], t);
+        else
+            hipLaunchKernelGGL((k_fft_z_c2r<T, N>), gd, dim3(256), 0, s, (const C *)spec, (C4 *)grid4,
+                               g.n[0], g.n[1], t);
+    } else if (pass == 1 || pass == 3) { // y
+        constexpr int B = kbudget<T>::E / N > 0 ? kbudget<T>::E / N : 1;
+        const long long npen = 3LL * g.n[0] * h2;
+        hipLaunchKernelGGL((k_fft_y<T, N>), dim3((unsigned)((npen + B - 1) / B)), dim3(256), 0, s, (C *)spec,
+                           g.n[0], h2, pass == 3 ? 1 : 0, t);
+    } else { // x with the multiply
+        constexpr int B = kbudget<T>::E / (3 * N) > 0 ? kbudget<T>::E / (3 * N) : 1;
+        const long long npen = (long long)g.n[1] * h2;
+        hipLaunchKernelGGL((k_fft_x_mult<T, N>), dim3((unsigned)((npen + B - 1) / B)), dim3(256), 0, s,
+                           (C *)spec, g.n[1], h2, km, fac, t);
     }
-    long long npencil = g.ng / N;
-    dim3 grid_dim((unsigned)(npencil / B), 3);
-    size_t lds = (size_t)2 * N * B * sizeof(C);
-    hipLaunchKernelGGL((k_fft_axis<T, N>), grid_dim, dim3(256), lds, s, grid, axis, g.n[0],
-                       g.n[1], g.n[2], B, tw, inverse, (long long)g.ng);
 }
 
+// pass: 0 z forward (r2c), 1 y forward, 2 x forward + multiply + inverse,
+// 3 y inverse, 4 z inverse (c2r)
 template <typename T>
-int fft_dispatch(hipStream_t s, typename cplx<T>::type *grid, const pb_kgrid &g, int axis,
-                 const typename cplx<T>::type *tw, int inverse)
+int fft_pass(hipStream_t s, int pass, const pb_kgrid &g, void *grid4, void *spec, const kmult &km,
+             const double *fac, const void *const *tws)
 {
-    switch (g.n[axis]) {
-    case 2: launch_fft<T, 2>(s, grid, g, axis, tw, inverse); break;
-    case 4: launch_fft<T, 4>(s, grid, g, axis, tw, inverse); break;
-    case 8: launch_fft<T, 8>(s, grid, g, axis, tw, inverse); break;
-    case 16: launch_fft<T, 16>(s, grid, g, axis, tw, inverse); break;
-    case 32: launch_fft<T, 32>(s, grid, g, axis, tw, inverse); break;
-    case 64: launch_fft<T, 64>(s, grid, g, axis, tw, inverse); break;
-    case 128: launch_fft<T, 128>(s, grid, g, axis, tw, inverse); break;
-    case 256: launch_fft<T, 256>(s, grid, g, axis, tw, inverse); break;
-    case 512: launch_fft<T, 512>(s, grid, g, axis, tw, inverse); break;
+    const int ax = (pass == 0 || pass == 4) ? 2 : (pass == 2 ? 0 : 1);
+    const void *tw = tws[ax];
+    switch (g.n[ax]) {
+    case 4: launch_axis<T, 4>(s, pass, g, grid4, spec, km, fac, tw); break;
+    case 8: launch_axis<T, 8>(s, pass, g, grid4, spec, km, fac, tw); break;
+    case 16: launch_axis<T, 16>(s, pass, g, grid4, spec, km, fac, tw); break;
+    case 32: launch_axis<T, 32>(s, pass, g, grid4, spec, km, fac, tw); break;
+    case 64: launch_axis<T, 64>(s, pass, g, grid4, spec, km, fac, tw); break;
+    case 128: launch_axis<T, 128>(s, pass, g, grid4, spec, km, fac, tw); break;
+    case 256: launch_axis<T, 256>(s, pass, g, grid4, spec, km, fac, tw); break;
+    case 512: launch_axis<T, 512>(s, pass, g, grid4, spec, km, fac, tw); break;
     default: return PB_ERR_UNSUPPORTED;
     }
     return PB_OK;
 }
 
-// Twiddles e^{-2 pi i k/n}, k < n/2, for the three axes, stored consecutively
-// at offsets 0, n0/2, n0/2 + n1/2 (computed in long double).
+// Twiddles e^{-2 pi i q/n}, q < n, for the three axes at offsets 0, n0,
+// n0 + n1 (computed in long double).
 template <typename T>
 int upload_twiddles(pb_context *c, hipStream_t s, const pb_kgrid &g)
 {
@@ -489,17 +797,17 @@ int upload_twiddles(pb_context *c, hipStream_t s, const pb_kgrid &g)
     int prec = sizeof(T) == 8 ? 0 : 1;
     int key = (g.n[0] * 1024 + g.n[1]) * 1024 + g.n[2];
     if (c->twiddle_n == key && c->twiddle_prec == prec) return PB_OK;
-    size_t tot = (size_t)(g.n[0] + g.n[1] + g.n[2]) / 2 + 3;
+    size_t tot = (size_t)g.n[0] + g.n[1] + g.n[2];
     std::vector<C> h(tot);
     size_t off = 0;
     for (int a = 0; a < 3; a++) {
         int n = g.n[a];
-        for (int k = 0; k < n / 2; k++) {
+        for (int k = 0; k < n; k++) {
             long double ang = -2.0L * 3.141592653589793238462643383279502884L * k / n;
             h[off + k].x = (T)cosl(ang);
             h[off + k].y = (T)sinl(ang);
         }
-        off += n / 2 > 0 ? n / 2 : 1;
+        off += n;
     }
     int st;
     if ((st = pb_reserve(c, &c->twiddle, tot * sizeof(C)))) return st;
@@ -517,6 +825,7 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
                  const pb_box &box, double a, double d, int ns, int nt, double *u, bool accumulate)
 {
     using C = typename cplx<T>::type;
+    using C4 = typename cplx<T>::type4;
     int st;
     for (int ax = 0; ax < 3; ax++) {
         int n = g.n[ax];
@@ -525,9 +834,11 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
         if (g.P[ax] < 1 || g.P[ax] > kPmax || g.P[ax] > n)
             return pb_fail(c, PB_ERR_INVALID, "window points P must be in [1, min(32, grid)]");
     }
-    if ((st = pb_reserve(c, &c->grid, 3 * (size_t)g.ng * sizeof(C)))) return st;
+    const size_t h2 = (size_t)g.n[2] / 2 + 1;
+    if ((st = pb_reserve(c, &c->grid, (size_t)g.ng * sizeof(C4)))) return st;
+    if ((st = pb_reserve(c, &c->spec, 3 * (size_t)g.n[0] * g.n[1] * h2 * sizeof(C)))) return st;
     if ((st = upload_twiddles<T>(c, s, g))) return st;
-    // per-axis factors e^{-(d^2/4 - eta/2) k^2}
+    // per-axis factors e^{-(d^2/4 - eta/2) k^2}, axis 2 only for kz <= n2/2
     std::vector<double> fac((size_t)g.n[0] + g.n[1] + g.n[2]);
     double beta = d * d / 4 - g.eta / 2;
     size_t off = 0;
@@ -543,30 +854,34 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
     if ((st = pb_hip_check(c, hipMemcpyAsync(c->axis_fac.ptr, fac.data(), fac.size() * sizeof(double),
                                              hipMemcpyHostToDevice, s), "factor upload")))
         return st;
-    C *grid = (C *)c->grid.ptr;
+    kmult km;
+    for (int ax = 0; ax < 3; ax++) {
+        km.k[ax] = 2 * M_PI / box.L[ax];
+        km.n[ax] = g.n[ax];
+    }
+    km.c = -std::expm1(-a * a * d * d / 4) / (a * a);
+    km.a2 = a * a;
+    km.inv_ng = 1.0 / (double)g.ng;
+    void *grid4 = c->grid.ptr, *spec = c->spec.ptr;
     pb_mark(c, s, 1);
     {
         long long nb = (long long)(g.n[0] / 4) * (g.n[1] / 4) * (g.n[2] / 4);
         hipLaunchKernelGGL(k_spread_blocks<T>, dim3((unsigned)((nb + kSpreadWaves - 1) / kSpreadWaves)),
                            dim3(64 * kSpreadWaves), 0, s, (const int *)c->cell_start.ptr,
                            (const double4 *)c->spos.ptr, (const double4 *)c->sfrc.ptr, cells, box, g,
-                           (T *)grid);
+                           (C4 *)grid4);
     }
     const C *tw = (const C *)c->twiddle.ptr;
-    const C *tws[3] = {tw, tw + g.n[0] / 2, tw + g.n[0] / 2 + g.n[1] / 2};
+    const void *tws[3] = {tw, tw + g.n[0], tw + g.n[0] + g.n[1]};
+    const double *fac_d = (const double *)c->axis_fac.ptr;
     pb_mark(c, s, 2);
-    for (int ax = 2; ax >= 0; ax--)
-        if ((st = fft_dispatch<T>(s, grid, g, ax, tws[ax], 0))) return pb_fail(c, st, "FFT size");
-    pb_mark(c, s, 3);
-    double om = -std::expm1(-a * a * d * d / 4);
-    hipLaunchKernelGGL(k_multiply<T>, dim3((unsigned)((g.ng + 255) / 256)), dim3(256), 0, s, grid, g,
-                       2 * M_PI / box.L[0], 2 * M_PI / box.L[1], 2 * M_PI / box.L[2], om / (a * a),
-                       a * a, (const double *)c->axis_fac.ptr);
-    pb_mark(c, s, 4);
-    for (int ax = 0; ax < 3; ax++)
-        if ((st = fft_dispatch<T>(s, grid, g, ax, tws[ax], 1))) return pb_fail(c, st, "FFT size");
+    for (int pass = 0; pass < 5; pass++) {
+        if (pass == 2) pb_mark(c, s, 3);
+        if (pass == 3) pb_mark(c, s, 4);
+        if ((st = fft_pass<T>(s, pass, g, grid4, spec, km, fac_d, tws))) return pb_fail(c, st, "FFT size");
+    }
     pb_mark(c, s, 5);
-    if (nt > 0) launch_interp<T>(s, nt, g, c->tpos.ptr, c->tperm.ptr, (const T *)grid, u, accumulate ? 1 : 0);
+    if (nt > 0) launch_interp<T>(s, nt, g, c->tpos.ptr, c->tperm.ptr, (const C4 *)grid4, u, accumulate ? 1 : 0);
     pb_mark(c, s, 6);
     return pb_hip_check(c, hipGetLastError(), "k-space launch");
 }
