@@ -41,144 +41,272 @@ __device__ __forceinline__ int wrapi(int i, int n)
 }
 
 // ---- spreading ----------------------------------------------------------------
-// One wavefront owns a 4 x 4 x 4 block of grid points, one point per lane, and
-// gathers every source whose centred window (first index
-// f0 = floor(x/h - P/2) + 1, the oracle's formula) touches the block.
-// Candidates come from the real-space cell list: the cells covering the
-// x-interval [(4b - P/2) h, (4b + 3 + P/2) h) per axis, unwrapped so that a
-// source near the opposite face enters with its periodic image.  Lanes test 64
-// candidates at a time and append the hits to an LDS ring; every 64 hits each
-// lane computes the 4+4+4 block-local weights of one hit, then all lanes
-// accumulate their own point.  Every grid point is written exactly once, so
-// there are no atomics and the grid needs no clearing.
+// One workgroup owns a region of 8 x 8 x 8 grid points (four wavefronts, each
+// two 4 x 4 x 4 sub-blocks, one point per lane) and gathers every source whose
+// centred window (first index f0 = floor(x/h - P/2) + 1, the oracle's
+// formula) touches the region.  Candidates come from the real-space cell list:
+// the cells covering [(gA - P/2) h, (gB + P/2) h) per axis, unwrapped so that a
+// source near the opposite face enters with its periodic image.  The
+// workgroup sets up the candidate runs (one per thread), scans their lengths,
+// tests the candidates 256 at a time and stages the hits in LDS in a fixed
+// order.  Each wavefront then takes the staged sources that touch its
+// sub-block, 32 at a time: one lane per source computes the sub-block's 4+4+4
+// window weights and stores P[j][k] = wy[j] wz[k] and wx[i] F; every lane then
+// accumulates its point, P[j][k] * (wx[i] F), three FMAs per source.  Every
+// grid point is written exactly once: no atomics, no clearing, and a fixed
+// summation order.
 constexpr int kSpreadWaves = 4;
 
+template <typename T> struct spread_cfg;
+template <> struct spread_cfg<float> { static constexpr int CAP = 448, HB = 32; };
+template <> struct spread_cfg<double> { static constexpr int CAP = 256, HB = 32; };
+
 template <typename T> struct spread_lds {
-    uint2 q[128];      // (source index, image code)
-    T w[64][17];       // wx[4], wy[4], wz[4], scale * f[3]; odd stride: no bank conflicts
+    static constexpr int CAP = spread_cfg<T>::CAP, HB = spread_cfg<T>::HB;
+    int run_j[256];       // first source of each candidate run of the chunk
+    int run_pre[257];     // exclusive prefix of the run lengths
+    unsigned run_code[256]; // image shifts (l0 + 4) | (l1 + 4) << 4 | (l2 + 4) << 8
+    int wcount[kSpreadWaves];
+    double sx[3][CAP];    // staged sources: wrapped position, shifted window start,
+    int sf0[3][CAP];      // image code and scaled force
+    unsigned scode[CAP];
+    T sF[3][CAP];
+    int ring[kSpreadWaves][128];    // staged indices touching the wavefront's sub-block
+                                    // (< HB pending + 64 appended)
+    T P[kSpreadWaves][HB][17];      // wy[j] wz[k] at 4 j + k (odd stride)
+    typename cplx<T>::type4 wxF[kSpreadWaves][HB][4]; // wx[i] (fx, fy, fz)
 };
 
+// Weights of hit slot `lane` (staged index e) on the sub-block at sb.
 template <typename T>
-__device__ __forceinline__ void spread_flush(spread_lds<T> &W, int qh, int cnt, int lane,
-                                             const double4 *__restrict__ spos,
-                                             const double4 *__restrict__ sfrc, const pb_box &box,
-                                             const pb_kgrid &g, const int *g0, const int *lidx,
+__device__ __forceinline__ void spread_weights(spread_lds<T> &S, int wave, int lane, int e, const int *sb,
+                                               const pb_kgrid &g)
+{
+    T w[3][4];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int l = (int)((S.scode[e] >> (4 * a)) & 15) - 4;
+        const double x = S.sx[a][e];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int gq = sb[a] + q;
+            const int m = gq - S.sf0[a][e];
+            T v = (T)0;
+            if (m >= 0 && m < g.P[a]) {
+                const double dx = (double)(gq - l * g.n[a]) * g.h[a] - x;
+                if (sizeof(T) == 8)
+                    v = (T)exp(-dx * dx / g.eta);
+                else
+                    v = (T)__expf(-(float)(dx * dx) * (float)(1.0 / g.eta));
+            }
+            w[a][q] = v;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) S.P[wave][lane][4 * j + k] = w[1][j] * w[2][k];
+    const T f0 = S.sF[0][e], f1 = S.sF[1][e], f2 = S.sF[2][e];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        typename cplx<T>::type4 v;
+        v.x = w[0][i] * f0;
+        v.y = w[0][i] * f1;
+        v.z = w[0][i] * f2;
+        v.w = (T)0;
+        S.wxF[wave][lane][i] = v;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void spread_batch(spread_lds<T> &S, int wave, int lane, int head, int cnt,
+                                             const int *sb, const pb_kgrid &g, int li, int lj, int lk,
                                              T &a0, T &a1, T &a2)
 {
-    if (lane < cnt) {
-        const uint2 e = W.q[(qh + lane) & 127];
-        const double4 x = spos[e.x];
-        const double4 f = sfrc[e.x];
-        const double xc[3] = {x.x, x.y, x.z};
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            const int l = (int)((e.y >> (3 * a)) & 7) - 4;
-            const int f0 = (int)floor(xc[a] * g.inv_h[a] - 0.5 * g.P[a]) + 1 + l * g.n[a];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int gq = g0[a] + q;
-                const int p = gq - f0;
-                const double dx = (double)(gq - l * g.n[a]) * g.h[a] - xc[a];
-                T w = (T)0;
-                if (p >= 0 && p < g.P[a]) {
-                    if (sizeof(T) == 8)
-                        w = (T)exp(-dx * dx / g.eta);
-                    else
-                        w = (T)__expf(-(float)(dx * dx) * (float)(1.0 / g.eta));
-                }
-                W.w[lane][4 * a + q] = w;
-            }
-        }
-        W.w[lane][12] = (T)(f.x * g.scale);
-        W.w[lane][13] = (T)(f.y * g.scale);
-        W.w[lane][14] = (T)(f.z * g.scale);
-    }
     __builtin_amdgcn_wave_barrier();
-    for (int k = 0; k < cnt; k++) {
-        const T t = W.w[k][lidx[0]] * W.w[k][4 + lidx[1]] * W.w[k][8 + lidx[2]];
-        a0 += t * W.w[k][12];
-        a1 += t * W.w[k][13];
-        a2 += t * W.w[k][14];
+    if (lane < cnt) spread_weights<T>(S, wave, lane, S.ring[wave][(head + lane) & 127], sb, g);
+    __builtin_amdgcn_wave_barrier();
+    for (int h = 0; h < cnt; h++) {
+        const T p = S.P[wave][h][4 * lj + lk];
+        const typename cplx<T>::type4 v = S.wxF[wave][h][li];
+        a0 += p * v.x;
+        a1 += p * v.y;
+        a2 += p * v.z;
     }
     __builtin_amdgcn_wave_barrier();
 }
 
+// The wavefront's two sub-blocks gather from the nst staged sources.
 template <typename T>
-__global__ __launch_bounds__(64 * kSpreadWaves) void k_spread_blocks(
+__device__ __forceinline__ void spread_process(spread_lds<T> &S, int wave, int lane, int nst, const int *sbxy,
+                                               int gz, const pb_kgrid &g, T (&acc)[2][3])
+{
+    constexpr int HB = spread_cfg<T>::HB;
+    const int li = lane >> 4, lj = (lane >> 2) & 3, lk = lane & 3;
+#pragma unroll
+    for (int sz = 0; sz < 2; sz++) {
+        const int sb[3] = {sbxy[0], sbxy[1], gz + 4 * sz};
+        if (sb[0] >= g.n[0] || sb[1] >= g.n[1] || sb[2] >= g.n[2]) continue; // uniform per wavefront
+        int head = 0, pend = 0;
+        for (int eb = 0; eb < nst; eb += 64) {
+            const int e = eb + lane;
+            bool in = e < nst;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const int f0 = in ? S.sf0[a][e] : 0;
+                in = in && f0 > sb[a] - g.P[a] && f0 <= sb[a] + 3;
+            }
+            const unsigned long long m = __ballot(in);
+            if (in) {
+                const unsigned pre = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                S.ring[wave][(head + pend + pre) & 127] = e;
+            }
+            pend += __popcll(m);
+            while (pend >= HB) {
+                spread_batch<T>(S, wave, lane, head, HB, sb, g, li, lj, lk, acc[sz][0], acc[sz][1], acc[sz][2]);
+                head = (head + HB) & 127;
+                pend -= HB;
+            }
+        }
+        if (pend > 0) spread_batch<T>(S, wave, lane, head, pend, sb, g, li, lj, lk, acc[sz][0], acc[sz][1], acc[sz][2]);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64 * kSpreadWaves) void k_spread_regions(
     const int *__restrict__ cstart, const double4 *__restrict__ spos,
-    const double4 *__restrict__ sfrc, pb_cells cells, pb_box box, pb_kgrid g,
+    const double4 *__restrict__ sfrc, pb_cells cells, pb_kgrid g,
     typename cplx<T>::type4 *__restrict__ grid4)
 {
-    __shared__ spread_lds<T> lds[kSpreadWaves];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int nb0 = g.n[0] >> 2, nb1 = g.n[1] >> 2, nb2 = g.n[2] >> 2;
-    const int bid = blockIdx.x * kSpreadWaves + wave;
-    if (bid >= nb0 * nb1 * nb2) return; // whole wavefront; no block barriers below
-    spread_lds<T> &W = lds[wave];
-    const int b[3] = {bid / (nb1 * nb2), (bid / nb2) % nb1, bid % nb2};
-    const int g0[3] = {4 * b[0], 4 * b[1], 4 * b[2]};
-    const int lidx[3] = {lane >> 4, (lane >> 2) & 3, lane & 3};
-    int ulo[3], uhi[3];
+    constexpr int CAP = spread_cfg<T>::CAP;
+    __shared__ spread_lds<T> S;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int nr1 = (g.n[1] + 7) / 8, nr2 = (g.n[2] + 7) / 8;
+    const int rid = blockIdx.x;
+    const int rr[3] = {rid / (nr1 * nr2), (rid / nr2) % nr1, rid % nr2};
+    int gA[3], gB[3], ulo[3], uhi[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        const double lo = (g0[a] - 0.5 * g.P[a]) * g.h[a], hi = (g0[a] + 3 + 0.5 * g.P[a]) * g.h[a];
+        gA[a] = 8 * rr[a];
+        gB[a] = min(gA[a] + 7, g.n[a] - 1);
+        const double lo = (gA[a] - 0.5 * g.P[a]) * g.h[a], hi = (gB[a] + 0.5 * g.P[a]) * g.h[a];
         ulo[a] = (int)floor(lo * cells.inv_h[a] - 1e-9);
         uhi[a] = (int)floor(hi * cells.inv_h[a] + 1e-9);
     }
-    T a0 = 0, a1 = 0, a2 = 0;
-    int qh = 0, qn = 0;
-    for (int u0 = ulo[0]; u0 <= uhi[0]; u0++) {
-        const int l0 = floordiv(u0, cells.nc[0]), w0 = u0 - l0 * cells.nc[0];
-        for (int u1 = ulo[1]; u1 <= uhi[1]; u1++) {
-            const int l1 = floordiv(u1, cells.nc[1]), w1 = u1 - l1 * cells.nc[1];
-            const int colbase = (w0 * cells.nc[1] + w1) * cells.nc[2];
-            // the z range splits into runs of one image shift, each a contiguous
-            // run of the z-major sorted sources
-            for (int zs = ulo[2]; zs <= uhi[2];) {
-                const int l2 = floordiv(zs, cells.nc[2]);
-                const int zend = min(uhi[2], (l2 + 1) * cells.nc[2] - 1);
-                const int js = cstart[colbase + zs - l2 * cells.nc[2]];
-                const int je = cstart[colbase + zend - l2 * cells.nc[2] + 1];
-                zs = zend + 1;
-                const unsigned code = (unsigned)(l0 + 4) | ((unsigned)(l1 + 4) << 3) | ((unsigned)(l2 + 4) << 6);
-                for (int j0 = js; j0 < je; j0 += 64) {
-                    const int j = j0 + lane;
-                    bool hit = false;
-                    if (j < je) {
-                        const double4 x = spos[j];
-                        const int f0x = (int)floor(x.x * g.inv_h[0] - 0.5 * g.P[0]) + 1 + l0 * g.n[0];
-                        const int f0y = (int)floor(x.y * g.inv_h[1] - 0.5 * g.P[1]) + 1 + l1 * g.n[1];
-                        const int f0z = (int)floor(x.z * g.inv_h[2] - 0.5 * g.P[2]) + 1 + l2 * g.n[2];
-                        hit = f0x > g0[0] - g.P[0] && f0x <= g0[0] + 3 && f0y > g0[1] - g.P[1] &&
-                              f0y <= g0[1] + 3 && f0z > g0[2] - g.P[2] && f0z <= g0[2] + 3;
-                    }
-                    const unsigned long long m = __ballot(hit);
-                    if (m == 0ull) continue;
-                    if (hit) {
-                        const unsigned pre = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                                       __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                        W.q[(qh + qn + pre) & 127] = make_uint2((unsigned)j, code);
-                    }
-                    qn += __popcll(m);
-                    if (qn >= 64) {
-                        __builtin_amdgcn_wave_barrier();
-                        spread_flush<T>(W, qh, 64, lane, spos, sfrc, box, g, g0, lidx, a0, a1, a2);
-                        qh = (qh + 64) & 127;
-                        qn -= 64;
-                    }
+    const int sbxy[2] = {gA[0] + 4 * (wave >> 1), gA[1] + 4 * (wave & 1)};
+    const int nc0 = cells.nc[0], nc1 = cells.nc[1], nc2 = cells.nc[2];
+    const int ncx = uhi[0] - ulo[0] + 1, ncy = uhi[1] - ulo[1] + 1;
+    const int lz0 = floordiv(ulo[2], nc2), nzseg = floordiv(uhi[2], nc2) - lz0 + 1;
+    const long long nruns = (long long)ncx * ncy * nzseg;
+    T acc[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    int nst = 0;
+    for (long long rb = 0; rb < nruns; rb += 256) {
+        // one candidate run per thread: column (u0, u1), z segment with one image
+        const long long r = rb + tid;
+        int j = 0, len = 0;
+        unsigned code = 0;
+        if (r < nruns) {
+            const int seg = (int)(r % nzseg);
+            const long long col = r / nzseg;
+            const int u0 = ulo[0] + (int)(col / ncy), u1 = ulo[1] + (int)(col % ncy);
+            const int l0 = floordiv(u0, nc0), l1 = floordiv(u1, nc1), l2 = lz0 + seg;
+            const int zs = max(ulo[2], l2 * nc2) - l2 * nc2, ze = min(uhi[2], (l2 + 1) * nc2 - 1) - l2 * nc2;
+            const int colbase = ((u0 - l0 * nc0) * nc1 + (u1 - l1 * nc1)) * nc2;
+            j = cstart[colbase + zs];
+            len = cstart[colbase + ze + 1] - j;
+            code = (unsigned)(l0 + 4) | ((unsigned)(l1 + 4) << 4) | ((unsigned)(l2 + 4) << 8);
+        }
+        // exclusive scan of the run lengths
+        S.run_pre[tid + 1] = len;
+        if (tid == 0) S.run_pre[0] = 0;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const int v = tid + 1 > off ? S.run_pre[tid + 1 - off] : 0;
+            __syncthreads();
+            S.run_pre[tid + 1] += v;
+            __syncthreads();
+        }
+        S.run_j[tid] = j;
+        S.run_code[tid] = code;
+        __syncthreads();
+        const int total = S.run_pre[256];
+        for (int qb = 0; qb < total; qb += 256) {
+            const int q = qb + tid;
+            bool hit = false;
+            int src = 0, f0[3] = {0, 0, 0};
+            unsigned cq = 0;
+            double xw[3] = {0, 0, 0};
+            if (q < total) {
+                int lo = 0, hi = 255; // last run with run_pre <= q
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (S.run_pre[mid] <= q)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                src = S.run_j[lo] + (q - S.run_pre[lo]);
+                cq = S.run_code[lo];
+                const double4 x = spos[src];
+                xw[0] = x.x;
+                xw[1] = x.y;
+                xw[2] = x.z;
+                hit = true;
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    const int l = (int)((cq >> (4 * a)) & 15) - 4;
+                    f0[a] = (int)floor(xw[a] * g.inv_h[a] - 0.5 * g.P[a]) + 1 + l * g.n[a];
+                    hit = hit && f0[a] > gA[a] - g.P[a] && f0[a] <= gB[a];
                 }
             }
+            // staged in a fixed order: wavefront by wavefront, lanes in order
+            const unsigned long long m = __ballot(hit);
+            if (lane == 0) S.wcount[wave] = __popcll(m);
+            __syncthreads();
+            int wbase = 0, tot = 0;
+#pragma unroll
+            for (int w = 0; w < kSpreadWaves; w++) {
+                wbase += w < wave ? S.wcount[w] : 0;
+                tot += S.wcount[w];
+            }
+            if (nst + tot > CAP) { // no room: gather from what is staged, then restart
+                spread_process<T>(S, wave, lane, nst, sbxy, gA[2], g, acc);
+                nst = 0;
+                __syncthreads();
+            }
+            if (hit) {
+                const int e = nst + wbase +
+                              (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                const double4 f = sfrc[src];
+                S.sx[0][e] = xw[0];
+                S.sx[1][e] = xw[1];
+                S.sx[2][e] = xw[2];
+                S.sf0[0][e] = f0[0];
+                S.sf0[1][e] = f0[1];
+                S.sf0[2][e] = f0[2];
+                S.scode[e] = cq;
+                S.sF[0][e] = (T)(f.x * g.scale);
+                S.sF[1][e] = (T)(f.y * g.scale);
+                S.sF[2][e] = (T)(f.z * g.scale);
+            }
+            nst += tot;
+            __syncthreads();
         }
+        __syncthreads(); // the run arrays are rewritten by the next chunk
     }
-    __builtin_amdgcn_wave_barrier();
-    if (qn > 0) spread_flush<T>(W, qh, qn, lane, spos, sfrc, box, g, g0, lidx, a0, a1, a2);
-    const size_t idx = ((size_t)(g0[0] + lidx[0]) * g.n[1] + (g0[1] + lidx[1])) * g.n[2] + (g0[2] + lidx[2]);
-    typename cplx<T>::type4 o;
-    o.x = a0;
-    o.y = a1;
-    o.z = a2;
-    o.w = (T)0;
-    grid4[idx] = o;
+    spread_process<T>(S, wave, lane, nst, sbxy, gA[2], g, acc);
+    const int li = lane >> 4, lj = (lane >> 2) & 3, lk = lane & 3;
+#pragma unroll
+    for (int sz = 0; sz < 2; sz++) {
+        const int p0 = sbxy[0] + li, p1 = sbxy[1] + lj, p2 = gA[2] + 4 * sz + lk;
+        if (sbxy[0] >= g.n[0] || sbxy[1] >= g.n[1] || gA[2] + 4 * sz >= g.n[2]) continue;
+        typename cplx<T>::type4 o;
+        o.x = acc[sz][0];
+        o.y = acc[sz][1];
+        o.z = acc[sz][2];
+        o.w = (T)0;
+        grid4[((size_t)p0 * g.n[1] + p1) * g.n[2] + p2] = o;
+    }
 }
 
 // ---- interpolation (gather) --------------------------------------------------
@@ -865,11 +993,10 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
     void *grid4 = c->grid.ptr, *spec = c->spec.ptr;
     pb_mark(c, s, 1);
     {
-        long long nb = (long long)(g.n[0] / 4) * (g.n[1] / 4) * (g.n[2] / 4);
-        hipLaunchKernelGGL(k_spread_blocks<T>, dim3((unsigned)((nb + kSpreadWaves - 1) / kSpreadWaves)),
-                           dim3(64 * kSpreadWaves), 0, s, (const int *)c->cell_start.ptr,
-                           (const double4 *)c->spos.ptr, (const double4 *)c->sfrc.ptr, cells, box, g,
-                           (C4 *)grid4);
+        const long long nreg = (long long)((g.n[0] + 7) / 8) * ((g.n[1] + 7) / 8) * ((g.n[2] + 7) / 8);
+        hipLaunchKernelGGL(k_spread_regions<T>, dim3((unsigned)nreg), dim3(64 * kSpreadWaves), 0, s,
+                           (const int *)c->cell_start.ptr, (const double4 *)c->spos.ptr,
+                           (const double4 *)c->sfrc.ptr, cells, g, (C4 *)grid4);
     }
     const C *tw = (const C *)c->twiddle.ptr;
     const void *tws[3] = {tw, tw + g.n[0], tw + g.n[0] + g.n[1]};
