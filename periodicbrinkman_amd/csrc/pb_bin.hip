@@ -232,5 +232,6 @@ int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_
                            is_source ? (double4 *)c->sfrc.ptr : nullptr,
                            is_source ? nullptr : (int *)c->tperm.ptr);
     }
+    if (!is_source && (st = pb_build_tiles(c, s, cells, n, &c->max_tiles))) return st;
     return pb_hip_check(c, hipGetLastError(), "binning launch");
 }

@@ -127,7 +127,8 @@ struct pb_context {
     pb_buffer cell_count, cell_start, cell_of_s, rank_s, cell_of_t, rank_t, tcell_count,
         tcell_start;
     pb_buffer spos, sfrc, tpos, tperm, counters, scan_tmp;
-    pb_buffer qcount, qoff, tiles; // target tiles of the pair kernel: int2 (first, count)
+    pb_buffer qcount, qoff, tiles; // target tiles: int2 (first, count); count at qoff[nkey_t/16]
+    int max_tiles = 0;             // bound on the tile count (launch size)
     // k space
     pb_buffer grid, spec, twiddle, axis_fac; // real grid (T4 per point), half spectrum
     int twiddle_n = 0;
@@ -164,8 +165,17 @@ int pb_hip_check(pb_context *c, hipError_t e, const char *what);
 int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_box &box, int n,
                   const double *pos, const double *frc, bool is_source);
 // Target tiles: runs of <= 64 sorted targets inside one quarter brick; the
-// tile count is left on the device, *max_tiles bounds it.
+// tile count is left on the device, *max_tiles bounds it.  Built by
+// pb_bin_points for targets.
 int pb_build_tiles(pb_context *c, hipStream_t s, const pb_cells &cells, int nt, int *max_tiles);
+
+// Workgroups are dispatched to the 8 XCDs of the device round-robin; this
+// renumbering gives each XCD one contiguous range of logical ids, so
+// neighbouring work shares an L2 (nblk8: launched workgroups, a multiple of 8).
+__device__ __forceinline__ int pb_xcd_block(int b, int nblk8)
+{
+    return (b & 7) * (nblk8 >> 3) + (b >> 3);
+}
 // far32: pairs beyond rfar in FP32 (table copy when table_doubles > 0, else
 // the FP32 closed form); otherwise every pair in FP64.
 int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp_dev,

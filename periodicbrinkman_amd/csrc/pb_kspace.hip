@@ -310,11 +310,13 @@ __global__ __launch_bounds__(64 * kSpreadWaves) void k_spread_regions(
 }
 
 // ---- interpolation (gather) --------------------------------------------------
-// u(y) = h^3 (pi eta)^{-3/2} sum_window e^{-|x_g - y|^2/eta} G(x_g).  One
-// wavefront per target: the 3P weights and wrapped indices are computed once
-// into LDS, then the lanes sweep the P^3 window with consecutive lanes on
-// consecutive z (contiguous memory) and the partial sums meet in a wave
-// reduction.
+// u(y) = h^3 (pi eta)^{-3/2} sum_window e^{-|x_g - y|^2/eta} G(x_g), G = (ux,
+// uy, uz) per grid point.  One wavefront per target (targets in their sorted,
+// spatially compact order, each XCD taking one contiguous range).  The 3P
+// window weights and wrapped offsets go to LDS; each lane owns (y, z) pairs
+// of the window, forms wy wz and the row offset once, and sweeps the P
+// x-planes (weights and offsets the same for the whole wavefront): one vector
+// load and three FMAs per point.  The partial sums meet in a wave reduction.
 constexpr int kInterpWaves = 4;
 
 template <typename T> __device__ __forceinline__ T wave_sum(T v)
@@ -325,15 +327,17 @@ template <typename T> __device__ __forceinline__ T wave_sum(T v)
 }
 
 template <typename T, int P>
-__global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, const double4 *__restrict__ tpos,
+__global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, int nblk8,
+                                                            const double4 *__restrict__ tpos,
                                                             const int *__restrict__ tperm, pb_kgrid g,
-                                                            const typename cplx<T>::type4 *__restrict__ grid4,
+                                                            const typename cplx<T>::type4 *__restrict__ U4,
                                                             double *__restrict__ u, int accumulate)
 {
+    constexpr int NQ = P * P, NR = (NQ + 63) / 64;
     __shared__ T wts[kInterpWaves][3][P];
-    __shared__ int ix[kInterpWaves][3][P];
+    __shared__ int off[kInterpWaves][3][P]; // wrapped index times its stride
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = blockIdx.x * kInterpWaves + wave;
+    const int i = pb_xcd_block(blockIdx.x, nblk8) * kInterpWaves + wave;
     if (i >= nt) return; // whole wavefront; no block barriers below
     const double4 y = tpos[i];
     for (int k = lane; k < 3 * P; k += 64) {
@@ -347,22 +351,25 @@ __global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, const doub
             wts[wave][a][p] = (T)__expf(-(float)(dx * dx) * (float)(1.0 / g.eta));
         int j = f0 + p;
         j = j < 0 ? j + g.n[a] : (j >= g.n[a] ? j - g.n[a] : j);
-        ix[wave][a][p] = j;
+        off[wave][a][p] = a == 0 ? j * g.n[1] * g.n[2] : (a == 1 ? j * g.n[2] : j);
     }
     __builtin_amdgcn_wave_barrier();
     T a0 = 0, a1 = 0, a2 = 0;
-    constexpr int kIter = (P * P * P + 63) / 64;
-#pragma unroll 9
-    for (int it = 0; it < kIter; it++) {
-        const int k = it * 64 + lane;
-        if (P * P * P % 64 == 0 || k < P * P * P) {
-            const int pz = k % P, py = (k / P) % P, px = k / (P * P);
-            const size_t off = ((size_t)ix[wave][0][px] * g.n[1] + ix[wave][1][py]) * g.n[2] + ix[wave][2][pz];
-            const T wt = wts[wave][0][px] * wts[wave][1][py] * wts[wave][2][pz];
-            const typename cplx<T>::type4 v = grid4[off];
-            a0 += wt * v.x;
-            a1 += wt * v.y;
-            a2 += wt * v.z;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int q = r * 64 + lane;
+        if (NQ % 64 == 0 || q < NQ) {
+            const int py = q / P, pz = q % P;
+            const T wyz = wts[wave][1][py] * wts[wave][2][pz];
+            const int oyz = off[wave][1][py] + off[wave][2][pz];
+#pragma unroll
+            for (int px = 0; px < P; px++) {
+                const T wq = wts[wave][0][px] * wyz;
+                const typename cplx<T>::type4 v = U4[(unsigned)(off[wave][0][px] + oyz)];
+                a0 += wq * v.x;
+                a1 += wq * v.y;
+                a2 += wq * v.z;
+            }
         }
     }
     a0 = wave_sum(a0);
@@ -387,7 +394,7 @@ __global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, const doub
 template <typename T>
 __global__ __launch_bounds__(256) void k_interp_any(int nt, const double4 *__restrict__ tpos,
                                                     const int *__restrict__ tperm, pb_kgrid g,
-                                                    const typename cplx<T>::type4 *__restrict__ grid4,
+                                                    const typename cplx<T>::type4 *__restrict__ U4,
                                                     double *__restrict__ u, int accumulate)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -409,7 +416,7 @@ __global__ __launch_bounds__(256) void k_interp_any(int nt, const double4 *__res
                 const double dxz = (double)(f0[2] + pz) * g.h[2] - yc[2];
                 const double wt = wxy * exp(-dxz * dxz / g.eta);
                 const int jz = ((f0[2] + pz) % g.n[2] + g.n[2]) % g.n[2];
-                const typename cplx<T>::type4 v = grid4[((size_t)jx * g.n[1] + jy) * g.n[2] + jz];
+                const typename cplx<T>::type4 v = U4[((size_t)jx * g.n[1] + jy) * g.n[2] + jz];
                 a0 += wt * (double)v.x;
                 a1 += wt * (double)v.y;
                 a2 += wt * (double)v.z;
@@ -431,24 +438,27 @@ __global__ __launch_bounds__(256) void k_interp_any(int nt, const double4 *__res
 
 template <typename T>
 void launch_interp(hipStream_t s, int nt, const pb_kgrid &g, const void *tpos, const void *tperm,
-                   const typename cplx<T>::type4 *grid, double *u, int acc)
+                   const typename cplx<T>::type4 *U4, double *u, int acc)
 {
-    dim3 gd((nt + kInterpWaves - 1) / kInterpWaves), bd(64 * kInterpWaves);
+    const int nblk8 = ((nt + kInterpWaves - 1) / kInterpWaves + 7) & ~7;
+    dim3 gd(nblk8), bd(64 * kInterpWaves);
     dim3 gd1((nt + 255) / 256), bd1(256);
     const double4 *tp = (const double4 *)tpos;
     const int *pm = (const int *)tperm;
     bool same = g.P[0] == g.P[1] && g.P[1] == g.P[2];
+#define PB_INTERP(PP) hipLaunchKernelGGL((k_interp<T, PP>), gd, bd, 0, s, nt, nblk8, tp, pm, g, U4, u, acc)
     switch (same ? g.P[0] : -1) {
-    case 4: hipLaunchKernelGGL((k_interp<T, 4>), gd, bd, 0, s, nt, tp, pm, g, grid, u, acc); break;
-    case 6: hipLaunchKernelGGL((k_interp<T, 6>), gd, bd, 0, s, nt, tp, pm, g, grid, u, acc); break;
-    case 8: hipLaunchKernelGGL((k_interp<T, 8>), gd, bd, 0, s, nt, tp, pm, g, grid, u, acc); break;
-    case 10: hipLaunchKernelGGL((k_interp<T, 10>), gd, bd, 0, s, nt, tp, pm, g, grid, u, acc); break;
-    case 12: hipLaunchKernelGGL((k_interp<T, 12>), gd, bd, 0, s, nt, tp, pm, g, grid, u, acc); break;
-    case 16: hipLaunchKernelGGL((k_interp<T, 16>), gd, bd, 0, s, nt, tp, pm, g, grid, u, acc); break;
-    case 20: hipLaunchKernelGGL((k_interp<T, 20>), gd, bd, 0, s, nt, tp, pm, g, grid, u, acc); break;
-    case 24: hipLaunchKernelGGL((k_interp<T, 24>), gd, bd, 0, s, nt, tp, pm, g, grid, u, acc); break;
-    default: hipLaunchKernelGGL(k_interp_any<T>, gd1, bd1, 0, s, nt, tp, pm, g, grid, u, acc); break;
+    case 4: PB_INTERP(4); break;
+    case 6: PB_INTERP(6); break;
+    case 8: PB_INTERP(8); break;
+    case 10: PB_INTERP(10); break;
+    case 12: PB_INTERP(12); break;
+    case 16: PB_INTERP(16); break;
+    case 20: PB_INTERP(20); break;
+    case 24: PB_INTERP(24); break;
+    default: hipLaunchKernelGGL(k_interp_any<T>, gd1, bd1, 0, s, nt, tp, pm, g, U4, u, acc); break;
     }
+#undef PB_INTERP
 }
 
 // ---- FFT ---------------------------------------------------------------------
@@ -963,6 +973,8 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
             return pb_fail(c, PB_ERR_INVALID, "window points P must be in [1, min(32, grid)]");
     }
     const size_t h2 = (size_t)g.n[2] / 2 + 1;
+    // real grid: (fx, fy, fz, 0) per point after the spread, (ux, uy, uz, 0)
+    // after the inverse transform
     if ((st = pb_reserve(c, &c->grid, (size_t)g.ng * sizeof(C4)))) return st;
     if ((st = pb_reserve(c, &c->spec, 3 * (size_t)g.n[0] * g.n[1] * h2 * sizeof(C)))) return st;
     if ((st = upload_twiddles<T>(c, s, g))) return st;

@@ -453,9 +453,7 @@ int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kp
     if (!accumulate &&
         (st = pb_hip_check(c, hipMemsetAsync(u, 0, 3 * (size_t)nt * sizeof(double), s), "memset u")))
         return st;
-    pb_mark(c, s, PB_NPHASE - 1); // the pair phase: tile list and pair kernel
-    int max_tiles = 0;
-    if ((st = pb_build_tiles(c, s, cells, nt, &max_tiles))) return st;
+    const int max_tiles = c->max_tiles; // tiles built with the target binning
     const int *ntiles = (const int *)c->qoff.ptr + cells.nkey_t / 16;
     // wavefronts per tile: about two rounds of resident wavefronts (16 per CU)
     // in total; each extra wavefront repeats the tile set-up
@@ -467,6 +465,7 @@ int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kp
     const bool mixed = far32;
     const size_t ntab = table_doubles; // 0: closed forms per pair
     const size_t lds = ntab ? ntab * sizeof(double) + (mixed ? ntab * sizeof(float) : 0) : 0;
+    pb_mark(c, s, PB_NPHASE - 1); // the pair-kernel interval covers only this launch
 #define PB_PAIR_ARGS                                                                                   \
     (const int2 *)c->tiles.ptr, ntiles, (const int *)c->cell_start.ptr, (const double4 *)c->spos.ptr,   \
         (const double4 *)c->sfrc.ptr, (const double4 *)c->tpos.ptr, (const int *)c->tperm.ptr, nsplit,    \
