@@ -29,8 +29,13 @@ sys.path.insert(0, ROOT)
 
 NS = NT = 100_000
 ALPHA, EPS, GRID, P, TOL = 10.0, 1e-3, (128, 128, 128), 12, 1e-6
-FLOP_PER_PAIR = 200.0  # algorithmic FLOP per (target, source image) inside rc
-PEAK_FP64, PEAK_FP32 = 78.6, 157.3  # TFLOP/s, MI355X vector
+# Algorithmic work of the pair kernel: 200 FLOP per (target, source image)
+# inside rc -- about 80 arithmetic operations (geometry, radial functions,
+# 3x3 apply) plus six exp/erfc at 20 FLOP each.
+FLOP_PER_PAIR = 200.0
+PEAK_FP64, PEAK_FP32 = 78.6, 157.3  # TFLOP/s, MI355X FP64 (vector = matrix) and FP32 peaks
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
+COPY_BYTES = 1 << 29  # copy-calibration size (512 MiB read + 512 MiB written)
 
 
 def log(msg):
@@ -83,6 +88,31 @@ def oracle_subsample_check(x, f, y, pref, uref, nsub):
     return relL2(uref[idx], want)
 
 
+def copy_calibration():
+    """A plain device-to-device copy of COPY_BYTES, three times: the HBM
+    counters (FETCH_SIZE, WRITE_SIZE) of its kernel scale the pair kernel's."""
+    import torch
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": COPY_BYTES, "copies": 3}))
+
+
+def pair_traffic(workload):
+    """HBM bytes per pair-kernel launch from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload:
+        return None
+    return t.get("bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,7 +121,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle check")
     ap.add_argument("--cpu-targets", type=int, default=8,
                     help="targets in the CPU real-space sample (all 1e5 sources x 27 images each)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of known size (for HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
@@ -153,6 +188,8 @@ def main():
     achieved = FLOP_PER_PAIR * pairs / t_pair / 1e12
     f64 = n64 / pairs
     peak = 1.0 / (f64 / PEAK_FP64 + (1 - f64) / PEAK_FP32)
+    workload = (f"single periodic system, {NS} sources -> {NT} separate targets, alpha {ALPHA:g}, "
+                f"eps {EPS:g}, {GRID[0]}^3 grid, P {P}, mixed FP32/FP64")
     result = {
         "metric": "velocity evals/sec at <=1e-6 rel. L2 vs FP64 Ewald oracle; % of roofline",
         "value": value,
@@ -166,8 +203,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic (xoshiro256** seed 0x5EED0001: uniform positions, N(0,1) forces)",
-        "config": {"workload": "single periodic system, 1e5 sources -> 1e5 separate targets, "
-                               "alpha 10, eps 1e-3, 128^3 grid, P 12, mixed FP32/FP64",
+        "config": {"workload": workload,
                    "ns": NS, "nt": NT, "d": p.d, "rc": p.rc, "grid": list(GRID), "P": P,
                    "m": p.m},
         "rel_l2_mixed": err_mixed,
@@ -187,11 +223,14 @@ def main():
         "speedup_mixed_vs_plain_fp64": ms_plain / ms_mixed,
         "phases_ms": ph_mixed,
         "pairs": {"fp64": n64, "fp32": n32},
-        "roofline": {"bound": "valu", "kernel": "k_pair<mixed> (real-space pair kernel)",
-                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "note": "200 FLOP per pair-image inside rc; peak = FP64/FP32 vector blend "
-                             "by the FP64 pair fraction %.3f" % f64},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": pair_traffic(workload),
+                     "kernel": "k_pair_table (real-space pair kernel)",
+                     "note": "compute roof of the FP64 pair kernel, which runs on the vector ALUs "
+                             "(MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s); "
+                             "%d FLOP per pair-image inside rc, %d pair-images per launch, "
+                             "%.4f ms per launch (HIP events on the launch stream); "
+                             "FP64 pair fraction %.3f" % (FLOP_PER_PAIR, pairs, ph_mixed["pair"], f64)},
     }
     if not args.no_cpu:
         log("cpu baseline")

@@ -172,6 +172,9 @@ int pb_build_tiles(pb_context *c, hipStream_t s, const pb_cells &cells, int nt, 
 // Workgroups are dispatched to the 8 XCDs of the device round-robin; this
 // renumbering gives each XCD one contiguous range of logical ids, so
 // neighbouring work shares an L2 (nblk8: launched workgroups, a multiple of 8).
+// It pays for the gather, whose grid reads have no other reuse; the pair and
+// spread kernels measured faster without it (their spatially adjacent
+// workgroups then contend for the same L2 channels).
 __device__ __forceinline__ int pb_xcd_block(int b, int nblk8)
 {
     return (b & 7) * (nblk8 >> 3) + (b >> 3);

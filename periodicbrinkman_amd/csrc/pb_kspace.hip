@@ -181,8 +181,9 @@ __global__ __launch_bounds__(64 * kSpreadWaves) void k_spread_regions(
     constexpr int CAP = spread_cfg<T>::CAP;
     __shared__ spread_lds<T> S;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int nr1 = (g.n[1] + 7) / 8, nr2 = (g.n[2] + 7) / 8;
+    const int nr0 = (g.n[0] + 7) / 8, nr1 = (g.n[1] + 7) / 8, nr2 = (g.n[2] + 7) / 8;
     const int rid = blockIdx.x;
+    if (rid >= nr0 * nr1 * nr2) return; // whole workgroup, before any barrier
     const int rr[3] = {rid / (nr1 * nr2), (rid / nr2) % nr1, rid % nr2};
     int gA[3], gB[3], ulo[3], uhi[3];
 #pragma unroll

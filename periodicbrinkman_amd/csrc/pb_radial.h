@@ -179,9 +179,12 @@ __device__ __forceinline__ void vR_H(double r, const pb_kparams &K, double &H1, 
 
 // All regions, not inlined: kernels whose common path is the table call this
 // for the rare near field so its library constants stay out of their loops.
-__device__ __noinline__ void vR_H_call(double r, const pb_kparams *K, double *H)
+// (H1, H2) come back in registers.
+__device__ __noinline__ double2 vR_H_call(double r, const pb_kparams *K)
 {
-    vR_H(r, *K, H[0], H[1]);
+    double2 H;
+    vR_H(r, *K, H.x, H.y);
+    return H;
 }
 
 // Where the table of diffH(d) lives: z = r/d in [tz0, tz0 + tn/tinv).

@@ -169,9 +169,8 @@ __device__ __forceinline__ void batch_rare(wave_lds &W, const int lane, const in
         const double d1 = W.y[t][1] - (xs.y + s1);
         const double d2 = W.y[t][2] - (xs.z + s2);
         const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-        double H[2];
-        pbdev::vR_H_call(r, kpp, H);
-        accumulate(W, t, H[0], H[1], d0, d1, d2, fs);
+        const double2 H = pbdev::vR_H_call(r, kpp);
+        accumulate(W, t, H.x, H.y, d0, d1, d2, fs);
     }
 }
 
@@ -426,10 +425,13 @@ __device__ __forceinline__ void pair_body(
         pb_box box, const pb_kparams *__restrict__ kpp, double *__restrict__ u, uint4 *__restrict__ stats
 #define PB_PAIR_FWD tiles, ntiles_ptr, cstart, spos, sfrc, tpos, tperm, nsplit, cells, box, kpp, u, stats
 
-// The table path fits 4 wavefronts per SIMD; the closed forms need more
-// registers and run with the compiler's own choice.
+// The table path fits 4 wavefronts per SIMD (build-time PB_PAIR_OCC); the
+// closed forms need more registers and run with the compiler's own choice.
+#ifndef PB_PAIR_OCC
+#define PB_PAIR_OCC 4
+#endif
 template <bool MIXED>
-__global__ __launch_bounds__(64 * kWaves, 4) void k_pair_table(PB_PAIR_PARAMS)
+__global__ __launch_bounds__(64 * kWaves, PB_PAIR_OCC) void k_pair_table(PB_PAIR_PARAMS)
 {
     pair_body<MIXED, true>(PB_PAIR_FWD);
 }
