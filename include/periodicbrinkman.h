@@ -126,6 +126,23 @@ void pb_params_default(pb_params *p);
  * from the error model; host only. */
 int pb_tune(pb_params *p, int ns, int nt);
 
+/* Error-model autotuner (replaces the reference's dead optimalNumShells,
+ * periodicBrinkman.h:826-1016, with an error model of this evaluator).  For
+ * the box, alpha, eps, tol and precision in p it scores every FFT grid (powers
+ * of two per axis, 4..512) and window P in {4,6,8,10,12,16,20,24} whose
+ * predicted errors meet tol, sets d and rc from the error model, and keeps
+ * the combination with the smallest predicted time on MI355X; p->kernel is
+ * kept.  The report (may be NULL) holds the prediction. */
+typedef struct pb_tune_report {
+    double err_real;   /* e^{-(rc/d)^2}: real-space truncation */
+    double err_kspace; /* e^{-(d^2/4 - eta/2) kmax^2}: dropped modes */
+    double err_window; /* e^{-pi P/2}: window truncation and aliasing (m = sqrt(pi P)) */
+    double pairs;      /* predicted (target, source image) pairs inside rc */
+    double t_real_ms, t_kspace_ms, t_bin_ms, t_total_ms; /* predicted phase times */
+    int candidates;    /* (grid, P) combinations scored */
+} pb_tune_report;
+int pb_autotune(pb_params *p, int ns, int nt, pb_tune_report *report);
+
 /* One context per host thread; it owns the device scratch (cell lists,
  * grids, twiddles).  device: HIP device ordinal. */
 int pb_create(pb_context **ctx, int device);

@@ -45,6 +45,14 @@ class _Params(ctypes.Structure):
                 ("tol", ctypes.c_double), ("kernel", ctypes.c_int)]
 
 
+class _TuneReport(ctypes.Structure):
+    _fields_ = [("err_real", ctypes.c_double), ("err_kspace", ctypes.c_double),
+                ("err_window", ctypes.c_double), ("pairs", ctypes.c_double),
+                ("t_real_ms", ctypes.c_double), ("t_kspace_ms", ctypes.c_double),
+                ("t_bin_ms", ctypes.c_double), ("t_total_ms", ctypes.c_double),
+                ("candidates", ctypes.c_int)]
+
+
 def build(jobs=8):
     """Compile libpbamd.so for gfx950 in-tree (hipcc; no GPU needed)."""
     subprocess.run(["make", "-s", "-j%d" % jobs, "-C", os.path.join(_HERE, "csrc")], check=True)
@@ -77,6 +85,7 @@ def _declare(L):
     L.pb_params_default.argtypes = [ctypes.POINTER(_Params)]
     L.pb_params_default.restype = None
     L.pb_tune.argtypes = [ctypes.POINTER(_Params), ii, ii]
+    L.pb_autotune.argtypes = [ctypes.POINTER(_Params), ii, ii, ctypes.POINTER(_TuneReport)]
     L.pb_create.argtypes = [ctypes.POINTER(_VP), ii]
     L.pb_destroy.argtypes = [_VP]
     L.pb_last_error.argtypes = [_VP]
@@ -175,6 +184,16 @@ class Params:
         self.d, self.rc, self.m = p.d, p.rc, p.m
         self.grid = tuple(p.grid[i] for i in range(3))
         return self
+
+    def autotune(self, ns, nt):
+        """Choose grid, P, m, d and rc for tol by the error model and the MI355X
+        cost model (pb_autotune); returns the report as a dict."""
+        p = self._struct()
+        r = _TuneReport()
+        _check(lib().pb_autotune(ctypes.byref(p), int(ns), int(nt), ctypes.byref(r)))
+        self.d, self.rc, self.m, self.P = p.d, p.rc, p.m, p.P
+        self.grid = tuple(p.grid[i] for i in range(3))
+        return {k: getattr(r, k) for k, _ in _TuneReport._fields_}
 
     def copy(self, **kw):
         q = Params(self.L, self.alpha, self.eps, self.d, self.rc, self.grid, self.P, self.m,

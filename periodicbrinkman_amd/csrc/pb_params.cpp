@@ -294,6 +294,99 @@ int pb_tune(pb_params *p, int ns, int nt)
     return PB_OK;
 }
 
+// ---- autotuner -----------------------------------------------------------------
+// Cost model of this build on MI355X (measured at 1e5/1e5, 128^3, P 12; per
+// unit of work): pair kernel per (target, source image) inside rc, spreading
+// and gathering per window point, FFT per grid point of the five passes, and
+// fixed costs (binning, launches).  FP64 k space costs about twice FP32.
+namespace {
+struct cost_model {
+    double pair_s;    // per pair-image
+    double spread_s;  // per source window point
+    double gather_s;  // per target window point
+    double fft_s;     // per grid point (five passes)
+    double fixed_s;   // binning and launches
+    double bin_s;     // per point binned
+};
+cost_model model_for(int precision)
+{
+    if (precision == PB_PREC_MIXED) return {9.9e-12, 1.23e-12, 0.83e-12, 5.8e-11, 6.0e-5, 3.0e-10};
+    return {9.8e-12, 3.1e-12, 1.8e-12, 8.6e-11, 6.0e-5, 3.0e-10};
+}
+} // namespace
+
+int pb_autotune(pb_params *p, int ns, int nt, pb_tune_report *rep)
+{
+    if (!p || !(p->alpha > 0) || !(p->L[0] > 0) || !(p->L[1] > 0) || !(p->L[2] > 0) || ns < 0 || nt < 0)
+        return PB_ERR_INVALID;
+    const double tol = p->tol > 0 ? p->tol : 1e-6;
+    const double lt = std::log(1.0 / tol);
+    const double Lmin = std::fmin(p->L[0], std::fmin(p->L[1], p->L[2]));
+    const double vol = p->L[0] * p->L[1] * p->L[2];
+    const cost_model cm = model_for(p->precision);
+    static const int Ps[] = {4, 6, 8, 10, 12, 16, 20, 24};
+    double best = 1e300;
+    pb_params bestp = *p;
+    pb_tune_report br{};
+    int scored = 0;
+    for (int P : Ps) {
+        const double m = std::sqrt(M_PI * P);
+        const double ew = std::exp(-M_PI * P / 2);
+        if (ew > tol) continue;
+        // grids: one spacing h for all axes, each axis rounded up to a power of two
+        for (int k = 2; k <= 9; k++) {
+            const double h = Lmin / (1 << k);
+            int g[3];
+            bool ok = true;
+            for (int i = 0; i < 3; i++) {
+                int n = 4;
+                while (n < 512 && p->L[i] / n > h * (1 + 1e-12)) n *= 2;
+                if (p->L[i] / n > h * (1 + 1e-12) || P > n) ok = false;
+                g[i] = n;
+            }
+            if (!ok) continue;
+            double kmax = 1e300, hx = p->L[0] / g[0];
+            for (int i = 0; i < 3; i++) kmax = std::fmin(kmax, M_PI * g[i] / p->L[i]);
+            const double w = P * hx / m, eta = w * w / 2;
+            const double d = 2.0 * std::sqrt((lt + 2.0) / (kmax * kmax) + eta / 2);
+            const double rc = d * std::sqrt(lt + 1.0);
+            if (rc > 3 * Lmin) continue;
+            const double pairs = (double)nt * ns / vol * (4.0 / 3.0) * M_PI * rc * rc * rc;
+            const double ng = (double)g[0] * g[1] * g[2];
+            const double P3 = (double)P * P * P;
+            const double t_real = pairs * cm.pair_s;
+            const double t_k = ns * P3 * cm.spread_s + nt * P3 * cm.gather_s + ng * cm.fft_s;
+            const double t_bin = cm.fixed_s + (ns + nt) * cm.bin_s;
+            const double t = t_real + t_k + t_bin;
+            scored++;
+            if (t < best) {
+                best = t;
+                bestp = *p;
+                bestp.grid[0] = g[0];
+                bestp.grid[1] = g[1];
+                bestp.grid[2] = g[2];
+                bestp.P = P;
+                bestp.m = m;
+                bestp.d = d;
+                bestp.rc = rc;
+                br.err_real = std::exp(-(rc / d) * (rc / d));
+                br.err_kspace = std::exp(-(d * d / 4 - eta / 2) * kmax * kmax);
+                br.err_window = ew;
+                br.pairs = pairs;
+                br.t_real_ms = t_real * 1e3;
+                br.t_kspace_ms = t_k * 1e3;
+                br.t_bin_ms = t_bin * 1e3;
+                br.t_total_ms = t * 1e3;
+            }
+        }
+    }
+    if (!scored) return PB_ERR_UNSUPPORTED;
+    br.candidates = scored;
+    *p = bestp;
+    if (rep) *rep = br;
+    return PB_OK;
+}
+
 // ---- xoshiro256** seeded by SplitMix64 -------------------------------------
 static inline unsigned long long rotl(unsigned long long x, int k)
 {
