@@ -163,6 +163,18 @@ int pb_velocity_real(pb_context *ctx, const pb_params *p, int ns, const double *
 int pb_velocity_kspace(pb_context *ctx, const pb_params *p, int ns, const double *xn,
                        const double *f, int nt, const double *y, double *u, void *stream);
 
+/* Many right-hand sides on one particle set: u (device, double, R blocks of
+ * 3 x nt; block r at r*3*nt) = velocities of the R force sets f (device,
+ * float, R blocks of 3 x ns; block r at r*3*ns) at the nt targets y, real
+ * space plus k space, OVERWRITTEN.  Real space is the contraction
+ * [3 T x 3 S] . [3 S x R] of the FP32 kernel matrix with the forces on the
+ * matrix cores (FP32 accumulation); k space runs in FP32 per force set.
+ * Always the mixed form (p->precision is not used); p->kernel must be a table
+ * kernel.  No reference counterpart: the reference evaluates one force set
+ * per call (periodicBrinkman.h:432, fftBrinkman.h:34). */
+int pb_velocity_multirhs(pb_context *ctx, const pb_params *p, int ns, const double *xn, int R,
+                         const float *f, int nt, const double *y, double *u, void *stream);
+
 /* Statistics of the last pb_velocity call on ctx (device work counters read
  * back synchronously): pairs evaluated in FP64 and FP32. */
 int pb_last_pair_counts(pb_context *ctx, long long *n_fp64, long long *n_fp32);

@@ -96,6 +96,7 @@ def _declare(L):
     L.pb_velocity.argtypes = vel
     L.pb_velocity_real.argtypes = vel
     L.pb_velocity_kspace.argtypes = vel
+    L.pb_velocity_multirhs.argtypes = [_VP, ctypes.POINTER(_Params), ii, _VP, ii, _VP, ii, _VP, _VP, _VP]
     L.pb_last_pair_stats.argtypes = [_VP, ctypes.POINTER(ctypes.c_longlong)]
     L.pb_last_pair_counts.argtypes = [_VP, ctypes.POINTER(ctypes.c_longlong),
                                       ctypes.POINTER(ctypes.c_longlong)]
@@ -250,6 +251,23 @@ class Context:
 
     def velocity_kspace(self, params, xn, f, y, u=None, stream=None):
         return self._vel(lib().pb_velocity_kspace, params, xn, f, y, u, stream)
+
+    def velocity_multirhs(self, params, xn, F, y, u=None, stream=None):
+        """Velocities of R force sets on one particle set (pb_velocity_multirhs).
+        xn (ns, 3) and y (nt, 3) float64, F (R, ns, 3) float32 device tensors;
+        returns u (R, nt, 3) float64."""
+        import torch
+        if not (F.is_cuda and F.dtype == torch.float32 and F.is_contiguous() and F.dim() == 3):
+            raise PBError(PB_ERR_INVALID, "expected a contiguous float32 CUDA tensor (R, ns, 3)")
+        R, nt = F.shape[0], y.shape[0]
+        if u is None:
+            u = torch.empty((R, nt, 3), dtype=torch.float64, device=y.device)
+        p = params._struct()
+        st = lib().pb_velocity_multirhs(self.handle, ctypes.byref(p), int(xn.shape[0]), _dptr(xn), int(R),
+                                        ctypes.c_void_p(F.data_ptr()), int(nt), _dptr(y), _dptr(u),
+                                        _stream_ptr(stream))
+        _check(st, self)
+        return u
 
     PHASES = ("binning", "spread", "fft_forward", "multiply", "fft_inverse", "gather", "pair")
 

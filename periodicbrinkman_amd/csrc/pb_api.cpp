@@ -363,6 +363,7 @@ int pb_destroy(pb_context *c)
     pb_buffer *bufs[] = {&c->cell_count, &c->cell_start, &c->cell_of_s, &c->rank_s, &c->cell_of_t,
                          &c->rank_t, &c->tcell_count, &c->tcell_start, &c->spos, &c->sfrc, &c->tpos,
                          &c->tperm, &c->counters, &c->scan_tmp, &c->qcount, &c->qoff, &c->tiles, &c->grid, &c->spec,
+                         &c->sperm, &c->mrhs_f,
                          &c->twiddle, &c->axis_fac, &c->dense_tmp, &c->kparams, &c->table};
     for (pb_buffer *b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
@@ -423,6 +424,44 @@ int pb_velocity_kspace(pb_context *ctx, const pb_params *p, int ns, const double
                        const double *f, int nt, const double *y, double *u, void *stream)
 {
     return velocity_impl(ctx, p, ns, xn, f, nt, y, u, stream, false, true);
+}
+
+int pb_velocity_multirhs(pb_context *c, const pb_params *p, int ns, const double *xn, int R,
+                         const float *f, int nt, const double *y, double *u, void *stream)
+{
+    if (!c) return PB_ERR_INVALID;
+    int st;
+    if ((st = check_params(c, p))) return st;
+    if (ns < 0 || nt < 0 || R < 1) return pb_fail(c, PB_ERR_INVALID, "negative point count or R < 1");
+    if ((ns > 0 && (!xn || !f)) || (nt > 0 && (!y || !u))) return pb_fail(c, PB_ERR_INVALID, "null point array");
+    if (p->kernel == PB_KERNEL_DIRECT)
+        return pb_fail(c, PB_ERR_UNSUPPORTED, "the multi-RHS path evaluates the radial table (PB_KERNEL_TABLE)");
+    if ((st = pb_hip_check(c, hipSetDevice(c->device), "hipSetDevice"))) return st;
+    hipStream_t s = (hipStream_t)stream;
+    pb_box box;
+    pb_fill_box(&box, p->L);
+    pb_cells cells;
+    if ((st = make_cells(c, p, &cells))) return st;
+    const int RP = (R + 31) / 32 * 32;
+    if ((st = pb_bin_points(c, s, cells, box, ns, xn, nullptr, true))) return st;
+    if ((st = pb_bin_points(c, s, cells, box, nt, y, nullptr, false))) return st;
+    if ((st = pb_sort_forces(c, s, ns, R, RP, f))) return st;
+    if (nt == 0) return PB_OK;
+    pb_kgrid g;
+    if ((st = make_kgrid(c, p, &g))) return st;
+    for (int r = 0; r < R; r++) {
+        if ((st = pb_force_set(c, s, ns, r, RP))) return st;
+        if ((st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u + (size_t)r * 3 * nt, false,
+                                PB_PREC_MIXED)))
+            return st;
+    }
+    double rfar = std::fmax(3 * p->d, 8 * p->eps);
+    pb_params pt = *p;
+    pt.kernel = PB_KERNEL_TABLE;
+    const pb_kparams *kp = nullptr;
+    size_t ntab = 0;
+    if ((st = device_kparams(c, s, &pt, rfar, &kp, &ntab))) return st;
+    return pb_run_mrhs_real(c, s, cells, kp, ntab, box, ns, nt, R, RP, u);
 }
 
 int pb_last_pair_stats(pb_context *c, long long out[4])

@@ -213,6 +213,7 @@ int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_
     if ((st = pb_reserve(c, &opos, sizeof(double4) * nn))) return st;
     if (is_source) {
         if ((st = pb_reserve(c, &c->sfrc, sizeof(double4) * nn))) return st;
+        if ((st = pb_reserve(c, &c->sperm, sizeof(int) * nn))) return st;
     } else {
         if ((st = pb_reserve(c, &c->tperm, sizeof(int) * nn))) return st;
     }
@@ -229,8 +230,8 @@ int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_
         hipLaunchKernelGGL(k_bin_scatter, dim3(blocks), dim3(256), 0, s, n, pos, frc, box,
                            (const int *)start.ptr, (const int *)key_of.ptr,
                            (const int *)rank.ptr, (double4 *)opos.ptr,
-                           is_source ? (double4 *)c->sfrc.ptr : nullptr,
-                           is_source ? nullptr : (int *)c->tperm.ptr);
+                           (is_source && frc) ? (double4 *)c->sfrc.ptr : nullptr,
+                           is_source ? (int *)c->sperm.ptr : (int *)c->tperm.ptr);
     }
     if (!is_source && (st = pb_build_tiles(c, s, cells, n, &c->max_tiles))) return st;
     return pb_hip_check(c, hipGetLastError(), "binning launch");

@@ -127,6 +127,8 @@ struct pb_context {
     pb_buffer cell_count, cell_start, cell_of_s, rank_s, cell_of_t, rank_t, tcell_count,
         tcell_start;
     pb_buffer spos, sfrc, tpos, tperm, counters, scan_tmp;
+    pb_buffer sperm;  // sorted source -> caller's index
+    pb_buffer mrhs_f; // forces of the sorted sources for many right-hand sides
     pb_buffer qcount, qoff, tiles; // target tiles: int2 (first, count); count at qoff[nkey_t/16]
     int max_tiles = 0;             // bound on the tile count (launch size)
     // k space
@@ -164,6 +166,15 @@ int pb_hip_check(pb_context *c, hipError_t e, const char *what);
 // ---- launchers (pb_bin.hip, pb_real.hip, pb_kspace.hip, pb_dense.hip) -------
 int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_box &box, int n,
                   const double *pos, const double *frc, bool is_source);
+// Many right-hand sides (pb_mrhs.hip): Fs[j][b][rp] from the caller's R force
+// sets (float, set r at r * 3 ns) padded to RP columns; the FP64 source forces
+// of set r for the k-space pass; the real-space contraction added into u
+// (double, set r at r * 3 nt).
+int pb_sort_forces(pb_context *c, hipStream_t s, int ns, int R, int RP, const float *f);
+int pb_force_set(pb_context *c, hipStream_t s, int ns, int r, int RP);
+int pb_run_mrhs_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp,
+                     size_t table_doubles, const pb_box &box, int ns, int nt, int R, int RP, double *u);
+
 // Target tiles: runs of <= 64 sorted targets inside one quarter brick; the
 // tile count is left on the device, *max_tiles bounds it.  Built by
 // pb_bin_points for targets.

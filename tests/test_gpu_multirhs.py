@@ -1,0 +1,55 @@
+"""Many right-hand sides on one particle set (pb_velocity_multirhs): every
+force set against the FP64 oracle and against the single-set mixed path."""
+import numpy as np
+import pytest
+
+import periodicbrinkman_amd as pb
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def relL2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+@pytest.mark.parametrize("R,same", [(5, False), (40, True)])
+def test_multirhs_matches_oracle(R, same):
+    import torch
+    ns, nt = 3000, 2000
+    L = (1.0, 1.0, 1.0)
+    x, _, y = pb.uniform_system(pb.config_seed(4), ns, nt, L)
+    if same:
+        y = x[:nt].copy()  # targets on sources: self pairs
+    rng = np.random.default_rng(7)
+    F = rng.standard_normal((R, ns, 3)).astype(np.float32)
+    p = pb.Params(L=L, alpha=10.0, eps=1e-3, grid=(32, 32, 32), P=12, tol=1e-7,
+                  precision="mixed").tune(ns, nt)
+    ctx = pb.Context(0)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+    u = ctx.velocity_multirhs(p, dev(x), dev(F), dev(y)).cpu().numpy()
+    torch.cuda.synchronize()
+    assert u.shape == (R, nt, 3)
+    for r in sorted({0, R // 2, R - 1}):
+        f64 = F[r].astype(np.float64)
+        want = po.velocity_ewald(y, x, f64, L, p.alpha, p.d, p.eps, p.rc, p.grid, (p.P,) * 3, p.m)
+        assert relL2(u[r], want) < 1e-6, r
+        single = ctx.velocity(p, dev(x), dev(f64), dev(y)).cpu().numpy()
+        assert relL2(u[r], single) < 1e-6, r
+    # linearity across sets: the sum of two sets is the velocity of their sum
+    F2 = np.stack([F[0] + F[1]])
+    u2 = ctx.velocity_multirhs(p, dev(x), dev(F2.astype(np.float32)), dev(y)).cpu().numpy()
+    assert relL2(u2[0], u[0] + u[1]) < 1e-6
+    ctx.close()
+
+
+def test_multirhs_rejects_closed_form_kernel():
+    import torch
+    ctx = pb.Context(0)
+    p = pb.Params(alpha=10.0, eps=1e-3, grid=(16, 16, 16), P=8, tol=1e-5, kernel="direct").tune(10, 10)
+    x = torch.rand((10, 3), dtype=torch.float64, device="cuda")
+    F = torch.rand((2, 10, 3), dtype=torch.float32, device="cuda")
+    with pytest.raises(pb.PBError) as ei:
+        ctx.velocity_multirhs(p, x, F, x)
+    assert ei.value.status == pb.PB_ERR_UNSUPPORTED
+    ctx.close()
