@@ -163,6 +163,19 @@ int pb_velocity_real(pb_context *ctx, const pb_params *p, int ns, const double *
 int pb_velocity_kspace(pb_context *ctx, const pb_params *p, int ns, const double *xn,
                        const double *f, int nt, const double *y, double *u, void *stream);
 
+/* Many independent periodic systems in one call.  System i has the sources
+ * src_off[i] .. src_off[i+1]-1 of xn and f and the targets tgt_off[i] ..
+ * tgt_off[i+1]-1 of y and u (device arrays of 3 x n doubles; offsets are host
+ * arrays of nsys + 1 points, starting at 0) and its own parameters params[i]
+ * (box, alpha, eps, precision, kernel, tol); a system whose d or grid is
+ * unset is tuned by pb_autotune, which writes the choice back into
+ * params[i].  The kernel constants and tables of all systems are built and
+ * uploaded once, then every system is queued on `stream` without further
+ * host synchronization.  u is OVERWRITTEN.  No reference counterpart: the
+ * reference evaluates one system per call. */
+int pb_velocity_batched(pb_context *ctx, int nsys, pb_params *params, const int *src_off, const double *xn,
+                        const double *f, const int *tgt_off, const double *y, double *u, void *stream);
+
 /* Many right-hand sides on one particle set: u (device, double, R blocks of
  * 3 x nt; block r at r*3*nt) = velocities of the R force sets f (device,
  * float, R blocks of 3 x ns; block r at r*3*ns) at the nt targets y, real

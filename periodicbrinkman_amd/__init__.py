@@ -96,6 +96,8 @@ def _declare(L):
     L.pb_velocity.argtypes = vel
     L.pb_velocity_real.argtypes = vel
     L.pb_velocity_kspace.argtypes = vel
+    L.pb_velocity_batched.argtypes = [_VP, ii, ctypes.POINTER(_Params), ctypes.POINTER(ctypes.c_int), _VP, _VP,
+                                      ctypes.POINTER(ctypes.c_int), _VP, _VP, _VP]
     L.pb_velocity_multirhs.argtypes = [_VP, ctypes.POINTER(_Params), ii, _VP, ii, _VP, ii, _VP, _VP, _VP]
     L.pb_last_pair_stats.argtypes = [_VP, ctypes.POINTER(ctypes.c_longlong)]
     L.pb_last_pair_counts.argtypes = [_VP, ctypes.POINTER(ctypes.c_longlong),
@@ -251,6 +253,26 @@ class Context:
 
     def velocity_kspace(self, params, xn, f, y, u=None, stream=None):
         return self._vel(lib().pb_velocity_kspace, params, xn, f, y, u, stream)
+
+    def velocity_batched(self, params, xn, f, src_off, y, tgt_off, u=None, stream=None):
+        """Independent periodic systems in one call (pb_velocity_batched).
+        params: list of Params (systems with unset d or grid are autotuned and
+        updated in place); xn, f (sum ns, 3) and y (sum nt, 3) float64 device
+        tensors; src_off, tgt_off: host sequences of nsys + 1 offsets."""
+        import torch
+        n = len(params)
+        structs = (_Params * n)(*[q._struct() for q in params])
+        so = (ctypes.c_int * (n + 1))(*[int(v) for v in src_off])
+        to = (ctypes.c_int * (n + 1))(*[int(v) for v in tgt_off])
+        if u is None:
+            u = torch.empty((y.shape[0], 3), dtype=torch.float64, device=y.device)
+        st = lib().pb_velocity_batched(self.handle, n, structs, so, _dptr(xn), _dptr(f), to, _dptr(y), _dptr(u),
+                                       _stream_ptr(stream))
+        _check(st, self)
+        for q, sp in zip(params, structs):
+            q.d, q.rc, q.m, q.P = sp.d, sp.rc, sp.m, sp.P
+            q.grid = tuple(sp.grid[i] for i in range(3))
+        return u
 
     def velocity_multirhs(self, params, xn, F, y, u=None, stream=None):
         """Velocities of R force sets on one particle set (pb_velocity_multirhs).

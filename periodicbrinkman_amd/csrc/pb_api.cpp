@@ -7,6 +7,8 @@
 
 #include "pb_internal.h"
 
+#include <algorithm>
+
 // ---- context helpers --------------------------------------------------------
 int pb_fail(pb_context *c, int status, const std::string &msg)
 {
@@ -169,8 +171,24 @@ const pb_kparams &cached_kparams(double a, double d, double e, double rfar)
     return k.kp;
 }
 
-// Device copy of the kernel constants (and, for PB_KERNEL_TABLE, the radial
-// table on [rcommon, rc]) for the given parameters, rebuilt on change.
+// Host copy of the kernel constants and (table kernels) the radial table on
+// [rcommon, rc]: coefficients in coef (empty without a table); the table
+// pointers of kp are left for the caller.
+void host_kparams(const pb_params *p, double rfar, pb_kparams *kp, std::vector<double> &coef)
+{
+    *kp = cached_kparams(p->alpha, p->d, p->eps, rfar);
+    kp->tn = 0;
+    kp->tab64 = nullptr;
+    kp->tab32 = nullptr;
+    coef.clear();
+    if (p->kernel != PB_KERNEL_DIRECT && p->rc > kp->rcommon) {
+        pb_build_table(kp, kp->rcommon / p->d, p->rc / p->d * (1 + 1e-9), coef);
+        if (kp->tn == 0) coef.clear();
+    }
+}
+
+// Device copy of the kernel constants and table for the given parameters,
+// rebuilt on change.
 int device_kparams(pb_context *c, hipStream_t s, const pb_params *p, double rfar,
                    const pb_kparams **out, size_t *table_doubles)
 {
@@ -180,26 +198,23 @@ int device_kparams(pb_context *c, hipStream_t s, const pb_params *p, double rfar
     bool same = true;
     for (int i = 0; i < 6; i++) same = same && c->kp_key[i] == key[i];
     if (!same) {
-        pb_kparams kp = cached_kparams(p->alpha, p->d, p->eps, rfar);
-        kp.tn = 0;
+        pb_kparams kp;
+        std::vector<double> coef;
+        host_kparams(p, rfar, &kp, coef);
         c->table_doubles = 0;
-        if (p->kernel != PB_KERNEL_DIRECT && p->rc > kp.rcommon) {
-            std::vector<double> coef;
-            pb_build_table(&kp, kp.rcommon / p->d, p->rc / p->d * (1 + 1e-9), coef);
-            if (kp.tn > 0) {
-                size_t n = coef.size();
-                std::vector<float> c32(coef.begin(), coef.end());
-                if ((st = pb_reserve(c, &c->table, n * (sizeof(double) + sizeof(float))))) return st;
-                kp.tab64 = (const double *)c->table.ptr;
-                kp.tab32 = (const float *)((char *)c->table.ptr + n * sizeof(double));
-                if ((st = pb_hip_check(c, hipMemcpyAsync(c->table.ptr, coef.data(), n * sizeof(double),
-                                                         hipMemcpyHostToDevice, s), "table upload")))
-                    return st;
-                if ((st = pb_hip_check(c, hipMemcpyAsync((void *)kp.tab32, c32.data(), n * sizeof(float),
-                                                         hipMemcpyHostToDevice, s), "table upload")))
-                    return st;
-                c->table_doubles = n;
-            }
+        if (!coef.empty()) {
+            size_t n = coef.size();
+            std::vector<float> c32(coef.begin(), coef.end());
+            if ((st = pb_reserve(c, &c->table, n * (sizeof(double) + sizeof(float))))) return st;
+            kp.tab64 = (const double *)c->table.ptr;
+            kp.tab32 = (const float *)((char *)c->table.ptr + n * sizeof(double));
+            if ((st = pb_hip_check(c, hipMemcpyAsync(c->table.ptr, coef.data(), n * sizeof(double),
+                                                     hipMemcpyHostToDevice, s), "table upload")))
+                return st;
+            if ((st = pb_hip_check(c, hipMemcpyAsync((void *)kp.tab32, c32.data(), n * sizeof(float),
+                                                     hipMemcpyHostToDevice, s), "table upload")))
+                return st;
+            c->table_doubles = n;
         }
         if ((st = pb_hip_check(c, hipMemcpyAsync(c->kparams.ptr, &kp, sizeof(kp), hipMemcpyHostToDevice, s),
                                "kparams upload")))
@@ -209,6 +224,41 @@ int device_kparams(pb_context *c, hipStream_t s, const pb_params *p, double rfar
     }
     *out = (const pb_kparams *)c->kparams.ptr;
     *table_doubles = c->table_doubles;
+    return PB_OK;
+}
+
+// FP32 far-field pairs: the closed-form kernel in mixed precision, or the
+// table on request (on gfx950 FP64 FMA runs at the non-packed FP32 rate, so
+// the FP64 table is the faster default).
+bool far32_of(const pb_params *p, size_t ntab)
+{
+    return p->precision == PB_PREC_MIXED &&
+           (p->kernel == PB_KERNEL_DIRECT || (p->kernel == PB_KERNEL_TABLE_FAR32 && ntab));
+}
+
+// The device work of one evaluation on stream s with kernel constants already
+// on the device (kp, ntab): binning, k space (overwrite u), real space (add).
+int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_kparams *kp, size_t ntab,
+                     int ns, const double *xn, const double *f, int nt, const double *y, double *u, bool do_real,
+                     bool do_k)
+{
+    int st;
+    pb_box box;
+    pb_fill_box(&box, p->L);
+    pb_cells cells;
+    if ((st = make_cells(c, p, &cells))) return st;
+    if ((st = pb_bin_points(c, s, cells, box, ns, xn, f, true))) return st;
+    if ((st = pb_bin_points(c, s, cells, box, nt, y, nullptr, false))) return st;
+    for (int i = 1; i < PB_NPHASE; i++) pb_mark(c, s, i);
+    bool acc = false;
+    if (do_k) {
+        pb_kgrid g;
+        if ((st = make_kgrid(c, p, &g))) return st;
+        if ((st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u, false, p->precision)))
+            return st;
+        acc = true;
+    }
+    if (do_real && (st = pb_run_real(c, s, cells, kp, ntab, box, ns, nt, u, acc, far32_of(p, ntab)))) return st;
     return PB_OK;
 }
 
@@ -223,34 +273,11 @@ int velocity_impl(pb_context *c, const pb_params *p, int ns, const double *xn, c
         return pb_fail(c, PB_ERR_INVALID, "null point array");
     if ((st = pb_hip_check(c, hipSetDevice(c->device), "hipSetDevice"))) return st;
     hipStream_t s = (hipStream_t)stream;
-    pb_box box;
-    pb_fill_box(&box, p->L);
-    pb_cells cells;
-    if ((st = make_cells(c, p, &cells))) return st;
+    const pb_kparams *kp = nullptr;
+    size_t ntab = 0;
+    if (do_real && (st = device_kparams(c, s, p, std::fmax(3 * p->d, 8 * p->eps), &kp, &ntab))) return st;
     if ((st = timing_begin(c, s))) return pb_fail(c, st, "timing events");
-    if ((st = pb_bin_points(c, s, cells, box, ns, xn, f, true))) return st;
-    if ((st = pb_bin_points(c, s, cells, box, nt, y, nullptr, false))) return st;
-    for (int i = 1; i < PB_NPHASE; i++) pb_mark(c, s, i);
-    bool acc = false;
-    if (do_k) {
-        pb_kgrid g;
-        if ((st = make_kgrid(c, p, &g))) return st;
-        if ((st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u, false, p->precision)))
-            return st;
-        acc = true;
-    }
-    if (do_real) {
-        double rfar = std::fmax(3 * p->d, 8 * p->eps);
-        const pb_kparams *kp = nullptr;
-        size_t ntab = 0;
-        if ((st = device_kparams(c, s, p, rfar, &kp, &ntab))) return st;
-        // FP32 far-field pairs: the closed-form kernel in mixed precision, or
-        // the table on request (on gfx950 FP64 FMA runs at the non-packed
-        // FP32 rate, so the FP64 table is the faster default)
-        const bool far32 = p->precision == PB_PREC_MIXED &&
-                           (p->kernel == PB_KERNEL_DIRECT || (p->kernel == PB_KERNEL_TABLE_FAR32 && ntab));
-        if ((st = pb_run_real(c, s, cells, kp, ntab, box, ns, nt, u, acc, far32))) return st;
-    }
+    if ((st = enqueue_velocity(c, s, p, kp, ntab, ns, xn, f, nt, y, u, do_real, do_k))) return st;
     pb_mark(c, s, PB_NPHASE);
     return PB_OK;
 }
@@ -364,7 +391,7 @@ int pb_destroy(pb_context *c)
                          &c->rank_t, &c->tcell_count, &c->tcell_start, &c->spos, &c->sfrc, &c->tpos,
                          &c->tperm, &c->counters, &c->scan_tmp, &c->qcount, &c->qoff, &c->tiles, &c->grid, &c->spec,
                          &c->sperm, &c->mrhs_f,
-                         &c->twiddle, &c->axis_fac, &c->dense_tmp, &c->kparams, &c->table};
+                         &c->twid[0], &c->twid[1], &c->batch_kp, &c->batch_tab, &c->axis_fac, &c->dense_tmp, &c->kparams, &c->table};
     for (pb_buffer *b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     if (c->ev_ready)
@@ -462,6 +489,81 @@ int pb_velocity_multirhs(pb_context *c, const pb_params *p, int ns, const double
     size_t ntab = 0;
     if ((st = device_kparams(c, s, &pt, rfar, &kp, &ntab))) return st;
     return pb_run_mrhs_real(c, s, cells, kp, ntab, box, ns, nt, R, RP, u);
+}
+
+int pb_velocity_batched(pb_context *c, int nsys, pb_params *params, const int *src_off, const double *xn,
+                        const double *f, const int *tgt_off, const double *y, double *u, void *stream)
+{
+    if (!c) return PB_ERR_INVALID;
+    if (nsys < 0 || (nsys > 0 && (!params || !src_off || !tgt_off)))
+        return pb_fail(c, PB_ERR_INVALID, "null system description");
+    if (nsys == 0) return PB_OK;
+    if (src_off[0] != 0 || tgt_off[0] != 0) return pb_fail(c, PB_ERR_INVALID, "offsets must start at 0");
+    int st;
+    for (int i = 0; i < nsys; i++) {
+        const int ns = src_off[i + 1] - src_off[i], nt = tgt_off[i + 1] - tgt_off[i];
+        if (ns < 0 || nt < 0) return pb_fail(c, PB_ERR_INVALID, "offsets must not decrease");
+        pb_params &p = params[i];
+        if (!(p.d > 0) || p.grid[0] <= 0 || p.grid[1] <= 0 || p.grid[2] <= 0) {
+            // unset split or grid: the autotuner chooses them for this system
+            if ((st = pb_autotune(&p, ns, nt, nullptr))) return pb_fail(c, st, "autotune failed for a system");
+        }
+        if ((st = check_params(c, &p))) return st;
+    }
+    if ((src_off[nsys] > 0 && (!xn || !f)) || (tgt_off[nsys] > 0 && (!y || !u)))
+        return pb_fail(c, PB_ERR_INVALID, "null point array");
+    if ((st = pb_hip_check(c, hipSetDevice(c->device), "hipSetDevice"))) return st;
+    hipStream_t s = (hipStream_t)stream;
+    // kernel constants and tables of every system, uploaded together
+    std::vector<pb_kparams> kps(nsys);
+    std::vector<size_t> toff(nsys + 1, 0);
+    std::vector<double> coefs;
+    for (int i = 0; i < nsys; i++) {
+        std::vector<double> coef;
+        host_kparams(&params[i], std::fmax(3 * params[i].d, 8 * params[i].eps), &kps[i], coef);
+        toff[i + 1] = toff[i] + coef.size();
+        coefs.insert(coefs.end(), coef.begin(), coef.end());
+    }
+    const size_t ntot = coefs.size();
+    std::vector<float> c32(coefs.begin(), coefs.end());
+    if ((st = pb_reserve(c, &c->batch_kp, sizeof(pb_kparams) * nsys))) return st;
+    if ((st = pb_reserve(c, &c->batch_tab, (ntot ? ntot : 1) * (sizeof(double) + sizeof(float))))) return st;
+    const double *tab64 = (const double *)c->batch_tab.ptr;
+    const float *tab32 = (const float *)((const char *)c->batch_tab.ptr + ntot * sizeof(double));
+    for (int i = 0; i < nsys; i++)
+        if (toff[i + 1] > toff[i]) {
+            kps[i].tab64 = tab64 + toff[i];
+            kps[i].tab32 = tab32 + toff[i];
+        }
+    if (ntot) {
+        if ((st = pb_hip_check(c, hipMemcpyAsync(c->batch_tab.ptr, coefs.data(), ntot * sizeof(double),
+                                                 hipMemcpyHostToDevice, s), "table upload")))
+            return st;
+        if ((st = pb_hip_check(c, hipMemcpyAsync((void *)tab32, c32.data(), ntot * sizeof(float),
+                                                 hipMemcpyHostToDevice, s), "table upload")))
+            return st;
+    }
+    if ((st = pb_hip_check(c, hipMemcpyAsync(c->batch_kp.ptr, kps.data(), sizeof(pb_kparams) * nsys,
+                                             hipMemcpyHostToDevice, s), "kparams upload")))
+        return st;
+    if ((st = pb_hip_check(c, hipStreamSynchronize(s), "batch upload"))) return st;
+    // largest systems first, so the scratch grows at most a few times
+    std::vector<int> order(nsys);
+    for (int i = 0; i < nsys; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const double wa = (double)params[a].grid[0] * params[a].grid[1] * params[a].grid[2] + src_off[a + 1] - src_off[a];
+        const double wb = (double)params[b].grid[0] * params[b].grid[1] * params[b].grid[2] + src_off[b + 1] - src_off[b];
+        return wa > wb;
+    });
+    const pb_kparams *kpd = (const pb_kparams *)c->batch_kp.ptr;
+    for (int i : order) {
+        const int ns = src_off[i + 1] - src_off[i], nt = tgt_off[i + 1] - tgt_off[i];
+        if ((st = enqueue_velocity(c, s, &params[i], kpd + i, toff[i + 1] - toff[i], ns,
+                                   xn + 3 * (size_t)src_off[i], f + 3 * (size_t)src_off[i], nt,
+                                   y + 3 * (size_t)tgt_off[i], u + 3 * (size_t)tgt_off[i], true, true)))
+            return st;
+    }
+    return PB_OK;
 }
 
 int pb_last_pair_stats(pb_context *c, long long out[4])

@@ -132,9 +132,11 @@ struct pb_context {
     pb_buffer qcount, qoff, tiles; // target tiles: int2 (first, count); count at qoff[nkey_t/16]
     int max_tiles = 0;             // bound on the tile count (launch size)
     // k space
-    pb_buffer grid, spec, twiddle, axis_fac; // real grid (T4 per point), half spectrum
-    int twiddle_n = 0;
-    int twiddle_prec = -1;
+    pb_buffer grid, spec, axis_fac; // real grid (T4 per point), half spectrum
+    pb_buffer twid[2];              // FFT twiddles for n = 4 .. 512: [0] FP64, [1] FP32
+    int twid_ok[2] = {0, 0};
+    // batched systems: kernel constants and radial tables of every system
+    pb_buffer batch_kp, batch_tab;
     // dense / shells
     pb_buffer dense_tmp;
     // kernel constants in device memory (kernels take a pointer: a by-value

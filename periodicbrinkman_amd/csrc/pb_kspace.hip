@@ -927,35 +927,30 @@ int fft_pass(hipStream_t s, int pass, const pb_kgrid &g, void *grid4, void *spec
     return PB_OK;
 }
 
-// Twiddles e^{-2 pi i q/n}, q < n, for the three axes at offsets 0, n0,
-// n0 + n1 (computed in long double).
+// Twiddles e^{-2 pi i q/n}, q < n, for every supported length n = 4 .. 512
+// (table n at offset n - 4), computed in long double and uploaded once per
+// precision, so calls with different grids need no further upload.
 template <typename T>
-int upload_twiddles(pb_context *c, hipStream_t s, const pb_kgrid &g)
+int upload_twiddles(pb_context *c, hipStream_t s)
 {
     using C = typename cplx<T>::type;
-    int prec = sizeof(T) == 8 ? 0 : 1;
-    int key = (g.n[0] * 1024 + g.n[1]) * 1024 + g.n[2];
-    if (c->twiddle_n == key && c->twiddle_prec == prec) return PB_OK;
-    size_t tot = (size_t)g.n[0] + g.n[1] + g.n[2];
+    const int prec = sizeof(T) == 8 ? 0 : 1;
+    if (c->twid_ok[prec]) return PB_OK;
+    const size_t tot = 1020; // 4 + 8 + ... + 512
     std::vector<C> h(tot);
-    size_t off = 0;
-    for (int a = 0; a < 3; a++) {
-        int n = g.n[a];
+    for (int n = 4; n <= 512; n *= 2)
         for (int k = 0; k < n; k++) {
             long double ang = -2.0L * 3.141592653589793238462643383279502884L * k / n;
-            h[off + k].x = (T)cosl(ang);
-            h[off + k].y = (T)sinl(ang);
+            h[n - 4 + k].x = (T)cosl(ang);
+            h[n - 4 + k].y = (T)sinl(ang);
         }
-        off += n;
-    }
     int st;
-    if ((st = pb_reserve(c, &c->twiddle, tot * sizeof(C)))) return st;
-    if ((st = pb_hip_check(c, hipMemcpyAsync(c->twiddle.ptr, h.data(), tot * sizeof(C),
+    if ((st = pb_reserve(c, &c->twid[prec], tot * sizeof(C)))) return st;
+    if ((st = pb_hip_check(c, hipMemcpyAsync(c->twid[prec].ptr, h.data(), tot * sizeof(C),
                                              hipMemcpyHostToDevice, s), "twiddle upload")))
         return st;
     if ((st = pb_hip_check(c, hipStreamSynchronize(s), "twiddle sync"))) return st;
-    c->twiddle_n = key;
-    c->twiddle_prec = prec;
+    c->twid_ok[prec] = 1;
     return PB_OK;
 }
 
@@ -978,7 +973,7 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
     // after the inverse transform
     if ((st = pb_reserve(c, &c->grid, (size_t)g.ng * sizeof(C4)))) return st;
     if ((st = pb_reserve(c, &c->spec, 3 * (size_t)g.n[0] * g.n[1] * h2 * sizeof(C)))) return st;
-    if ((st = upload_twiddles<T>(c, s, g))) return st;
+    if ((st = upload_twiddles<T>(c, s))) return st;
     // per-axis factors e^{-(d^2/4 - eta/2) k^2}, axis 2 only for kz <= n2/2
     std::vector<double> fac((size_t)g.n[0] + g.n[1] + g.n[2]);
     double beta = d * d / 4 - g.eta / 2;
@@ -1011,8 +1006,8 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
                            (const int *)c->cell_start.ptr, (const double4 *)c->spos.ptr,
                            (const double4 *)c->sfrc.ptr, cells, g, (C4 *)grid4);
     }
-    const C *tw = (const C *)c->twiddle.ptr;
-    const void *tws[3] = {tw, tw + g.n[0], tw + g.n[0] + g.n[1]};
+    const C *tw = (const C *)c->twid[sizeof(T) == 8 ? 0 : 1].ptr;
+    const void *tws[3] = {tw + (g.n[0] - 4), tw + (g.n[1] - 4), tw + (g.n[2] - 4)};
     const double *fac_d = (const double *)c->axis_fac.ptr;
     pb_mark(c, s, 2);
     for (int pass = 0; pass < 5; pass++) {

@@ -1,0 +1,62 @@
+"""Independent periodic systems of uneven size in one call
+(pb_velocity_batched): each system against the FP64 oracle with its own
+parameters, and against a single-system call."""
+import numpy as np
+import pytest
+
+import periodicbrinkman_amd as pb
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def relL2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-6)])
+def test_batched_systems_match_oracle(precision, tol):
+    import torch
+    rng = np.random.default_rng(21)
+    sizes = [(120, 100), (2500, 900), (700, 1300), (40, 40), (1600, 1600), (300, 500)]
+    params, xs, fs, ys = [], [], [], []
+    for i, (ns, nt) in enumerate(sizes):
+        Ls = float(rng.uniform(0.5, 2.0))
+        L = (Ls, Ls * float(rng.uniform(0.8, 1.25)), Ls)
+        alpha = float(np.exp(rng.uniform(0, np.log(50))))
+        x, f, y = pb.uniform_system(pb.config_seed(100 + i), ns, nt, L)
+        q = pb.Params(L=L, alpha=alpha, eps=1e-3 * Ls, precision=precision, tol=1e-8 if precision == "fp64" else 1e-7)
+        if i % 2 == 0:  # half the systems with a fixed grid, half autotuned
+            q.grid, q.P = (32, 32, 32), 12
+            q.tune(ns, nt)
+        params.append(q)
+        xs.append(x), fs.append(f), ys.append(y)
+    so = np.concatenate([[0], np.cumsum([s[0] for s in sizes])])
+    to = np.concatenate([[0], np.cumsum([s[1] for s in sizes])])
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+    ctx = pb.Context(0)
+    u = ctx.velocity_batched(params, dev(np.concatenate(xs)), dev(np.concatenate(fs)), so,
+                             dev(np.concatenate(ys)), to).cpu().numpy()
+    torch.cuda.synchronize()
+    for i, q in enumerate(params):
+        assert q.d > 0 and min(q.grid) >= 4
+        ui = u[to[i]:to[i + 1]]
+        want = po.velocity_ewald(ys[i], xs[i], fs[i], q.L, q.alpha, q.d, q.eps, q.rc, q.grid, (q.P,) * 3, q.m)
+        assert relL2(ui, want) < tol, (i, relL2(ui, want))
+        single = ctx.velocity(q, dev(xs[i]), dev(fs[i]), dev(ys[i])).cpu().numpy()
+        # the same kernels; in mixed precision the FP32 spread sums in the
+        # (atomic, run-dependent) source order of the counting sort
+        assert relL2(ui, single) < (1e-12 if precision == "fp64" else 1e-6), i
+    ctx.close()
+
+
+def test_batched_empty_and_invalid():
+    import torch
+    ctx = pb.Context(0)
+    z = torch.zeros((0, 3), dtype=torch.float64, device="cuda")
+    u = ctx.velocity_batched([], z, z, [0], z, [0])
+    assert u.shape == (0, 3)
+    with pytest.raises(pb.PBError) as ei:
+        ctx.velocity_batched([pb.Params(alpha=1.0)], z, z, [1, 0], z, [0, 0])
+    assert ei.value.status == pb.PB_ERR_INVALID
+    ctx.close()
