@@ -8,6 +8,8 @@
 #include "pb_internal.h"
 
 #include <algorithm>
+#include <cstdint>
+#include <thread>
 
 // ---- context helpers --------------------------------------------------------
 int pb_fail(pb_context *c, int status, const std::string &msg)
@@ -387,11 +389,19 @@ int pb_destroy(pb_context *c)
 {
     if (!c) return PB_OK;
     (void)hipSetDevice(c->device);
+    for (int k = 0; k < 4; k++) {
+        if (c->sub[k]) pb_destroy(c->sub[k]);
+        if (c->sub_stream[k]) (void)hipStreamDestroy(c->sub_stream[k]);
+    }
+    for (int k = 0; k < 7; k++)
+        if (c->sub_event[k]) (void)hipEventDestroy(c->sub_event[k]);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
     pb_buffer *bufs[] = {&c->cell_count, &c->cell_start, &c->cell_of_s, &c->rank_s, &c->cell_of_t,
                          &c->rank_t, &c->tcell_count, &c->tcell_start, &c->spos, &c->sfrc, &c->tpos,
                          &c->tperm, &c->counters, &c->scan_tmp, &c->qcount, &c->qoff, &c->tiles, &c->grid, &c->spec,
                          &c->sperm, &c->mrhs_f,
-                         &c->twid[0], &c->twid[1], &c->batch_kp, &c->batch_tab, &c->axis_fac, &c->dense_tmp, &c->kparams, &c->table};
+                         &c->twid[0], &c->twid[1], &c->batch_kp, &c->batch_tab, &c->dense_tmp, &c->kparams, &c->table};
     for (pb_buffer *b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     if (c->ev_ready)
@@ -514,56 +524,185 @@ int pb_velocity_batched(pb_context *c, int nsys, pb_params *params, const int *s
         return pb_fail(c, PB_ERR_INVALID, "null point array");
     if ((st = pb_hip_check(c, hipSetDevice(c->device), "hipSetDevice"))) return st;
     hipStream_t s = (hipStream_t)stream;
-    // kernel constants and tables of every system, uploaded together
-    std::vector<pb_kparams> kps(nsys);
-    std::vector<size_t> toff(nsys + 1, 0);
-    std::vector<double> coefs;
+    // kernel constants and tables of every system, uploaded together; kept on
+    // the device while the systems' parameters do not change
+    std::vector<double> key((size_t)nsys * 6);
     for (int i = 0; i < nsys; i++) {
-        std::vector<double> coef;
-        host_kparams(&params[i], std::fmax(3 * params[i].d, 8 * params[i].eps), &kps[i], coef);
-        toff[i + 1] = toff[i] + coef.size();
-        coefs.insert(coefs.end(), coef.begin(), coef.end());
+        const pb_params &p = params[i];
+        const double k6[6] = {p.alpha, p.d, p.eps, std::fmax(3 * p.d, 8 * p.eps), p.rc, (double)p.kernel};
+        std::copy(k6, k6 + 6, key.begin() + 6 * (size_t)i);
     }
-    const size_t ntot = coefs.size();
-    std::vector<float> c32(coefs.begin(), coefs.end());
-    if ((st = pb_reserve(c, &c->batch_kp, sizeof(pb_kparams) * nsys))) return st;
-    if ((st = pb_reserve(c, &c->batch_tab, (ntot ? ntot : 1) * (sizeof(double) + sizeof(float))))) return st;
-    const double *tab64 = (const double *)c->batch_tab.ptr;
-    const float *tab32 = (const float *)((const char *)c->batch_tab.ptr + ntot * sizeof(double));
-    for (int i = 0; i < nsys; i++)
-        if (toff[i + 1] > toff[i]) {
-            kps[i].tab64 = tab64 + toff[i];
-            kps[i].tab32 = tab32 + toff[i];
+    if (key != c->batch_key) {
+        std::vector<pb_kparams> kps(nsys);
+        std::vector<std::vector<double>> coef(nsys);
+        // distinct parameter sets, built on several host threads
+        std::vector<int> uniq, rep(nsys);
+        for (int i = 0; i < nsys; i++) {
+            rep[i] = i;
+            for (int j : uniq)
+                if (std::equal(key.begin() + 6 * (size_t)i, key.begin() + 6 * (size_t)i + 6, key.begin() + 6 * (size_t)j)) {
+                    rep[i] = j;
+                    break;
+                }
+            if (rep[i] == i) uniq.push_back(i);
         }
-    if (ntot) {
-        if ((st = pb_hip_check(c, hipMemcpyAsync(c->batch_tab.ptr, coefs.data(), ntot * sizeof(double),
-                                                 hipMemcpyHostToDevice, s), "table upload")))
+        const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nth; t++)
+            pool.emplace_back([&, t]() {
+                for (size_t q = t; q < uniq.size(); q += nth) {
+                    const int i = uniq[q];
+                    host_kparams(&params[i], std::fmax(3 * params[i].d, 8 * params[i].eps), &kps[i], coef[i]);
+                }
+            });
+        for (auto &th : pool) th.join();
+        for (int i = 0; i < nsys; i++)
+            if (rep[i] != i) {
+                kps[i] = kps[rep[i]];
+                coef[i] = coef[rep[i]];
+            }
+        std::vector<size_t> toff(nsys + 1, 0);
+        for (int i = 0; i < nsys; i++) toff[i + 1] = toff[i] + coef[i].size();
+        const size_t ntot = toff[nsys];
+        std::vector<double> coefs;
+        coefs.reserve(ntot);
+        for (int i = 0; i < nsys; i++) coefs.insert(coefs.end(), coef[i].begin(), coef[i].end());
+        std::vector<float> c32(coefs.begin(), coefs.end());
+        if ((st = pb_reserve(c, &c->batch_kp, sizeof(pb_kparams) * nsys))) return st;
+        if ((st = pb_reserve(c, &c->batch_tab, (ntot ? ntot : 1) * (sizeof(double) + sizeof(float))))) return st;
+        const double *tab64 = (const double *)c->batch_tab.ptr;
+        const float *tab32 = (const float *)((const char *)c->batch_tab.ptr + ntot * sizeof(double));
+        for (int i = 0; i < nsys; i++)
+            if (toff[i + 1] > toff[i]) {
+                kps[i].tab64 = tab64 + toff[i];
+                kps[i].tab32 = tab32 + toff[i];
+            }
+        if (ntot) {
+            if ((st = pb_hip_check(c, hipMemcpyAsync(c->batch_tab.ptr, coefs.data(), ntot * sizeof(double),
+                                                     hipMemcpyHostToDevice, s), "table upload")))
+                return st;
+            if ((st = pb_hip_check(c, hipMemcpyAsync((void *)tab32, c32.data(), ntot * sizeof(float),
+                                                     hipMemcpyHostToDevice, s), "table upload")))
+                return st;
+        }
+        if ((st = pb_hip_check(c, hipMemcpyAsync(c->batch_kp.ptr, kps.data(), sizeof(pb_kparams) * nsys,
+                                                 hipMemcpyHostToDevice, s), "kparams upload")))
             return st;
-        if ((st = pb_hip_check(c, hipMemcpyAsync((void *)tab32, c32.data(), ntot * sizeof(float),
-                                                 hipMemcpyHostToDevice, s), "table upload")))
+        if ((st = pb_hip_check(c, hipStreamSynchronize(s), "batch upload"))) return st;
+        c->batch_key = key;
+        c->batch_ntab.assign(nsys, 0);
+        for (int i = 0; i < nsys; i++) c->batch_ntab[i] = toff[i + 1] - toff[i];
+    }
+    // the systems go to 4 streams, each with its own scratch; largest first,
+    // each to the least loaded stream (by grid points plus particles)
+    constexpr int K = 4;
+    for (int k = 0; k < K; k++) {
+        if (!c->sub[k] && (st = pb_create(&c->sub[k], c->device))) return pb_fail(c, st, "sub-context");
+        if (!c->sub_stream[k] &&
+            (st = pb_hip_check(c, hipStreamCreateWithFlags(&c->sub_stream[k], hipStreamNonBlocking), "stream")))
             return st;
     }
-    if ((st = pb_hip_check(c, hipMemcpyAsync(c->batch_kp.ptr, kps.data(), sizeof(pb_kparams) * nsys,
-                                             hipMemcpyHostToDevice, s), "kparams upload")))
-        return st;
-    if ((st = pb_hip_check(c, hipStreamSynchronize(s), "batch upload"))) return st;
-    // largest systems first, so the scratch grows at most a few times
+    for (int k = 0; k < K + 3; k++)
+        if (!c->sub_event[k] &&
+            (st = pb_hip_check(c, hipEventCreateWithFlags(&c->sub_event[k], hipEventDisableTiming), "event")))
+            return st;
+    auto work = [&](int i) {
+        return (double)params[i].grid[0] * params[i].grid[1] * params[i].grid[2] + 4.0 * (src_off[i + 1] - src_off[i]) +
+               4.0 * (tgt_off[i + 1] - tgt_off[i]);
+    };
     std::vector<int> order(nsys);
     for (int i = 0; i < nsys; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        const double wa = (double)params[a].grid[0] * params[a].grid[1] * params[a].grid[2] + src_off[a + 1] - src_off[a];
-        const double wb = (double)params[b].grid[0] * params[b].grid[1] * params[b].grid[2] + src_off[b + 1] - src_off[b];
-        return wa > wb;
-    });
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+    double load[K] = {0, 0, 0, 0};
+    std::vector<int> lane_of(nsys);
+    for (int i : order) {
+        int k = 0;
+        for (int q = 1; q < K; q++)
+            if (load[q] < load[k]) k = q;
+        lane_of[i] = k;
+        load[k] += work(i);
+    }
+    // Everything below runs on an internal stream g ordered after s (events),
+    // fanned out to the four system streams and joined back.  A call that
+    // repeats the previous one (same systems, parameters, arrays and stream)
+    // replays the graph captured from it: one launch instead of some twenty
+    // per system.  The first of two identical calls runs eagerly (it sizes
+    // every buffer); the second captures, and both leave s ordered after the
+    // work.
+    if (!c->cap_stream &&
+        (st = pb_hip_check(c, hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking), "stream")))
+        return st;
+    hipStream_t g = c->cap_stream;
+    std::vector<double> gkey = key;
+    for (int i = 0; i <= nsys; i++) {
+        gkey.push_back(src_off[i]);
+        gkey.push_back(tgt_off[i]);
+    }
+    const double ptrs[5] = {(double)(uintptr_t)xn, (double)(uintptr_t)f, (double)(uintptr_t)y,
+                            (double)(uintptr_t)u, (double)(uintptr_t)s};
+    gkey.insert(gkey.end(), ptrs, ptrs + 5);
+    if ((st = pb_hip_check(c, hipEventRecord(c->sub_event[K], s), "order"))) return st;
+    if ((st = pb_hip_check(c, hipStreamWaitEvent(g, c->sub_event[K], 0), "order"))) return st;
+    auto finish = [&]() -> int {
+        int q;
+        if ((q = pb_hip_check(c, hipEventRecord(c->sub_event[K + 2], g), "order"))) return q;
+        return pb_hip_check(c, hipStreamWaitEvent(s, c->sub_event[K + 2], 0), "order");
+    };
+    if (c->graph_exec && gkey == c->graph_key) {
+        if ((st = pb_hip_check(c, hipGraphLaunch(c->graph_exec, g), "graph launch"))) return st;
+        return finish();
+    }
+    const bool capture = gkey == c->last_batch_key;
+    c->last_batch_key = gkey;
+    if (c->graph_exec) {
+        (void)hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+        c->graph_key.clear();
+    }
+    if (capture && (st = pb_hip_check(c, hipStreamBeginCapture(g, hipStreamCaptureModeThreadLocal), "capture")))
+        return st;
+    // fork
+    if ((st = pb_hip_check(c, hipEventRecord(c->sub_event[K + 1], g), "fork"))) return st;
+    for (int k = 0; k < K; k++)
+        if ((st = pb_hip_check(c, hipStreamWaitEvent(c->sub_stream[k], c->sub_event[K + 1], 0), "fork")))
+            return st;
     const pb_kparams *kpd = (const pb_kparams *)c->batch_kp.ptr;
+    int est = PB_OK;
     for (int i : order) {
         const int ns = src_off[i + 1] - src_off[i], nt = tgt_off[i + 1] - tgt_off[i];
-        if ((st = enqueue_velocity(c, s, &params[i], kpd + i, toff[i + 1] - toff[i], ns,
-                                   xn + 3 * (size_t)src_off[i], f + 3 * (size_t)src_off[i], nt,
-                                   y + 3 * (size_t)tgt_off[i], u + 3 * (size_t)tgt_off[i], true, true)))
-            return st;
+        pb_context *sc = c->sub[lane_of[i]];
+        if ((est = enqueue_velocity(sc, c->sub_stream[lane_of[i]], &params[i], kpd + i, c->batch_ntab[i], ns,
+                                    xn + 3 * (size_t)src_off[i], f + 3 * (size_t)src_off[i], nt,
+                                    y + 3 * (size_t)tgt_off[i], u + 3 * (size_t)tgt_off[i], true, true))) {
+            est = pb_fail(c, est, sc->err);
+            break;
+        }
     }
-    return PB_OK;
+    // join (also after a failure, so a capture can be closed)
+    for (int k = 0; k < K; k++) {
+        if ((st = pb_hip_check(c, hipEventRecord(c->sub_event[k], c->sub_stream[k]), "join"))) return st;
+        if ((st = pb_hip_check(c, hipStreamWaitEvent(g, c->sub_event[k], 0), "join"))) return st;
+    }
+    if (capture) {
+        hipGraph_t graph = nullptr;
+        st = pb_hip_check(c, hipStreamEndCapture(g, &graph), "end capture");
+        if (st || est) {
+            if (graph) (void)hipGraphDestroy(graph);
+            c->last_batch_key.clear();
+            return est ? est : st;
+        }
+        st = pb_hip_check(c, hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0), "instantiate");
+        (void)hipGraphDestroy(graph);
+        if (st) {
+            c->graph_exec = nullptr;
+            return st;
+        }
+        c->graph_key = gkey;
+        if ((st = pb_hip_check(c, hipGraphLaunch(c->graph_exec, g), "graph launch"))) return st;
+    } else if (est) {
+        return est;
+    }
+    return finish();
 }
 
 int pb_last_pair_stats(pb_context *c, long long out[4])

@@ -142,6 +142,14 @@ __global__ __launch_bounds__(256) void k_scan_apply(int n, const int *__restrict
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = sums[nb];
 }
 
+// Plain fill (used instead of hipMemsetAsync so the binning can be captured
+// into a graph).
+__global__ void k_fill_int(int n, int *__restrict__ a, int v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
 __global__ void k_quarter_tiles(int nq, const int *__restrict__ tstart, int *__restrict__ qcount)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -217,8 +225,7 @@ int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_
     } else {
         if ((st = pb_reserve(c, &c->tperm, sizeof(int) * nn))) return st;
     }
-    if ((st = pb_hip_check(c, hipMemsetAsync(count.ptr, 0, sizeof(int) * (nkey + 1), s), "memset")))
-        return st;
+    hipLaunchKernelGGL(k_fill_int, dim3((nkey + 1 + 255) / 256), dim3(256), 0, s, nkey + 1, (int *)count.ptr, 0);
     if (n > 0) {
         int blocks = (n + 255) / 256;
         hipLaunchKernelGGL(k_bin_count, dim3(blocks), dim3(256), 0, s, n, pos, box, cells,

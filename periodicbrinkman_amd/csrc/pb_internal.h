@@ -132,11 +132,23 @@ struct pb_context {
     pb_buffer qcount, qoff, tiles; // target tiles: int2 (first, count); count at qoff[nkey_t/16]
     int max_tiles = 0;             // bound on the tile count (launch size)
     // k space
-    pb_buffer grid, spec, axis_fac; // real grid (T4 per point), half spectrum
+    pb_buffer grid, spec; // real grid (T4 per point), half spectrum
     pb_buffer twid[2];              // FFT twiddles for n = 4 .. 512: [0] FP64, [1] FP32
     int twid_ok[2] = {0, 0};
-    // batched systems: kernel constants and radial tables of every system
+    // batched systems: kernel constants and radial tables of every system,
+    // with the parameters they were built for
     pb_buffer batch_kp, batch_tab;
+    std::vector<double> batch_key;
+    std::vector<size_t> batch_ntab;
+    // batched systems run on PB_BATCH_LANES streams of their own, each with
+    // its own scratch (a sub-context), so small systems overlap on the device
+    pb_context *sub[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipStream_t sub_stream[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t sub_event[7] = {};
+    hipStream_t cap_stream = nullptr; // origin of the batched call's fan-out and graph
+    // the batched call's captured graph and the calls it replays
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<double> graph_key, last_batch_key;
     // dense / shells
     pb_buffer dense_tmp;
     // kernel constants in device memory (kernels take a pointer: a by-value

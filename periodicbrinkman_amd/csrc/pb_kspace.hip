@@ -715,6 +715,7 @@ struct kmult {
     double c;        // (1 - e^{-a^2 d^2/4}) / a^2
     double a2;       // a^2
     double inv_ng;   // 1 / (n0 n1 n2)
+    double beta;     // d^2/4 - eta/2: e^{-beta k^2} removes the split and the two window convolutions
     int n[3];
 };
 
@@ -722,15 +723,14 @@ struct kmult {
 // transforms: pencils along i0 of the three fields for B consecutive
 // (i1, kz).  Each mode gets
 //   U = b(k) (|k|^2 F - (k (k.F) + k' (k'.F)) / 2),
-//   b = (1 + c |k|^2) fac0 fac1 fac2 / (|k|^2 (|k|^2 + a^2) n0 n1 n2),
+//   b = (1 + c |k|^2) e^{-beta |k|^2} / (|k|^2 (|k|^2 + a^2) n0 n1 n2),
 // with Nyquist frequencies taken as -n/2 and k' = k with its Nyquist
 // components negated: averaging over their sign keeps the spectrum Hermitian,
 // which equals the real part of the full complex transform (the
 // reference's convention).  k = 0 is set to zero.
 template <typename T, int N>
 __global__ __launch_bounds__(256) void k_fft_x_mult(typename cplx<T>::type *__restrict__ S, int n1, int h2,
-                                                    kmult km, const double *__restrict__ fac,
-                                                    const typename cplx<T>::type *__restrict__ gtw)
+                                                    kmult km, const typename cplx<T>::type *__restrict__ gtw)
 {
     using C = typename cplx<T>::type;
     constexpr int NP = N + 1;
@@ -773,8 +773,7 @@ __global__ __launch_bounds__(256) void k_fft_x_mult(typename cplx<T>::type *__re
             *F2 = z;
             continue;
         }
-        const double b = (1.0 + km.c * kn) * fac[i0] * fac[N + i1] * fac[N + n1 + kz] * km.inv_ng /
-                         (kn * (kn + km.a2));
+        const double b = (1.0 + km.c * kn) * exp(-km.beta * kn) * km.inv_ng / (kn * (kn + km.a2));
         const T tx = (T)kx, ty = (T)ky, tz = (T)kz_;
         const T sx = (2 * i0 == N) ? -tx : tx, sy = (2 * i1 == n1) ? -ty : ty, sz = (2 * kz == n2) ? -tz : tz;
         const C v0 = *F0, v1 = *F1, v2 = *F2;
@@ -876,7 +875,7 @@ __global__ __launch_bounds__(256) void k_fft_z_c2r(const typename cplx<T>::type 
 
 template <typename T, int N>
 void launch_axis(hipStream_t s, int pass, const pb_kgrid &g, void *grid4, void *spec, const kmult &km,
-                 const double *fac, const void *tw)
+                 const void *tw)
 {
     using C = typename cplx<T>::type;
     using C4 = typename cplx<T>::type4;
@@ -901,7 +900,7 @@ void launch_axis(hipStream_t s, int pass, const pb_kgrid &g, void *grid4, void *
         constexpr int B = kbudget<T>::E / (3 * N) > 0 ? kbudget<T>::E / (3 * N) : 1;
         const long long npen = (long long)g.n[1] * h2;
         hipLaunchKernelGGL((k_fft_x_mult<T, N>), dim3((unsigned)((npen + B - 1) / B)), dim3(256), 0, s,
-                           (C *)spec, g.n[1], h2, km, fac, t);
+                           (C *)spec, g.n[1], h2, km, t);
     }
 }
 
@@ -909,19 +908,19 @@ void launch_axis(hipStream_t s, int pass, const pb_kgrid &g, void *grid4, void *
 // 3 y inverse, 4 z inverse (c2r)
 template <typename T>
 int fft_pass(hipStream_t s, int pass, const pb_kgrid &g, void *grid4, void *spec, const kmult &km,
-             const double *fac, const void *const *tws)
+             const void *const *tws)
 {
     const int ax = (pass == 0 || pass == 4) ? 2 : (pass == 2 ? 0 : 1);
     const void *tw = tws[ax];
     switch (g.n[ax]) {
-    case 4: launch_axis<T, 4>(s, pass, g, grid4, spec, km, fac, tw); break;
-    case 8: launch_axis<T, 8>(s, pass, g, grid4, spec, km, fac, tw); break;
-    case 16: launch_axis<T, 16>(s, pass, g, grid4, spec, km, fac, tw); break;
-    case 32: launch_axis<T, 32>(s, pass, g, grid4, spec, km, fac, tw); break;
-    case 64: launch_axis<T, 64>(s, pass, g, grid4, spec, km, fac, tw); break;
-    case 128: launch_axis<T, 128>(s, pass, g, grid4, spec, km, fac, tw); break;
-    case 256: launch_axis<T, 256>(s, pass, g, grid4, spec, km, fac, tw); break;
-    case 512: launch_axis<T, 512>(s, pass, g, grid4, spec, km, fac, tw); break;
+    case 4: launch_axis<T, 4>(s, pass, g, grid4, spec, km, tw); break;
+    case 8: launch_axis<T, 8>(s, pass, g, grid4, spec, km, tw); break;
+    case 16: launch_axis<T, 16>(s, pass, g, grid4, spec, km, tw); break;
+    case 32: launch_axis<T, 32>(s, pass, g, grid4, spec, km, tw); break;
+    case 64: launch_axis<T, 64>(s, pass, g, grid4, spec, km, tw); break;
+    case 128: launch_axis<T, 128>(s, pass, g, grid4, spec, km, tw); break;
+    case 256: launch_axis<T, 256>(s, pass, g, grid4, spec, km, tw); break;
+    case 512: launch_axis<T, 512>(s, pass, g, grid4, spec, km, tw); break;
     default: return PB_ERR_UNSUPPORTED;
     }
     return PB_OK;
@@ -974,22 +973,6 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
     if ((st = pb_reserve(c, &c->grid, (size_t)g.ng * sizeof(C4)))) return st;
     if ((st = pb_reserve(c, &c->spec, 3 * (size_t)g.n[0] * g.n[1] * h2 * sizeof(C)))) return st;
     if ((st = upload_twiddles<T>(c, s))) return st;
-    // per-axis factors e^{-(d^2/4 - eta/2) k^2}, axis 2 only for kz <= n2/2
-    std::vector<double> fac((size_t)g.n[0] + g.n[1] + g.n[2]);
-    double beta = d * d / 4 - g.eta / 2;
-    size_t off = 0;
-    for (int ax = 0; ax < 3; ax++) {
-        for (int q = 0; q < g.n[ax]; q++) {
-            int f = (2 * q >= g.n[ax]) ? q - g.n[ax] : q;
-            double k = 2 * M_PI * f / box.L[ax];
-            fac[off + q] = std::exp(-beta * k * k);
-        }
-        off += g.n[ax];
-    }
-    if ((st = pb_reserve(c, &c->axis_fac, fac.size() * sizeof(double)))) return st;
-    if ((st = pb_hip_check(c, hipMemcpyAsync(c->axis_fac.ptr, fac.data(), fac.size() * sizeof(double),
-                                             hipMemcpyHostToDevice, s), "factor upload")))
-        return st;
     kmult km;
     for (int ax = 0; ax < 3; ax++) {
         km.k[ax] = 2 * M_PI / box.L[ax];
@@ -998,6 +981,7 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
     km.c = -std::expm1(-a * a * d * d / 4) / (a * a);
     km.a2 = a * a;
     km.inv_ng = 1.0 / (double)g.ng;
+    km.beta = d * d / 4 - g.eta / 2;
     void *grid4 = c->grid.ptr, *spec = c->spec.ptr;
     pb_mark(c, s, 1);
     {
@@ -1008,12 +992,11 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
     }
     const C *tw = (const C *)c->twid[sizeof(T) == 8 ? 0 : 1].ptr;
     const void *tws[3] = {tw + (g.n[0] - 4), tw + (g.n[1] - 4), tw + (g.n[2] - 4)};
-    const double *fac_d = (const double *)c->axis_fac.ptr;
     pb_mark(c, s, 2);
     for (int pass = 0; pass < 5; pass++) {
         if (pass == 2) pb_mark(c, s, 3);
         if (pass == 3) pb_mark(c, s, 4);
-        if ((st = fft_pass<T>(s, pass, g, grid4, spec, km, fac_d, tws))) return pb_fail(c, st, "FFT size");
+        if ((st = fft_pass<T>(s, pass, g, grid4, spec, km, tws))) return pb_fail(c, st, "FFT size");
     }
     pb_mark(c, s, 5);
     if (nt > 0) launch_interp<T>(s, nt, g, c->tpos.ptr, c->tperm.ptr, (const C4 *)grid4, u, accumulate ? 1 : 0);

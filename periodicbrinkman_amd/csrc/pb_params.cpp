@@ -152,6 +152,17 @@ double pb_build_table(pb_kparams *k, double z0, double z1, std::vector<double> &
 {
     const long double a = k->pd.a, d = k->pd.d;
     const int deg = PB_TAB_DEG;
+    // Chebyshev nodes and the cosine matrix of the coefficient transform
+    struct cheb_consts {
+        long double node[PB_TAB_DEG + 1], cosm[PB_TAB_DEG + 1][PB_TAB_DEG + 1];
+        cheb_consts()
+        {
+            for (int j = 0; j <= PB_TAB_DEG; j++) node[j] = cosl(kPi * (j + 0.5L) / (PB_TAB_DEG + 1));
+            for (int q = 0; q <= PB_TAB_DEG; q++)
+                for (int j = 0; j <= PB_TAB_DEG; j++) cosm[q][j] = cosl(kPi * q * (j + 0.5L) / (PB_TAB_DEG + 1));
+        }
+    };
+    static const cheb_consts CC;
     double err = 1;
     for (int per_unit = 8; per_unit <= 256; per_unit *= 2) {
         int n = (int)std::ceil((z1 - z0) * per_unit);
@@ -161,17 +172,13 @@ double pb_build_table(pb_kparams *k, double z0, double z1, std::vector<double> &
         for (int i = 0; i < n; i++) {
             long double zl = z0 + (long double)i / per_unit, hw = 0.5L / per_unit, zc = zl + hw;
             long double f1[PB_TAB_MAXDEG + 1], f2[PB_TAB_MAXDEG + 1];
-            for (int j = 0; j <= deg; j++) {
-                long double t = cosl(kPi * (j + 0.5L) / (deg + 1));
-                diffH_ld((zc + hw * t) * d, a, d, f1[j], f2[j]);
-            }
+            for (int j = 0; j <= deg; j++) diffH_ld((zc + hw * CC.node[j]) * d, a, d, f1[j], f2[j]);
             long double c1[PB_TAB_MAXDEG + 1], c2[PB_TAB_MAXDEG + 1], m1[PB_TAB_MAXDEG + 1], m2[PB_TAB_MAXDEG + 1];
             for (int q = 0; q <= deg; q++) {
                 long double s1 = 0, s2 = 0;
                 for (int j = 0; j <= deg; j++) {
-                    long double cj = cosl(kPi * q * (j + 0.5L) / (deg + 1));
-                    s1 += f1[j] * cj;
-                    s2 += f2[j] * cj;
+                    s1 += f1[j] * CC.cosm[q][j];
+                    s2 += f2[j] * CC.cosm[q][j];
                 }
                 c1[q] = s1 * (q == 0 ? 1.0L : 2.0L) / (deg + 1);
                 c2[q] = s2 * (q == 0 ? 1.0L : 2.0L) / (deg + 1);

@@ -418,6 +418,12 @@ __device__ __forceinline__ void pair_body(
     if (lane == 0) stats[item] = make_uint4(n64, n32, ntest, nread);
 }
 
+__global__ void k_zero(long long n, double *__restrict__ a)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.0;
+}
+
 #define PB_PAIR_PARAMS                                                                                 \
     const int2 *__restrict__ tiles, const int *__restrict__ ntiles_ptr, const int *__restrict__ cstart,  \
         const double4 *__restrict__ spos, const double4 *__restrict__ sfrc,                              \
@@ -452,9 +458,10 @@ int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kp
     int st;
     c->stat_items = 0;
     if (nt <= 0) return PB_OK;
-    if (!accumulate &&
-        (st = pb_hip_check(c, hipMemsetAsync(u, 0, 3 * (size_t)nt * sizeof(double), s), "memset u")))
-        return st;
+    if (!accumulate) {
+        const long long n3 = 3LL * nt;
+        hipLaunchKernelGGL(k_zero, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, n3, u);
+    }
     const int max_tiles = c->max_tiles; // tiles built with the target binning
     const int *ntiles = (const int *)c->qoff.ptr + cells.nkey_t / 16;
     // wavefronts per tile: about two rounds of resident wavefronts (16 per CU)

@@ -60,3 +60,32 @@ def test_batched_empty_and_invalid():
         ctx.velocity_batched([pb.Params(alpha=1.0)], z, z, [1, 0], z, [0, 0])
     assert ei.value.status == pb.PB_ERR_INVALID
     ctx.close()
+
+
+def test_batched_graph_replays_follow_the_inputs():
+    # the second identical call captures a graph, later ones replay it; the
+    # replays read the current forces (scaled in place between calls), with
+    # eager device work and synchronizations in between
+    import torch
+    sizes = [(400, 300), (900, 800), (150, 150)]
+    params, xs, fs, ys = [], [], [], []
+    for i, (ns, nt) in enumerate(sizes):
+        L = (1.0 + 0.2 * i,) * 3
+        x, f, y = pb.uniform_system(pb.config_seed(300 + i), ns, nt, L)
+        params.append(pb.Params(L=L, alpha=5.0 + 3 * i, eps=1e-3, precision="fp64", tol=1e-8))
+        xs.append(x), fs.append(f), ys.append(y)
+    so = np.concatenate([[0], np.cumsum([s[0] for s in sizes])])
+    to = np.concatenate([[0], np.cumsum([s[1] for s in sizes])])
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+    X, F, Y = dev(np.concatenate(xs)), dev(np.concatenate(fs)), dev(np.concatenate(ys))
+    U = torch.empty((Y.shape[0], 3), dtype=torch.float64, device="cuda")
+    ctx = pb.Context(0)
+    ref = ctx.velocity_batched(params, X, F, so, Y, to, U).cpu().numpy()  # eager
+    for k, scale in enumerate([1.0, 2.0, -0.5, 1.0]):
+        F.copy_(dev(np.concatenate(fs) * scale))
+        junk = torch.randn(1 << 16, device="cuda").sum()  # eager work between replays
+        torch.cuda.synchronize()
+        got = ctx.velocity_batched(params, X, F, so, Y, to, U).cpu().numpy()
+        assert relL2(got, scale * ref) < 1e-12, (k, scale)
+    del junk
+    ctx.close()
