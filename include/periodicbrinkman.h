@@ -188,6 +188,27 @@ int pb_velocity_batched(pb_context *ctx, int nsys, pb_params *params, const int 
 int pb_velocity_multirhs(pb_context *ctx, const pb_params *p, int ns, const double *xn, int R,
                          const float *f, int nt, const double *y, double *u, void *stream);
 
+/* ---- the reference driver (periodicBrinkman.cu:32-219) -------------------- */
+
+/* Parameters of the reference driver's parameter file: one "key: value" pair
+ * per line, split at ':' and ' ' (periodicBrinkman.cu:44-70); keys Lx, Ly,
+ * Lz, d, e, M, alpha, boxSize, toPrint.  Keys the file does not set keep the
+ * reference defaults (periodicBrinkman.cu:79-87): 1, 1, 1, 0.6, 0, 4, 1, 1,
+ * 0.  Other keys and lines without a value are ignored. */
+typedef struct pb_driver_params {
+    double Lx, Ly, Lz, d, e, alpha;
+    int M, boxSize, toPrint;
+} pb_driver_params;
+void pb_driver_defaults(pb_driver_params *p);
+int pb_read_param_file(const char *path, pb_driver_params *p); /* PB_ERR_INVALID: cannot open */
+
+/* The driver's split sweep (periodicBrinkman.cu:141-203): the 3 nt x 3
+ * Ewald matrix of the nt targets x (host, 3 x nt) for one source at
+ * (0.5, 0.5, 0.5), by realShells (home cell, then layers to M) plus
+ * fourierShells (to M), at d0 = p->d and at dval[i] = (0.1 + 0.95 i) d0,
+ * i < 8; err[i] = max |A(dval[i]) - A(d0)|.  Evaluated on the GPU. */
+int pb_split_sweep(const pb_driver_params *p, const double *x, int nt, double dval[8], double err[8]);
+
 /* Statistics of the last pb_velocity call on ctx (device work counters read
  * back synchronously): pairs evaluated in FP64 and FP32. */
 int pb_last_pair_counts(pb_context *ctx, long long *n_fp64, long long *n_fp32);

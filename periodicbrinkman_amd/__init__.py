@@ -45,6 +45,12 @@ class _Params(ctypes.Structure):
                 ("tol", ctypes.c_double), ("kernel", ctypes.c_int)]
 
 
+class _DriverParams(ctypes.Structure):
+    _fields_ = [("Lx", ctypes.c_double), ("Ly", ctypes.c_double), ("Lz", ctypes.c_double),
+                ("d", ctypes.c_double), ("e", ctypes.c_double), ("alpha", ctypes.c_double),
+                ("M", ctypes.c_int), ("boxSize", ctypes.c_int), ("toPrint", ctypes.c_int)]
+
+
 class _TuneReport(ctypes.Structure):
     _fields_ = [("err_real", ctypes.c_double), ("err_kspace", ctypes.c_double),
                 ("err_window", ctypes.c_double), ("pairs", ctypes.c_double),
@@ -85,6 +91,10 @@ def _declare(L):
     L.pb_params_default.argtypes = [ctypes.POINTER(_Params)]
     L.pb_params_default.restype = None
     L.pb_tune.argtypes = [ctypes.POINTER(_Params), ii, ii]
+    L.pb_driver_defaults.argtypes = [ctypes.POINTER(_DriverParams)]
+    L.pb_driver_defaults.restype = None
+    L.pb_read_param_file.argtypes = [ctypes.c_char_p, ctypes.POINTER(_DriverParams)]
+    L.pb_split_sweep.argtypes = [ctypes.POINTER(_DriverParams), _D, ii, _D, _D]
     L.pb_autotune.argtypes = [ctypes.POINTER(_Params), ii, ii, ctypes.POINTER(_TuneReport)]
     L.pb_create.argtypes = [ctypes.POINTER(_VP), ii]
     L.pb_destroy.argtypes = [_VP]
@@ -440,3 +450,30 @@ def clustered_system(seed, n, nclusters=64, sigma=0.02, L=(1.0, 1.0, 1.0)):
     x = np.mod(x, L)
     f = r.normal(3 * n).reshape(n, 3)
     return x, f
+
+
+# ---- the reference driver ---------------------------------------------------------
+def read_param_file(path):
+    """Parameters of the reference driver's parameter file (pb_read_param_file):
+    a dict with keys Lx, Ly, Lz, d, e, alpha, M, boxSize, toPrint; raises
+    PBError if the file cannot be opened."""
+    p = _DriverParams()
+    _check(lib().pb_read_param_file(os.fsencode(path), ctypes.byref(p)))
+    return {k: getattr(p, k) for k, _ in _DriverParams._fields_}
+
+
+def driver_defaults():
+    p = _DriverParams()
+    lib().pb_driver_defaults(ctypes.byref(p))
+    return {k: getattr(p, k) for k, _ in _DriverParams._fields_}
+
+
+def split_sweep(params, x):
+    """The driver's split sweep (pb_split_sweep) for targets x (nt, 3): returns
+    (dval[8], err[8])."""
+    p = _DriverParams(**{k: params[k] for k, _ in _DriverParams._fields_})
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    dval, err = np.zeros(8), np.zeros(8)
+    _check(lib().pb_split_sweep(ctypes.byref(p), _dp(x), int(x.shape[0]), _dp(dval), _dp(err)))
+    return dval, err
+

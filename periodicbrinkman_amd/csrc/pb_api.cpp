@@ -774,6 +774,37 @@ int fourierShells(double *A, double *absA, int lda, const double *x, int nt, con
     return shells_impl(A, absA, lda, x, nt, x0, ns, newM, oldM, L, a, d, e, true);
 }
 
+int pb_split_sweep(const pb_driver_params *p, const double *x, int nt, double dval[8], double err[8])
+{
+    if (!p || !dval || !err || nt < 1 || !x || p->M < 0 || !(p->d > 0) || !(p->alpha > 0) || !(p->e >= 0))
+        return PB_ERR_INVALID;
+    const double L[3] = {p->Lx, p->Ly, p->Lz};
+    const double x0[3] = {0.5, 0.5, 0.5}; // the driver's single source (periodicBrinkman.cu:114)
+    const int lda = 3 * nt;
+    const size_t n = (size_t)lda * 3;
+    const int zero[3] = {0, 0, 0}, Ms[3] = {p->M, p->M, p->M};
+    // A = realShells(home) + realShells(layers 1..M) + fourierShells(1..M)
+    auto matrix = [&](double d, std::vector<double> &A) -> int {
+        std::vector<double> absA(n, 0.0);
+        A.assign(n, 0.0);
+        int st;
+        if ((st = realShells(A.data(), absA.data(), lda, x, nt, x0, 1, zero, zero, L, p->alpha, d, p->e))) return st;
+        if ((st = realShells(A.data(), absA.data(), lda, x, nt, x0, 1, Ms, zero, L, p->alpha, d, p->e))) return st;
+        return fourierShells(A.data(), absA.data(), lda, x, nt, x0, 1, Ms, zero, L, p->alpha, d, p->e);
+    };
+    std::vector<double> A0, A;
+    int st;
+    if ((st = matrix(p->d, A0))) return st;
+    for (int i = 0; i < 8; i++) {
+        dval[i] = (0.1 + i * 0.95) * p->d;
+        if ((st = matrix(dval[i], A))) return st;
+        double m = 0;
+        for (size_t q = 0; q < n; q++) m = std::fmax(m, std::fabs(A[q] - A0[q]));
+        err[i] = m;
+    }
+    return PB_OK;
+}
+
 int pb_vR_batch(double *A, const double *xh, int n, double a, double d, double e, void *stream)
 {
     if (n < 0 || (n > 0 && (!A || !xh)) || !(a > 0) || !(d > 0) || !(e >= 0)) return PB_ERR_INVALID;

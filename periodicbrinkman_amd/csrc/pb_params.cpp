@@ -1,6 +1,8 @@
 // pb_params.cpp -- host-side constants for the device kernels, the parameter
 // tuner and the deterministic input generator.
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "pb_internal.h"
@@ -391,6 +393,49 @@ int pb_autotune(pb_params *p, int ns, int nt, pb_tune_report *rep)
     br.candidates = scored;
     *p = bestp;
     if (rep) *rep = br;
+    return PB_OK;
+}
+
+// ---- the reference driver's parameter file ----------------------------------
+void pb_driver_defaults(pb_driver_params *p)
+{
+    p->Lx = p->Ly = p->Lz = 1.0;
+    p->d = 0.6;
+    p->e = 0.0;
+    p->alpha = 1.0;
+    p->M = 4;
+    p->boxSize = 1;
+    p->toPrint = 0;
+}
+
+int pb_read_param_file(const char *path, pb_driver_params *p)
+{
+    if (!path || !p) return PB_ERR_INVALID;
+    pb_driver_defaults(p);
+    FILE *fh = std::fopen(path, "r");
+    if (!fh) return PB_ERR_INVALID;
+    char line[1024];
+    while (std::fgets(line, sizeof line, fh)) {
+        // tokens separated by ':' , ' ' and line ends; the first is the key,
+        // the second the value
+        const char *sep = ": \t\r\n";
+        char *save = nullptr;
+        char *key = strtok_r(line, sep, &save);
+        if (!key) continue;
+        char *val = strtok_r(nullptr, sep, &save);
+        if (!val) continue;
+        const double v = std::atof(val);
+        if (!std::strcmp(key, "Lx")) p->Lx = v;
+        else if (!std::strcmp(key, "Ly")) p->Ly = v;
+        else if (!std::strcmp(key, "Lz")) p->Lz = v;
+        else if (!std::strcmp(key, "d")) p->d = v;
+        else if (!std::strcmp(key, "e")) p->e = v;
+        else if (!std::strcmp(key, "alpha")) p->alpha = v;
+        else if (!std::strcmp(key, "M")) p->M = (int)v;
+        else if (!std::strcmp(key, "boxSize")) p->boxSize = (int)v;
+        else if (!std::strcmp(key, "toPrint")) p->toPrint = (int)v;
+    }
+    std::fclose(fh);
     return PB_OK;
 }
 

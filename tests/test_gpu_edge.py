@@ -1,0 +1,68 @@
+"""Edge regimes of the Ewald evaluator against the FP64 oracle: strongly
+anisotropic boxes, small alpha L (the 1/alpha^2 cancellation of the closed
+forms grows), the singular Brinkmanlet (eps = 0) with targets on the
+sources, and clustered sources (cell-list load imbalance)."""
+import numpy as np
+import pytest
+
+import periodicbrinkman_amd as pb
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def relL2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def dev(a):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device="cuda")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = pb.Context(0)
+    yield c
+    c.close()
+
+
+def _check(ctx, x, f, y, p, tol_fp64, tol_mixed=1e-6):
+    want = po.velocity_ewald(y, x, f, p.L, p.alpha, p.d, p.eps, p.rc, p.grid, (p.P,) * 3, p.m)
+    got = ctx.velocity(p, dev(x), dev(f), dev(y)).cpu().numpy()
+    assert relL2(got, want) < tol_fp64, relL2(got, want)
+    gm = ctx.velocity(p.copy(precision="mixed"), dev(x), dev(f), dev(y)).cpu().numpy()
+    assert relL2(gm, want) < tol_mixed, relL2(gm, want)
+
+
+def test_strongly_anisotropic_box(ctx):
+    L = (1.0, 2.5, 0.4)
+    x, f, y = pb.uniform_system(pb.config_seed(40), 2500, 1500, L)
+    p = pb.Params(L=L, alpha=8.0, eps=2e-3, grid=(32, 64, 16), P=12, tol=1e-8).tune(2500, 1500)
+    _check(ctx, x, f, y, p, 1e-11)
+
+
+@pytest.mark.parametrize("alpha,tol", [(0.05, 2e-9), (0.5, 1e-10)])
+def test_small_alpha_L(ctx, alpha, tol):
+    # at alpha d ~ 5e-3 the oracle's FP64 closed forms lose ~(alpha d)^-2
+    # rounding units to cancellation (its golden-vector tolerance there is
+    # 1e-9); the device evaluates that range from a long-double table
+    L = (1.0, 1.0, 1.0)
+    x, f, y = pb.uniform_system(pb.config_seed(41), 2000, 1500, L)
+    p = pb.Params(L=L, alpha=alpha, eps=1e-3, grid=(32, 32, 32), P=12, tol=1e-8).tune(2000, 1500)
+    _check(ctx, x, f, y, p, tol)
+
+
+def test_singular_brinkmanlet_on_sources(ctx):
+    L = (1.0, 1.0, 1.0)
+    x, f, _ = pb.uniform_system(pb.config_seed(42), 2000, 0, L)
+    p = pb.Params(L=L, alpha=10.0, eps=0.0, grid=(32, 32, 32), P=12, tol=1e-8).tune(2000, 2000)
+    _check(ctx, x, f, x.copy(), p, 1e-11)
+
+
+def test_clustered_sources(ctx):
+    # sources in 8 Gaussian clusters (sigma 0.02, wrapped), targets on the sources
+    L = (1.0, 1.0, 1.0)
+    x, f = pb.clustered_system(pb.config_seed(43), 4000, 8, 0.02, L)
+    p = pb.Params(L=L, alpha=25.0, eps=5e-4, grid=(64, 64, 64), P=12, tol=1e-8).tune(4000, 4000)
+    _check(ctx, x, f, x.copy(), p, 1e-11)
