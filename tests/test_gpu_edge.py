@@ -66,3 +66,25 @@ def test_clustered_sources(ctx):
     x, f = pb.clustered_system(pb.config_seed(43), 4000, 8, 0.02, L)
     p = pb.Params(L=L, alpha=25.0, eps=5e-4, grid=(64, 64, 64), P=12, tol=1e-8).tune(4000, 4000)
     _check(ctx, x, f, x.copy(), p, 1e-11)
+
+
+def test_clustered_full_size_split_invariance(ctx):
+    # 1e6 sources in 64 Gaussian clusters (sigma 0.02), targets on the sources,
+    # alpha 25, eps 5e-4, 256^3 grid, P 16: too large for the oracle, so two
+    # different splits d must give the same field (the reference's own check,
+    # periodicBrinkman.cu:184-203); mixed precision meets 1e-6 against FP64
+    import torch
+    n = 1_000_000
+    x, f = pb.clustered_system(pb.config_seed(2), n, 64, 0.02)
+    X, F = dev(x), dev(f)
+    us, ds = [], []
+    for tol in (1e-8, 1e-10):
+        p = pb.Params(alpha=25.0, eps=5e-4, grid=(256, 256, 256), P=16, tol=tol).tune(n, n)
+        ds.append(p.d)
+        us.append(ctx.velocity(p, X, F, X).cpu().numpy())
+    torch.cuda.synchronize()
+    assert ds[1] > 1.05 * ds[0]
+    assert relL2(us[0], us[1]) < 1e-7
+    pm = pb.Params(alpha=25.0, eps=5e-4, grid=(256, 256, 256), P=16, tol=1e-10, precision="mixed").tune(n, n)
+    um = ctx.velocity(pm, X, F, X).cpu().numpy()
+    assert relL2(um, us[1]) < 1e-6
