@@ -249,8 +249,7 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
     pb_fill_box(&box, p->L);
     pb_cells cells;
     if ((st = make_cells(c, p, &cells))) return st;
-    if ((st = pb_bin_points(c, s, cells, box, ns, xn, f, true))) return st;
-    if ((st = pb_bin_points(c, s, cells, box, nt, y, nullptr, false))) return st;
+    if ((st = pb_bin_all(c, s, cells, box, ns, xn, f, nt, y))) return st;
     for (int i = 1; i < PB_NPHASE; i++) pb_mark(c, s, i);
     bool acc = false;
     if (do_k) {
@@ -480,8 +479,7 @@ int pb_velocity_multirhs(pb_context *c, const pb_params *p, int ns, const double
     pb_cells cells;
     if ((st = make_cells(c, p, &cells))) return st;
     const int RP = (R + 31) / 32 * 32;
-    if ((st = pb_bin_points(c, s, cells, box, ns, xn, nullptr, true))) return st;
-    if ((st = pb_bin_points(c, s, cells, box, nt, y, nullptr, false))) return st;
+    if ((st = pb_bin_all(c, s, cells, box, ns, xn, nullptr, nt, y))) return st;
     if ((st = pb_sort_forces(c, s, ns, R, RP, f))) return st;
     if (nt == 0) return PB_OK;
     pb_kgrid g;

@@ -168,6 +168,186 @@ __global__ void k_write_tiles(int nq, const int *__restrict__ tstart, const int 
     for (int i = 0; 64 * i < cnt; i++) tiles[o + i] = make_int2(first + 64 * i, min(64, cnt - 64 * i));
 }
 
+// ---- fused binning of sources and targets (small key spaces) -----------------
+// Four launches instead of seventeen: one fill, one count and one scatter
+// over both point sets, and one workgroup that scans both count arrays and
+// writes the target tiles.  Used when ncell + nkey_t <= kFusedKeys.
+constexpr int kFusedKeys = 1 << 18;
+
+__global__ void k_fill_int2(int n0, int *__restrict__ a0, int n1, int *__restrict__ a1)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n0)
+        a0[i] = 0;
+    else if (i - n0 < n1)
+        a1[i - n0] = 0;
+}
+
+__device__ __forceinline__ int cell_key(const double *__restrict__ pos, size_t i, const pb_box &box,
+                                        const pb_cells &cells, bool brick_order)
+{
+    int ci[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double w = wrap_coord(pos[3 * i + c], box.L[c]);
+        const int k = (int)(w * cells.inv_h[c]);
+        ci[c] = k >= cells.nc[c] ? cells.nc[c] - 1 : k;
+    }
+    if (brick_order) {
+        const int brick = ((ci[0] >> 2) * cells.nb[1] + (ci[1] >> 2)) * cells.nb[2] + (ci[2] >> 2);
+        return brick * 64 + morton3_2bit(ci[0], ci[1], ci[2]);
+    }
+    return (ci[0] * cells.nc[1] + ci[1]) * cells.nc[2] + ci[2];
+}
+
+// threads [0, ns): sources (z-major cell keys); [ns, ns + nt): targets
+// (brick-Morton keys)
+__global__ void k_bin_count2(int ns, const double *__restrict__ xs, int nt, const double *__restrict__ yt,
+                             pb_box box, pb_cells cells, int *__restrict__ scount, int *__restrict__ skey,
+                             int *__restrict__ srank, int *__restrict__ tcount, int *__restrict__ tkey,
+                             int *__restrict__ trank)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ns) {
+        const int key = cell_key(xs, (size_t)i, box, cells, false);
+        skey[i] = key;
+        srank[i] = atomicAdd(&scount[key], 1);
+    } else if (i - ns < nt) {
+        const int j = i - ns;
+        const int key = cell_key(yt, (size_t)j, box, cells, true);
+        tkey[j] = key;
+        trank[j] = atomicAdd(&tcount[key], 1);
+    }
+}
+
+__global__ void k_bin_scatter2(int ns, const double *__restrict__ xs, const double *__restrict__ fs, int nt,
+                               const double *__restrict__ yt, pb_box box, const int *__restrict__ sstart,
+                               const int *__restrict__ skey, const int *__restrict__ srank,
+                               double4 *__restrict__ spos, double4 *__restrict__ sfrc, int *__restrict__ sperm,
+                               const int *__restrict__ tstart, const int *__restrict__ tkey,
+                               const int *__restrict__ trank, double4 *__restrict__ tpos, int *__restrict__ tperm)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool src = i < ns;
+    const int j = src ? i : i - ns;
+    if (!src && j >= nt) return;
+    const double *pos = src ? xs : yt;
+    const int dst = (src ? sstart[skey[j]] + srank[j] : tstart[tkey[j]] + trank[j]);
+    double w[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) w[c] = wrap_coord(pos[3 * (size_t)j + c], box.L[c]);
+    if (src) {
+        spos[dst] = make_double4(w[0], w[1], w[2], 0.0);
+        if (fs) sfrc[dst] = make_double4(fs[3 * (size_t)j], fs[3 * (size_t)j + 1], fs[3 * (size_t)j + 2], 0.0);
+        sperm[dst] = j;
+    } else {
+        tpos[dst] = make_double4(w[0], w[1], w[2], 0.0);
+        tperm[dst] = j;
+    }
+}
+
+// Exclusive scan across the 1024 threads of the workgroup; total to every thread.
+__device__ __forceinline__ int scan1024(int v, int *wsum, int &total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const int sk = wsum[k];
+        pre += k < w ? sk : 0;
+        tot += sk;
+    }
+    __syncthreads(); // wsum is rewritten by the next call
+    total = tot;
+    return pre + x - v;
+}
+
+// Exclusive scan of n counts into start[0..n] (start[n] = total), 16
+// consecutive counts per thread and 16384 per round.  With tiles != null the
+// counts are target keys (n a multiple of 64): each thread's 16 keys are one
+// quarter brick, whose targets become ceil(count / 64) tiles of <= 64.
+__device__ void scan_counts(const int *__restrict__ count, int n, int *__restrict__ start, int *wsum,
+                            int2 *__restrict__ tiles, int *__restrict__ ntiles)
+{
+    int carry = 0, tcarry = 0;
+    for (int base = 0; base < n; base += 16 * 1024) {
+        const int i0 = base + 16 * (int)threadIdx.x;
+        int v[16];
+        if (i0 + 16 <= n) {
+            const int4 *c4 = reinterpret_cast<const int4 *>(count + i0);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int4 t = c4[q];
+                v[4 * q] = t.x;
+                v[4 * q + 1] = t.y;
+                v[4 * q + 2] = t.z;
+                v[4 * q + 3] = t.w;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; q++) v[q] = i0 + q < n ? count[i0 + q] : 0;
+        }
+        int sum = 0;
+#pragma unroll
+        for (int q = 0; q < 16; q++) sum += v[q];
+        int tot;
+        const int first = scan1024(sum, wsum, tot) + carry;
+        int run = first;
+        if (i0 + 16 <= n) {
+            int4 *o4 = reinterpret_cast<int4 *>(start + i0);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                int4 t;
+                t.x = run;
+                run += v[4 * q];
+                t.y = run;
+                run += v[4 * q + 1];
+                t.z = run;
+                run += v[4 * q + 2];
+                t.w = run;
+                run += v[4 * q + 3];
+                o4[q] = t;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                if (i0 + q < n) start[i0 + q] = run;
+                run += v[q];
+            }
+        }
+        carry += tot;
+        if (tiles) {
+            const int nq = (sum + 63) >> 6;
+            int ttot;
+            int o = scan1024(nq, wsum, ttot) + tcarry;
+            for (int k = 0; k < nq; k++) tiles[o + k] = make_int2(first + 64 * k, min(64, sum - 64 * k));
+            tcarry += ttot;
+        }
+    }
+    if (threadIdx.x == 0) {
+        start[n] = carry;
+        if (ntiles) *ntiles = tcarry;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_bin_scan2(const int *__restrict__ scount, int ncell,
+                                                    int *__restrict__ sstart, const int *__restrict__ tcount,
+                                                    int nkey_t, int *__restrict__ tstart,
+                                                    int2 *__restrict__ tiles, int *__restrict__ ntiles)
+{
+    __shared__ int wsum[16];
+    scan_counts(scount, ncell, sstart, wsum, nullptr, nullptr);
+    scan_counts(tcount, nkey_t, tstart, wsum, tiles, ntiles);
+}
+
 } // namespace
 
 // Exclusive scan of n ints from in to out (out[n] = total) on stream s.
@@ -241,5 +421,53 @@ int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_
                            is_source ? (int *)c->sperm.ptr : (int *)c->tperm.ptr);
     }
     if (!is_source && (st = pb_build_tiles(c, s, cells, n, &c->max_tiles))) return st;
+    return pb_hip_check(c, hipGetLastError(), "binning launch");
+}
+
+int pb_bin_all(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_box &box, int ns, const double *xn,
+               const double *f, int nt, const double *y)
+{
+    if ((long long)cells.ncell + cells.nkey_t > kFusedKeys || cells.nkey_t % 64 != 0) {
+        int st;
+        if ((st = pb_bin_points(c, s, cells, box, ns, xn, f, true))) return st;
+        return pb_bin_points(c, s, cells, box, nt, y, nullptr, false);
+    }
+    int st;
+    const size_t nsn = (size_t)(ns > 0 ? ns : 1), ntn = (size_t)(nt > 0 ? nt : 1);
+    const int ncell = cells.ncell, nkt = cells.nkey_t, nq = nkt / 16;
+    const size_t mt = (size_t)nq + (size_t)nt / 64 + 1;
+    if ((st = pb_reserve(c, &c->cell_count, sizeof(int) * (ncell + 1)))) return st;
+    if ((st = pb_reserve(c, &c->cell_start, sizeof(int) * (ncell + 1)))) return st;
+    if ((st = pb_reserve(c, &c->cell_of_s, sizeof(int) * nsn))) return st;
+    if ((st = pb_reserve(c, &c->rank_s, sizeof(int) * nsn))) return st;
+    if ((st = pb_reserve(c, &c->spos, sizeof(double4) * nsn))) return st;
+    if ((st = pb_reserve(c, &c->sfrc, sizeof(double4) * nsn))) return st;
+    if ((st = pb_reserve(c, &c->sperm, sizeof(int) * nsn))) return st;
+    if ((st = pb_reserve(c, &c->tcell_count, sizeof(int) * (nkt + 1)))) return st;
+    if ((st = pb_reserve(c, &c->tcell_start, sizeof(int) * (nkt + 1)))) return st;
+    if ((st = pb_reserve(c, &c->cell_of_t, sizeof(int) * ntn))) return st;
+    if ((st = pb_reserve(c, &c->rank_t, sizeof(int) * ntn))) return st;
+    if ((st = pb_reserve(c, &c->tpos, sizeof(double4) * ntn))) return st;
+    if ((st = pb_reserve(c, &c->tperm, sizeof(int) * ntn))) return st;
+    if ((st = pb_reserve(c, &c->qoff, sizeof(int) * (nq + 1)))) return st;
+    if ((st = pb_reserve(c, &c->tiles, sizeof(int2) * mt))) return st;
+    int *scount = (int *)c->cell_count.ptr, *tcount = (int *)c->tcell_count.ptr;
+    const int nfill = ncell + nkt;
+    hipLaunchKernelGGL(k_fill_int2, dim3((nfill + 255) / 256), dim3(256), 0, s, ncell, scount, nkt, tcount);
+    const long long npts = (long long)ns + nt;
+    if (npts > 0)
+        hipLaunchKernelGGL(k_bin_count2, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, ns, xn, nt, y, box,
+                           cells, scount, (int *)c->cell_of_s.ptr, (int *)c->rank_s.ptr, tcount,
+                           (int *)c->cell_of_t.ptr, (int *)c->rank_t.ptr);
+    hipLaunchKernelGGL(k_bin_scan2, dim3(1), dim3(1024), 0, s, (const int *)scount, ncell, (int *)c->cell_start.ptr,
+                       (const int *)tcount, nkt, (int *)c->tcell_start.ptr, (int2 *)c->tiles.ptr,
+                       (int *)c->qoff.ptr + nq);
+    if (npts > 0)
+        hipLaunchKernelGGL(k_bin_scatter2, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, ns, xn, f, nt, y,
+                           box, (const int *)c->cell_start.ptr, (const int *)c->cell_of_s.ptr,
+                           (const int *)c->rank_s.ptr, (double4 *)c->spos.ptr, f ? (double4 *)c->sfrc.ptr : nullptr,
+                           (int *)c->sperm.ptr, (const int *)c->tcell_start.ptr, (const int *)c->cell_of_t.ptr,
+                           (const int *)c->rank_t.ptr, (double4 *)c->tpos.ptr, (int *)c->tperm.ptr);
+    c->max_tiles = (int)mt;
     return pb_hip_check(c, hipGetLastError(), "binning launch");
 }
