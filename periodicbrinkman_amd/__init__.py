@@ -304,8 +304,11 @@ class Context:
     PHASES = ("binning", "spread", "fft_forward", "multiply", "fft_inverse", "gather", "pair")
 
     def set_timing(self, enable=True):
-        """Record HIP events around each phase of velocity() on its stream."""
-        _check(lib().pb_set_timing(self.handle, 1 if enable else 0), self)
+        """Record HIP events around each phase of velocity() on its stream
+        (enable="pair": only around the pair kernel; each event costs the
+        stream about 6 us)."""
+        mode = 2 if enable == "pair" else (1 if enable else 0)
+        _check(lib().pb_set_timing(self.handle, mode), self)
 
     def last_timings(self):
         """Average milliseconds per phase of the velocity() calls since set_timing()."""

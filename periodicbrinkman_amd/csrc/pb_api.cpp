@@ -45,25 +45,35 @@ int pb_reserve(pb_context *c, pb_buffer *b, size_t bytes)
     return PB_OK;
 }
 
+// Each recorded event drains the queue (about 6 us on the device timeline),
+// so only the boundaries that a call reaches are recorded, and mode 2 records
+// only the two around the pair kernel.
 void pb_mark(pb_context *c, hipStream_t s, int i)
 {
-    if (c && c->timing && c->ev_ready && i >= 0 && i <= PB_NPHASE)
-        (void)hipEventRecord(c->ev[c->cur][i], s);
+    if (!c || !c->timing || !c->ev_ready || i < 0 || i > PB_NPHASE) return;
+    if (c->timing == 2 && i < PB_NPHASE - 1) return;
+    (void)hipEventRecord(c->ev[c->cur][i], s);
+    c->recorded[c->cur] |= 1u << i;
 }
 
 namespace {
 
 // Fold a finished slot's phase durations into the running sums.
+// (A phase counts only when the call recorded both its boundaries: phases
+// that did not run have zero length.)
 int timing_collect(pb_context *c, int slot)
 {
     if (!c->slot_used[slot]) return PB_OK;
+    c->slot_used[slot] = 0;
+    const unsigned m = c->recorded[slot];
+    if (!(m & (1u << PB_NPHASE))) return PB_OK; // the call stopped early: no sample
     if (hipEventSynchronize(c->ev[slot][PB_NPHASE]) != hipSuccess) return PB_ERR_HIP;
     for (int i = 0; i < PB_NPHASE; i++) {
         float t = 0;
-        if (hipEventElapsedTime(&t, c->ev[slot][i], c->ev[slot][i + 1]) != hipSuccess) return PB_ERR_HIP;
+        if (((m >> i) & 3u) == 3u && hipEventElapsedTime(&t, c->ev[slot][i], c->ev[slot][i + 1]) != hipSuccess)
+            return PB_ERR_HIP;
         c->tsum[i] += t;
     }
-    c->slot_used[slot] = 0;
     c->nsummed++;
     return PB_OK;
 }
@@ -78,8 +88,9 @@ int timing_begin(pb_context *c, hipStream_t s)
     if (st) return st;
     c->cur = slot;
     c->slot_used[slot] = 1;
+    c->recorded[slot] = 0;
     c->ncalls++;
-    for (int i = 0; i <= PB_NPHASE; i++) pb_mark(c, s, i);
+    pb_mark(c, s, 0);
     return PB_OK;
 }
 
@@ -250,7 +261,7 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
     pb_cells cells;
     if ((st = make_cells(c, p, &cells))) return st;
     if ((st = pb_bin_all(c, s, cells, box, ns, xn, f, nt, y))) return st;
-    for (int i = 1; i < PB_NPHASE; i++) pb_mark(c, s, i);
+    pb_mark(c, s, 1);
     bool acc = false;
     if (do_k) {
         pb_kgrid g;
@@ -428,7 +439,7 @@ int pb_set_timing(pb_context *c, int enable)
     for (int k = 0; k < PB_TSLOTS; k++) c->slot_used[k] = 0;
     for (int i = 0; i < PB_NPHASE; i++) c->tsum[i] = 0;
     c->ncalls = c->nsummed = 0;
-    c->timing = enable ? 1 : 0;
+    c->timing = enable == 2 ? 2 : (enable ? 1 : 0);
     return PB_OK;
 }
 

@@ -22,11 +22,6 @@ __device__ __forceinline__ double wrap_coord(double v, double L)
     return w;
 }
 
-__device__ __forceinline__ int morton3_2bit(int x, int y, int z)
-{
-    // interleave the low 2 bits of x, y, z: x1 y1 z1 x0 y0 z0
-    return ((x & 2) << 4) | ((y & 2) << 3) | ((z & 2) << 2) | ((x & 1) << 2) | ((y & 1) << 1) | (z & 1);
-}
 
 __global__ void k_bin_count(int n, const double *__restrict__ pos, pb_box box, pb_cells cells,
                             int brick_order, int *__restrict__ count, int *__restrict__ key_of,
@@ -41,13 +36,8 @@ __global__ void k_bin_count(int n, const double *__restrict__ pos, pb_box box, p
         int k = (int)(w * cells.inv_h[c]);
         ci[c] = k >= cells.nc[c] ? cells.nc[c] - 1 : k;
     }
-    int key;
-    if (brick_order) {
-        int brick = ((ci[0] >> 2) * cells.nb[1] + (ci[1] >> 2)) * cells.nb[2] + (ci[2] >> 2);
-        key = brick * 64 + morton3_2bit(ci[0], ci[1], ci[2]);
-    } else {
-        key = (ci[0] * cells.nc[1] + ci[1]) * cells.nc[2] + ci[2];
-    }
+    const int key = brick_order ? pb_target_key(ci[0], ci[1], ci[2], cells)
+                                : (ci[0] * cells.nc[1] + ci[1]) * cells.nc[2] + ci[2];
     key_of[i] = key;
     rank[i] = atomicAdd(&count[key], 1);
 }
@@ -193,10 +183,7 @@ __device__ __forceinline__ int cell_key(const double *__restrict__ pos, size_t i
         const int k = (int)(w * cells.inv_h[c]);
         ci[c] = k >= cells.nc[c] ? cells.nc[c] - 1 : k;
     }
-    if (brick_order) {
-        const int brick = ((ci[0] >> 2) * cells.nb[1] + (ci[1] >> 2)) * cells.nb[2] + (ci[2] >> 2);
-        return brick * 64 + morton3_2bit(ci[0], ci[1], ci[2]);
-    }
+    if (brick_order) return pb_target_key(ci[0], ci[1], ci[2], cells);
     return (ci[0] * cells.nc[1] + ci[1]) * cells.nc[2] + ci[2];
 }
 

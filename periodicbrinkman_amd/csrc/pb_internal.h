@@ -167,6 +167,7 @@ struct pb_context {
     long long ncalls = 0, nsummed = 0;
     hipEvent_t ev[PB_TSLOTS][PB_NPHASE + 1] = {};
     int slot_used[PB_TSLOTS] = {};
+    unsigned recorded[PB_TSLOTS] = {}; // boundaries recorded in the slot's call (bit i)
     double tsum[PB_NPHASE] = {};
 };
 
@@ -204,6 +205,14 @@ int pb_bin_all(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_box
 // It pays for the gather, whose grid reads have no other reuse; the pair and
 // spread kernels measured faster without it (their spatially adjacent
 // workgroups then contend for the same L2 channels).
+// Sort key of a target cell (cx, cy, cz): bricks of 4 x 4 x 4 cells in z-major
+// order, Morton order (x1 y1 z1 x0 y0 z0) of the low two bits inside a brick.
+__device__ __forceinline__ int pb_target_key(int cx, int cy, int cz, const pb_cells &cells)
+{
+    const int brick = ((cx >> 2) * cells.nb[1] + (cy >> 2)) * cells.nb[2] + (cz >> 2);
+    const int m = ((cx & 2) << 4) | ((cy & 2) << 3) | ((cz & 2) << 2) | ((cx & 1) << 2) | ((cy & 1) << 1) | (cz & 1);
+    return brick * 64 + m;
+}
 __device__ __forceinline__ int pb_xcd_block(int b, int nblk8)
 {
     return (b & 7) * (nblk8 >> 3) + (b >> 3);

@@ -50,18 +50,36 @@ __device__ __forceinline__ int wrapi(int i, int n)
 // workgroup sets up the candidate runs (one per thread), scans their lengths,
 // tests the candidates 256 at a time and stages the hits in LDS in a fixed
 // order.  Each wavefront then takes the staged sources that touch its
-// sub-block, 32 at a time: one lane per source computes the sub-block's 4+4+4
-// window weights and stores P[j][k] = wy[j] wz[k] and wx[i] F; every lane then
-// accumulates its point, P[j][k] * (wx[i] F), three FMAs per source.  Every
-// grid point is written exactly once: no atomics, no clearing, and a fixed
-// summation order.
+// sub-block in batches.  FP64, 32 at a time: one lane per source computes the
+// sub-block's 4+4+4 window weights and stores P[j][k] = wy[j] wz[k] and
+// wx[i] F in LDS; every lane then accumulates its point, P[j][k] * (wx[i] F),
+// three FMAs per source.  FP32, 16 at a time: the lanes of row i (16 lanes)
+// evaluate the weights at offset i of the 16 sources, one source per lane, and
+// keep wx[i] F in registers; each lane of the row then takes it by a DPP row
+// broadcast folded into its multiply-adds, so only P[j][k] comes from LDS (a
+// b96 broadcast read of wx F per source and lane would make the loop LDS
+// bound).  Every grid point is written exactly once: no atomics, no clearing,
+// and a fixed summation order.
 constexpr int kSpreadWaves = 4;
 
 template <typename T> struct spread_cfg;
-template <> struct spread_cfg<float> { static constexpr int CAP = 448, HB = 32; };
+template <> struct spread_cfg<float> { static constexpr int CAP = 448, HB = 16; };
 template <> struct spread_cfg<double> { static constexpr int CAP = 256, HB = 32; };
 
-template <typename T> struct spread_lds {
+// Per-batch weight arrays.  FP64: one lane per source writes P[h][4 j + k] =
+// wy[j] wz[k] and wx[i] F.  FP32: wx F stays in the registers of the lane that
+// computed it (broadcast along its row of 16 lanes with DPP), wz goes through
+// Wz, and P[h][4 j + k] is formed by the lanes.
+template <typename T> struct spread_batch_lds {
+    T P[kSpreadWaves][spread_cfg<T>::HB][17];                      // wy[j] wz[k] at 4 j + k (odd stride)
+    typename cplx<T>::type4 wxF[kSpreadWaves][spread_cfg<T>::HB][4]; // wx[i] (fx, fy, fz)
+};
+template <> struct alignas(16) spread_batch_lds<float> {
+    alignas(16) float Wz[kSpreadWaves][16][4]; // wz[k] of source h
+    alignas(16) float P[kSpreadWaves][16][16]; // wy[j] wz[k] of source h at 4 j + k
+};
+
+template <typename T> struct spread_lds : spread_batch_lds<T> {
     static constexpr int CAP = spread_cfg<T>::CAP, HB = spread_cfg<T>::HB;
     int run_j[256];       // first source of each candidate run of the chunk
     int run_pre[257];     // exclusive prefix of the run lengths
@@ -73,8 +91,6 @@ template <typename T> struct spread_lds {
     T sF[3][CAP];
     int ring[kSpreadWaves][128];    // staged indices touching the wavefront's sub-block
                                     // (< HB pending + 64 appended)
-    T P[kSpreadWaves][HB][17];      // wy[j] wz[k] at 4 j + k (odd stride)
-    typename cplx<T>::type4 wxF[kSpreadWaves][HB][4]; // wx[i] (fx, fy, fz)
 };
 
 // Weights of hit slot `lane` (staged index e) on the sub-block at sb.
@@ -118,22 +134,100 @@ __device__ __forceinline__ void spread_weights(spread_lds<T> &S, int wave, int l
     }
 }
 
+// Value v of lane H of this lane's row of 16 (DPP row_newbcast).
+template <int H> __device__ __forceinline__ float row_bcast(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x150 + H, 0xf,
+                                                                 0xf, true));
+}
+
+template <int H>
+__device__ __forceinline__ void spread_steps16(const float (&P)[16][16], int jk, float f0, float f1, float f2,
+                                               float &a0, float &a1, float &a2)
+{
+    const float p = P[H][jk];
+    a0 = fmaf(row_bcast<H>(f0), p, a0);
+    a1 = fmaf(row_bcast<H>(f1), p, a1);
+    a2 = fmaf(row_bcast<H>(f2), p, a2);
+    if constexpr (H < 15) spread_steps16<H + 1>(P, jk, f0, f1, f2, a0, a1, a2);
+}
+
+// FP32 batch of up to 16 staged sources.  Lane (i, h) = (lane / 16, lane % 16)
+// evaluates source h's window weight at offset i of the sub-block on each axis
+// (zero outside the window or for h >= cnt); it keeps wx F, publishes wz and
+// forms P[h][4 i + k] = wy[i] wz[k].  The lane at point (i, j, k) then adds,
+// for each source, P[h][4 j + k] times wx F of lane h of its own row.
+__device__ __forceinline__ void spread_batch16(spread_lds<float> &S, int wave, int lane, int head, int cnt,
+                                               const int *sb, const pb_kgrid &g, float &a0, float &a1, float &a2)
+{
+    const int i = lane >> 4, h = lane & 15;
+    __builtin_amdgcn_wave_barrier();
+    const int e = S.ring[wave][(head + (h < cnt ? h : 0)) & 127];
+    float w[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int l = (int)((S.scode[e] >> (4 * a)) & 15) - 4;
+        const int gq = sb[a] + i;
+        const int m = gq - S.sf0[a][e];
+        float v = 0.0f;
+        if (h < cnt && m >= 0 && m < g.P[a]) {
+            const double dx = (double)(gq - l * g.n[a]) * g.h[a] - S.sx[a][e];
+            v = __expf(-(float)(dx * dx) * (float)(1.0 / g.eta));
+        }
+        w[a] = v;
+    }
+    const float f0 = w[0] * S.sF[0][e], f1 = w[0] * S.sF[1][e], f2 = w[0] * S.sF[2][e];
+    S.Wz[wave][h][i] = w[2];
+    __builtin_amdgcn_wave_barrier();
+    const float4 z = *reinterpret_cast<const float4 *>(&S.Wz[wave][h][0]);
+    *reinterpret_cast<float4 *>(&S.P[wave][h][4 * i]) = make_float4(w[1] * z.x, w[1] * z.y, w[1] * z.z, w[1] * z.w);
+    __builtin_amdgcn_wave_barrier();
+#ifndef PB_SPREAD_DPP_MOV
+    // the 48 multiply-adds with the row broadcast folded into each
+    // (v_fmac_f32 with a DPP source).  The leading s_nop 4 covers the wait
+    // states a DPP read needs after a VALU write of its source (two) or of
+    // the execution mask (five).  Build with PB_SPREAD_DPP_MOV for the
+    // compiler's form (a DPP move before each multiply-add).
+    float p[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) p[q] = S.P[wave][q][lane & 15];
+#define PB_FD(H, A, F, Q) "v_fmac_f32_dpp %" #A ", %" #F ", %" #Q " row_newbcast:" #H " row_mask:0xf bank_mask:0xf\n"
+#define PB_F3(H, Q) PB_FD(H, 0, 3, Q) PB_FD(H, 1, 4, Q) PB_FD(H, 2, 5, Q)
+    asm volatile("s_nop 4\n" PB_F3(0, 6) PB_F3(1, 7) PB_F3(2, 8) PB_F3(3, 9) PB_F3(4, 10) PB_F3(5, 11) PB_F3(6, 12)
+                     PB_F3(7, 13) PB_F3(8, 14) PB_F3(9, 15) PB_F3(10, 16) PB_F3(11, 17) PB_F3(12, 18) PB_F3(13, 19)
+                         PB_F3(14, 20) PB_F3(15, 21)
+                 : "+v"(a0), "+v"(a1), "+v"(a2)
+                 : "v"(f0), "v"(f1), "v"(f2), "v"(p[0]), "v"(p[1]), "v"(p[2]), "v"(p[3]), "v"(p[4]), "v"(p[5]),
+                   "v"(p[6]), "v"(p[7]), "v"(p[8]), "v"(p[9]), "v"(p[10]), "v"(p[11]), "v"(p[12]), "v"(p[13]),
+                   "v"(p[14]), "v"(p[15]));
+#undef PB_F3
+#undef PB_FD
+#else
+    spread_steps16<0>(S.P[wave], lane & 15, f0, f1, f2, a0, a1, a2);
+#endif
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <typename T>
 __device__ __forceinline__ void spread_batch(spread_lds<T> &S, int wave, int lane, int head, int cnt,
                                              const int *sb, const pb_kgrid &g, int li, int lj, int lk,
                                              T &a0, T &a1, T &a2)
 {
-    __builtin_amdgcn_wave_barrier();
-    if (lane < cnt) spread_weights<T>(S, wave, lane, S.ring[wave][(head + lane) & 127], sb, g);
-    __builtin_amdgcn_wave_barrier();
-    for (int h = 0; h < cnt; h++) {
-        const T p = S.P[wave][h][4 * lj + lk];
-        const typename cplx<T>::type4 v = S.wxF[wave][h][li];
-        a0 += p * v.x;
-        a1 += p * v.y;
-        a2 += p * v.z;
+    if constexpr (sizeof(T) == 4) {
+        spread_batch16(S, wave, lane, head, cnt, sb, g, a0, a1, a2);
+    } else {
+        __builtin_amdgcn_wave_barrier();
+        if (lane < cnt) spread_weights<T>(S, wave, lane, S.ring[wave][(head + lane) & 127], sb, g);
+        __builtin_amdgcn_wave_barrier();
+        for (int h = 0; h < cnt; h++) {
+            const T p = S.P[wave][h][4 * lj + lk];
+            const typename cplx<T>::type4 v = S.wxF[wave][h][li];
+            a0 += p * v.x;
+            a1 += p * v.y;
+            a2 += p * v.z;
+        }
+        __builtin_amdgcn_wave_barrier();
     }
-    __builtin_amdgcn_wave_barrier();
 }
 
 // The wavefront's two sub-blocks gather from the nst staged sources.
@@ -983,7 +1077,6 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
     km.inv_ng = 1.0 / (double)g.ng;
     km.beta = d * d / 4 - g.eta / 2;
     void *grid4 = c->grid.ptr, *spec = c->spec.ptr;
-    pb_mark(c, s, 1);
     {
         const long long nreg = (long long)((g.n[0] + 7) / 8) * ((g.n[1] + 7) / 8) * ((g.n[2] + 7) / 8);
         hipLaunchKernelGGL(k_spread_regions<T>, dim3((unsigned)nreg), dim3(64 * kSpreadWaves), 0, s,
@@ -1000,7 +1093,7 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
     }
     pb_mark(c, s, 5);
     if (nt > 0) launch_interp<T>(s, nt, g, c->tpos.ptr, c->tperm.ptr, (const C4 *)grid4, u, accumulate ? 1 : 0);
-    pb_mark(c, s, 6);
+    if (c->timing != 2) pb_mark(c, s, 6); // (mode 2: the pair kernel records boundary 6 itself)
     return pb_hip_check(c, hipGetLastError(), "k-space launch");
 }
 

@@ -15,15 +15,18 @@
 // then walks the runs one source at a time: the source is a uniform scalar
 // load and is tested against the 64 lane targets.
 //
-// In-range (target, source, image) triples are compacted with a ballot into
-// an LDS ring, one LDS write per tested source; each time 64 are pending
-// every lane evaluates one, so the evaluation runs with full lanes however
-// sparse the test is.  An evaluated batch moves its rare pairs with
-// r < rcommon (series and blob regions) to a third ring that is evaluated 64
-// at a time out of line.  Results go to per-target FP64 accumulators in LDS
-// (ds_add_f64).  With an FP32 far field (the MIXED template), pairs beyond
-// rfar (>= 3d and >= 8e) are classified at the test into the second half of
-// the ring and evaluated in FP32, and still accumulated in FP64.
+// Each candidate is tested against the 64 lane targets in FP32 (positions
+// relative to the tile's lower corner, rc enlarged by 1e-4 relative, so every
+// pair in range passes); the passing (target, source, image) triples are
+// compacted with a ballot into an LDS ring, one LDS write per tested source;
+// each time 64 are pending every lane evaluates one, so the evaluation runs
+// with full lanes however sparse the test is.  A batch applies the exact
+// cutoff r < rc in FP64 and moves its rare pairs with r < rcommon (series and
+// blob regions) to a third ring that is evaluated 64 at a time out of line.
+// Results go to per-target FP64 accumulators in LDS (ds_add_f64).  With an
+// FP32 far field (the MIXED template), pairs beyond rfar (>= 3d and >= 8e) by
+// the FP32 distance go to the second half of the ring and are evaluated in
+// FP32, and still accumulated in FP64.
 #include "pb_radial.h"
 
 namespace {
@@ -36,7 +39,7 @@ constexpr int kQ = 128;   // ring capacity (power of two)
 struct wave_lds {
     double y[64][3];
     double acc[64][3];
-    double4 stage[64]; // candidate sources of the current run: image position, index
+    float4 stage[64];  // candidates of the current run: position - tile origin (FP32), index bits
     uint2 q[2 * kQ];   // [0, kQ): FP64 ring; [kQ, 2 kQ): FP32 far ring
     uint2 qr[kQ];      // r < rcommon, moved out of evaluated FP64 batches
 };
@@ -113,12 +116,12 @@ struct batch_consts {
 
 template <bool TABLE>
 __device__ __forceinline__ void batch64(wave_lds &W, const int lane, const int cnt, const int head,
-                                        const int rh, int &rn, const double4 *__restrict__ spos,
-                                        const double4 *__restrict__ sfrc, const pb_box &box,
+                                        const int rh, int &rn, unsigned &nin, const double4 *__restrict__ spos,
+                                        const double4 *__restrict__ sfrc, const pb_box &box, const double rc2,
                                         const batch_consts &K, const double2 *__restrict__ tab)
 {
     __builtin_amdgcn_wave_barrier();
-    const bool on = lane < cnt;
+    bool on = lane < cnt;
     uint2 e = make_uint2(0u, 0u);
     int t = 0;
     double d0 = 0, d1 = 0, d2 = 0, r = 0;
@@ -132,8 +135,11 @@ __device__ __forceinline__ void batch64(wave_lds &W, const int lane, const int c
         d0 = W.y[t][0] - (xs.x + s0);
         d1 = W.y[t][1] - (xs.y + s1);
         d2 = W.y[t][2] - (xs.z + s2);
-        r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        on = r2 < rc2; // the exact cutoff (the FP32 candidate test lets a few more through)
+        r = sqrt(r2);
     }
+    nin += __popcll(__ballot(on));
     const bool rare = on && r < K.rcommon;
     const unsigned long long mr = __ballot(rare);
     if (mr) {
@@ -176,22 +182,30 @@ __device__ __forceinline__ void batch_rare(wave_lds &W, const int lane, const in
 
 // FP32 batch of the far ring (r >= rfar: only diffH(d), in FP32).
 template <bool TABLE>
-__device__ __forceinline__ void batch32(wave_lds &W, const int lane, const int cnt, const int head,
+__device__ __forceinline__ void batch32(wave_lds &W, const int lane, const int cnt, const int head, unsigned &nin,
                                         const double4 *__restrict__ spos,
-                                        const double4 *__restrict__ sfrc, const pb_box &box,
+                                        const double4 *__restrict__ sfrc, const pb_box &box, const double rc2,
                                         const batch_consts &K, const float2 *__restrict__ tab32)
 {
     __builtin_amdgcn_wave_barrier();
-    if (lane < cnt) {
-        const uint2 e = W.q[kQ + ((head + lane) & (kQ - 1))];
-        int t;
+    bool on = lane < cnt;
+    uint2 e = make_uint2(0u, 0u);
+    int t = 0;
+    double dd0 = 0, dd1 = 0, dd2 = 0;
+    if (on) {
+        e = W.q[kQ + ((head + lane) & (kQ - 1))];
         double s0, s1, s2;
         decode(e, box, t, s0, s1, s2);
         const double4 xs = spos[e.x];
+        dd0 = W.y[t][0] - (xs.x + s0);
+        dd1 = W.y[t][1] - (xs.y + s1);
+        dd2 = W.y[t][2] - (xs.z + s2);
+        on = dd0 * dd0 + dd1 * dd1 + dd2 * dd2 < rc2; // the exact cutoff
+    }
+    nin += __popcll(__ballot(on));
+    if (on) {
         const double4 fs = sfrc[e.x];
-        const float d0 = (float)(W.y[t][0] - (xs.x + s0));
-        const float d1 = (float)(W.y[t][1] - (xs.y + s1));
-        const float d2 = (float)(W.y[t][2] - (xs.z + s2));
+        const float d0 = (float)dd0, d1 = (float)dd1, d2 = (float)dd2;
         const float f0 = (float)fs.x, f1 = (float)fs.y, f2 = (float)fs.z;
         const float r = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
         float H1, H2;
@@ -251,8 +265,12 @@ __device__ __forceinline__ void pair_body(
     // bounding box of the tile (inactive lanes repeat the first target)
     const double bmin0 = wave_min(p.x), bmin1 = wave_min(p.y), bmin2 = wave_min(p.z);
     const double bmax0 = wave_max(p.x), bmax1 = wave_max(p.y), bmax2 = wave_max(p.z);
-    // inactive lanes are parked far away, so the distance test never passes
-    const double y0 = active ? p.x : 1e30, y1 = active ? p.y : 1e30, y2 = active ? p.z : 1e30;
+    // the candidate test runs in FP32 on positions relative to the tile's
+    // lower corner, against rc enlarged by 1e-4 (relative) so that every pair
+    // in range passes; the batches apply the exact cutoff in FP64.  Inactive
+    // lanes are parked far away, so the test never passes for them.
+    const float y0 = active ? (float)(p.x - bmin0) : 1e18f, y1 = active ? (float)(p.y - bmin1) : 1e18f,
+                y2 = active ? (float)(p.z - bmin2) : 1e18f;
     __builtin_amdgcn_wave_barrier();
 
     const int nc0 = cells.nc[0], nc1 = cells.nc[1], nc2 = cells.nc[2];
@@ -263,6 +281,7 @@ __device__ __forceinline__ void pair_body(
     const int ny = uhi1 - ulo1 + 1;
     const int ncols = (uhi0 - ulo0 + 1) * ny;
     const double rc2 = cells.rc2;
+    const float rc2f = (float)(rc2 * (1.0 + 1e-4)), rfar2f = (float)rfar2;
 
     int qh = 0, qn = 0, fh = 0, fn = 0, rh = 0, rn = 0;
     unsigned n64 = 0, n32 = 0, ntest = 0, nread = 0;
@@ -270,46 +289,48 @@ __device__ __forceinline__ void pair_body(
     auto flush = [&](const bool last) {
         while (qn >= 64 || (last && qn > 0)) {
             const int c = qn < 64 ? qn : 64;
-            batch64<TABLE>(W, lane, c, qh, rh, rn, spos, sfrc, box, K, tab);
+            batch64<TABLE>(W, lane, c, qh, rh, rn, n64, spos, sfrc, box, rc2, K, tab);
             qh = (qh + c) & (kQ - 1);
             qn -= c;
-            n64 += c;
         }
         while (rn >= 64 || (last && rn > 0)) {
             const int c = rn < 64 ? rn : 64;
             batch_rare(W, lane, c, rh, spos, sfrc, box, kpp);
             rh = (rh + c) & (kQ - 1);
-            rn -= c;
-            n64 += c;
+            rn -= c; // (counted when classified in batch64)
         }
         if (MIXED) {
             while (fn >= 64 || (last && fn > 0)) {
                 const int c = fn < 64 ? fn : 64;
-                batch32<TABLE>(W, lane, c, fh, spos, sfrc, box, K, tab32);
+                batch32<TABLE>(W, lane, c, fh, n32, spos, sfrc, box, rc2, K, tab32);
                 fh = (fh + c) & (kQ - 1);
                 fn -= c;
-                n32 += c;
             }
         }
     };
 
-    // append the in-range lanes of one tested candidate to the rings
-    auto append = [&](const unsigned jk, const double r2, const unsigned code) {
-        const bool in = r2 < rc2;
+    // append the lanes of one tested candidate that pass the FP32 test to the
+    // rings (with the FP32 far field: r >= rfar by the FP32 distance to the
+    // far ring; the FP32 decision near rfar only picks the precision)
+    auto append = [&](const unsigned jk, const float r2, const unsigned code) {
+        const bool in = r2 < rc2f;
         const unsigned long long m = __ballot(in);
         if (!MIXED) {
             if (in) W.q[(qh + qn + lane_prefix(m)) & (kQ - 1)] = make_uint2(jk, code);
             qn += __popcll(m);
             if (qn >= 64) flush(false);
         } else {
-            const bool far = r2 >= rfar2;
-            const unsigned long long mf = __ballot(in && far), mn = m & ~mf;
+            const bool far = in && r2 >= rfar2f;
+            const unsigned long long mf = __ballot(far);
+            const unsigned pm = lane_prefix(m), pf = lane_prefix(mf);
             if (in) {
-                const unsigned v = (unsigned)(far ? fh + fn : qh + qn) + lane_prefix(far ? mf : mn);
-                W.q[(v & (kQ - 1)) | (far ? kQ : 0)] = make_uint2(jk, code);
+                const unsigned v = far ? (((unsigned)(fh + fn) + pf) & (kQ - 1)) | kQ
+                                       : ((unsigned)(qh + qn) + pm - pf) & (kQ - 1);
+                W.q[v] = make_uint2(jk, code);
             }
-            qn += __popcll(mn);
-            fn += __popcll(mf);
+            const int nf = __popcll(mf);
+            qn += __popcll(m) - nf;
+            fn += nf;
             if (qn >= 64 || fn >= 64) flush(false);
         }
     };
@@ -375,23 +396,24 @@ __device__ __forceinline__ void pair_body(
                     ntest += nkeep;
                     __builtin_amdgcn_wave_barrier();
                     if (keep)
-                        W.stage[lane_prefix(km)] = make_double4(sx, sy, sz, __longlong_as_double((long long)(j + lane)));
+                        W.stage[lane_prefix(km)] = make_float4((float)(sx - bmin0), (float)(sy - bmin1),
+                                                               (float)(sz - bmin2), __int_as_float(j + lane));
                     __builtin_amdgcn_wave_barrier();
                     // two candidates per step: independent dependency chains
                     int k = 0;
                     for (; k + 1 < nkeep; k += 2) {
-                        const double4 sa = W.stage[k], sb = W.stage[k + 1]; // broadcast reads
-                        const double a0 = y0 - sa.x, a1 = y1 - sa.y, a2 = y2 - sa.z;
-                        const double b0 = y0 - sb.x, b1 = y1 - sb.y, b2 = y2 - sb.z;
-                        const double ra2 = a0 * a0 + a1 * a1 + a2 * a2;
-                        const double rb2 = b0 * b0 + b1 * b1 + b2 * b2;
-                        append((unsigned)__double_as_longlong(sa.w), ra2, code);
-                        append((unsigned)__double_as_longlong(sb.w), rb2, code);
+                        const float4 sa = W.stage[k], sb = W.stage[k + 1]; // broadcast reads
+                        const float a0 = y0 - sa.x, a1 = y1 - sa.y, a2 = y2 - sa.z;
+                        const float b0 = y0 - sb.x, b1 = y1 - sb.y, b2 = y2 - sb.z;
+                        const float ra2 = a0 * a0 + a1 * a1 + a2 * a2;
+                        const float rb2 = b0 * b0 + b1 * b1 + b2 * b2;
+                        append((unsigned)__float_as_int(sa.w), ra2, code);
+                        append((unsigned)__float_as_int(sb.w), rb2, code);
                     }
                     if (k < nkeep) {
-                        const double4 sa = W.stage[k];
-                        const double a0 = y0 - sa.x, a1 = y1 - sa.y, a2 = y2 - sa.z;
-                        append((unsigned)__double_as_longlong(sa.w), a0 * a0 + a1 * a1 + a2 * a2, code);
+                        const float4 sa = W.stage[k];
+                        const float a0 = y0 - sa.x, a1 = y1 - sa.y, a2 = y2 - sa.z;
+                        append((unsigned)__float_as_int(sa.w), a0 * a0 + a1 * a1 + a2 * a2, code);
                     }
                     __builtin_amdgcn_wave_barrier();
                 }

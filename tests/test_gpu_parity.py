@@ -215,3 +215,32 @@ def test_split_invariance_on_gpu(ctx):
     torch.cuda.synchronize()
     assert ds[1] > 1.05 * ds[0]
     assert relL2(us[1], us[0]) < 1e-9
+
+
+def test_phase_timing_modes(ctx):
+    """Every boundary (mode 1) gives every phase of a full call; the pair-only
+    mode records just the pair kernel; a real-space call has no k-space phases."""
+    import torch
+    x, f, y = _system(21, 3000, 2000, (1.0, 1.0, 1.0))
+    p = pb.Params(alpha=10.0, eps=1e-3, grid=(32, 32, 32), P=8, tol=1e-6,
+                  precision="mixed").tune(3000, 2000)
+    xs, fs, ys = dev(x), dev(f), dev(y)
+    ctx.velocity(p, xs, fs, ys)
+    ctx.set_timing(True)
+    for _ in range(3):
+        ctx.velocity(p, xs, fs, ys)
+    full = ctx.last_timings()
+    ctx.set_timing("pair")
+    for _ in range(3):
+        ctx.velocity(p, xs, fs, ys)
+    pair = ctx.last_timings()
+    ctx.set_timing(True)
+    for _ in range(3):
+        ctx.velocity_real(p, xs, fs, ys)
+    real = ctx.last_timings()
+    ctx.set_timing(False)
+    torch.cuda.synchronize()
+    assert all(v > 0 for v in full.values()), full
+    assert pair["pair"] > 0 and all(v == 0 for k, v in pair.items() if k != "pair"), pair
+    assert real["binning"] > 0 and real["pair"] > 0, real
+    assert all(real[k] == 0 for k in ("spread", "fft_forward", "multiply", "fft_inverse", "gather")), real
