@@ -187,6 +187,14 @@ const pb_kparams &cached_kparams(double a, double d, double e, double rfar)
 // Host copy of the kernel constants and (table kernels) the radial table on
 // [rcommon, rc]: coefficients in coef (empty without a table); the table
 // pointers of kp are left for the caller.
+// Degree of the radial table for a precision: the FP64 form keeps the table
+// at FP64 rounding (4e-15), the mixed form needs 1e-10 and takes degree 6
+// (112 bytes of coefficients per pair instead of 208, 14 FMAs instead of 26).
+int table_degree(const pb_params *p)
+{
+    return p->precision == PB_PREC_MIXED ? PB_TAB_DEG_MIXED : PB_TAB_DEG;
+}
+
 void host_kparams(const pb_params *p, double rfar, pb_kparams *kp, std::vector<double> &coef)
 {
     *kp = cached_kparams(p->alpha, p->d, p->eps, rfar);
@@ -195,7 +203,9 @@ void host_kparams(const pb_params *p, double rfar, pb_kparams *kp, std::vector<d
     kp->tab32 = nullptr;
     coef.clear();
     if (p->kernel != PB_KERNEL_DIRECT && p->rc > kp->rcommon) {
-        pb_build_table(kp, kp->rcommon / p->d, p->rc / p->d * (1 + 1e-9), coef);
+        const bool mixed = p->precision == PB_PREC_MIXED;
+        pb_build_table(kp, kp->rcommon / p->d, p->rc / p->d * (1 + 1e-9), coef, table_degree(p),
+                       mixed ? 1e-10 : 4e-15);
         if (kp->tn == 0) coef.clear();
     }
 }
@@ -206,10 +216,10 @@ int device_kparams(pb_context *c, hipStream_t s, const pb_params *p, double rfar
                    const pb_kparams **out, size_t *table_doubles)
 {
     int st;
-    const double key[6] = {p->alpha, p->d, p->eps, rfar, p->rc, (double)p->kernel};
+    const double key[7] = {p->alpha, p->d, p->eps, rfar, p->rc, (double)p->kernel, (double)table_degree(p)};
     if ((st = pb_reserve(c, &c->kparams, sizeof(pb_kparams)))) return st;
     bool same = true;
-    for (int i = 0; i < 6; i++) same = same && c->kp_key[i] == key[i];
+    for (int i = 0; i < 7; i++) same = same && c->kp_key[i] == key[i];
     if (!same) {
         pb_kparams kp;
         std::vector<double> coef;
@@ -233,7 +243,7 @@ int device_kparams(pb_context *c, hipStream_t s, const pb_params *p, double rfar
                                "kparams upload")))
             return st;
         if ((st = pb_hip_check(c, hipStreamSynchronize(s), "kparams upload"))) return st;
-        for (int i = 0; i < 6; i++) c->kp_key[i] = key[i];
+        for (int i = 0; i < 7; i++) c->kp_key[i] = key[i];
     }
     *out = (const pb_kparams *)c->kparams.ptr;
     *table_doubles = c->table_doubles;
@@ -270,7 +280,9 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
             return st;
         acc = true;
     }
-    if (do_real && (st = pb_run_real(c, s, cells, kp, ntab, box, ns, nt, u, acc, far32_of(p, ntab)))) return st;
+    if (do_real &&
+        (st = pb_run_real(c, s, cells, kp, ntab, table_degree(p), box, ns, nt, u, acc, far32_of(p, ntab))))
+        return st;
     return PB_OK;
 }
 
@@ -507,7 +519,7 @@ int pb_velocity_multirhs(pb_context *c, const pb_params *p, int ns, const double
     const pb_kparams *kp = nullptr;
     size_t ntab = 0;
     if ((st = device_kparams(c, s, &pt, rfar, &kp, &ntab))) return st;
-    return pb_run_mrhs_real(c, s, cells, kp, ntab, box, ns, nt, R, RP, u);
+    return pb_run_mrhs_real(c, s, cells, kp, ntab, table_degree(&pt), box, ns, nt, R, RP, u);
 }
 
 int pb_velocity_batched(pb_context *c, int nsys, pb_params *params, const int *src_off, const double *xn,
@@ -535,11 +547,13 @@ int pb_velocity_batched(pb_context *c, int nsys, pb_params *params, const int *s
     hipStream_t s = (hipStream_t)stream;
     // kernel constants and tables of every system, uploaded together; kept on
     // the device while the systems' parameters do not change
-    std::vector<double> key((size_t)nsys * 6);
+    constexpr int KW = 7; // key words per system
+    std::vector<double> key((size_t)nsys * KW);
     for (int i = 0; i < nsys; i++) {
         const pb_params &p = params[i];
-        const double k6[6] = {p.alpha, p.d, p.eps, std::fmax(3 * p.d, 8 * p.eps), p.rc, (double)p.kernel};
-        std::copy(k6, k6 + 6, key.begin() + 6 * (size_t)i);
+        const double k7[KW] = {p.alpha, p.d, p.eps, std::fmax(3 * p.d, 8 * p.eps), p.rc, (double)p.kernel,
+                               (double)table_degree(&p)};
+        std::copy(k7, k7 + KW, key.begin() + KW * (size_t)i);
     }
     if (key != c->batch_key) {
         std::vector<pb_kparams> kps(nsys);
@@ -549,7 +563,8 @@ int pb_velocity_batched(pb_context *c, int nsys, pb_params *params, const int *s
         for (int i = 0; i < nsys; i++) {
             rep[i] = i;
             for (int j : uniq)
-                if (std::equal(key.begin() + 6 * (size_t)i, key.begin() + 6 * (size_t)i + 6, key.begin() + 6 * (size_t)j)) {
+                if (std::equal(key.begin() + KW * (size_t)i, key.begin() + KW * (size_t)i + KW,
+                               key.begin() + KW * (size_t)j)) {
                     rep[i] = j;
                     break;
                 }

@@ -71,7 +71,8 @@ struct pb_kparams {
 };
 
 #define PB_TAB_MAXDEG 15
-#define PB_TAB_DEG 12 // degree of the table polynomials
+#define PB_TAB_DEG 12      // degree of the table polynomials (FP64 form, error <= 4e-15)
+#define PB_TAB_DEG_MIXED 6 // degree for the mixed form (error <= 1e-10): fewer bytes and FMAs per pair
 #define PB_TAB_LDS_DOUBLES 2048 // LDS budget of the FP64 table (16 KB)
 
 struct pb_box {
@@ -105,10 +106,12 @@ struct pb_kgrid {
 };
 
 // ---- host helpers (pb_params.cpp) ----------------------------------------
-// Build the table of diffH(d) on z in [z0, z1]: fills tz0, tinv, tn, tdeg and
-// the coefficients (layout [tn][deg+1][2]); returns the verified max error
-// relative to each interval's largest |H|.
-double pb_build_table(pb_kparams *k, double z0, double z1, std::vector<double> &coef);
+// Build the table of diffH(d) on z in [z0, z1] with polynomials of degree deg
+// whose verified error (relative to each interval's largest |H|) is <= tol:
+// fills tz0, tinv, tn, tdeg and the coefficients (layout [tn][deg+1][2]);
+// returns that error (tn = 0: no table within the LDS budget).
+double pb_build_table(pb_kparams *k, double z0, double z1, std::vector<double> &coef, int deg = PB_TAB_DEG,
+                      double tol = 4e-15);
 void pb_fill_radial(pb_radial *r, double a, double d);
 void pb_fill_kparams(pb_kparams *k, double a, double d, double e, double rfar);
 void pb_fill_box(pb_box *b, const double *L);
@@ -154,7 +157,7 @@ struct pb_context {
     // kernel constants in device memory (kernels take a pointer: a by-value
     // copy of the series tables in kernel arguments costs registers)
     pb_buffer kparams, table;
-    double kp_key[6] = {-1, -1, -1, -1, -1, -1};
+    double kp_key[7] = {-1, -1, -1, -1, -1, -1, -1};
     size_t table_doubles = 0;
     // per-work-item statistics of the last real-space pass (uint4 each in
     // counters): FP64 pairs, FP32 pairs, candidates tested, candidates read
@@ -188,7 +191,8 @@ int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_
 int pb_sort_forces(pb_context *c, hipStream_t s, int ns, int R, int RP, const float *f);
 int pb_force_set(pb_context *c, hipStream_t s, int ns, int r, int RP);
 int pb_run_mrhs_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp,
-                     size_t table_doubles, const pb_box &box, int ns, int nt, int R, int RP, double *u);
+                     size_t table_doubles, int table_deg, const pb_box &box, int ns, int nt, int R, int RP,
+                     double *u);
 
 // Target tiles: runs of <= 64 sorted targets inside one quarter brick; the
 // tile count is left on the device, *max_tiles bounds it.  Built by
@@ -220,7 +224,7 @@ __device__ __forceinline__ int pb_xcd_block(int b, int nblk8)
 // far32: pairs beyond rfar in FP32 (table copy when table_doubles > 0, else
 // the FP32 closed form); otherwise every pair in FP64.
 int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp_dev,
-                size_t table_doubles, const pb_box &box, int ns, int nt, double *u, bool accumulate,
+                size_t table_doubles, int table_deg, const pb_box &box, int ns, int nt, double *u, bool accumulate,
                 bool far32);
 int pb_run_kspace(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells,
                   const pb_box &box, double a, double d, int ns, int nt, double *u, bool accumulate,

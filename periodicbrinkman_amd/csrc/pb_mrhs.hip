@@ -61,7 +61,7 @@ __device__ __forceinline__ int floordiv(int a, int b)
 }
 
 // Contract the 16 ring entries from head (cnt <= 16 of them valid) into acc.
-template <int NB>
+template <int NB, int DEG>
 __device__ __forceinline__ void contract16(const mrhs_lds &W, int head, int cnt, int lane, double y0, double y1,
                                            double y2, bool active, double rc2, double rcommon,
                                            const pbdev::tabview &tv, const double2 *__restrict__ tab,
@@ -84,7 +84,7 @@ __device__ __forceinline__ void contract16(const mrhs_lds &W, int head, int cnt,
         if (in) {
             const double r = sqrt(r2);
             if (r >= rcommon) {
-                pbdev::table_H(r, tv, tab, h1, h2);
+                pbdev::table_H<DEG>(r, tv, tab, h1, h2);
             } else {
                 const double2 H = pbdev::vR_H_call(r, kpp);
                 h1 = H.x;
@@ -115,7 +115,7 @@ __device__ __forceinline__ void contract16(const mrhs_lds &W, int head, int cnt,
     }
 }
 
-template <int NB>
+template <int NB, int DEG>
 __global__ __launch_bounds__(64 * kMWaves, 2) void k_pair_mrhs(
     const int2 *__restrict__ tiles, const int *__restrict__ ntiles_ptr, const int *__restrict__ cstart,
     const double4 *__restrict__ spos, const float *__restrict__ Fs, int RP, int c0, int R,
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(64 * kMWaves, 2) void k_pair_mrhs(
     const double rcommon = kp.rcommon;
     __shared__ mrhs_lds lds[kMWaves];
     extern __shared__ double dyn_lds[]; // the radial table
-    const int ntab = kp.tn * (PB_TAB_DEG + 1) * 2;
+    const int ntab = kp.tn * (DEG + 1) * 2;
     for (int k = threadIdx.x; k < ntab; k += blockDim.x) dyn_lds[k] = kp.tab64[k];
     __syncthreads();
     const double2 *tab = reinterpret_cast<const double2 *>(dyn_lds);
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(64 * kMWaves, 2) void k_pair_mrhs(
                     pend += __popcll(km);
                     __builtin_amdgcn_wave_barrier();
                     while (pend >= 16) {
-                        contract16<NB>(W, head, 16, lane, y0, y1, y2, active, rc2, rcommon, tv, tab, kpp, Fs, RP,
+                        contract16<NB, DEG>(W, head, 16, lane, y0, y1, y2, active, rc2, rcommon, tv, tab, kpp, Fs, RP,
                                        c0, acc);
                         head = (head + 16) & 127;
                         pend -= 16;
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(64 * kMWaves, 2) void k_pair_mrhs(
     }
     __builtin_amdgcn_wave_barrier();
     if (pend > 0)
-        contract16<NB>(W, head, pend, lane, y0, y1, y2, active, rc2, rcommon, tv, tab, kpp, Fs, RP, c0, acc);
+        contract16<NB, DEG>(W, head, pend, lane, y0, y1, y2, active, rc2, rcommon, tv, tab, kpp, Fs, RP, c0, acc);
     // C[i][j]: column j = lane & 31, row i = 8 (v / 4) + 4 (lane / 32) + v % 4
     const int kh = lane >> 5;
 #pragma unroll
@@ -299,7 +299,8 @@ int pb_force_set(pb_context *c, hipStream_t s, int ns, int r, int RP)
 }
 
 int pb_run_mrhs_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp,
-                     size_t table_doubles, const pb_box &box, int ns, int nt, int R, int RP, double *u)
+                     size_t table_doubles, int table_deg, const pb_box &box, int ns, int nt, int R, int RP,
+                     double *u)
 {
     (void)ns;
     if (nt <= 0 || R <= 0) return PB_OK;
@@ -309,8 +310,11 @@ int pb_run_mrhs_real(pb_context *c, hipStream_t s, const pb_cells &cells, const 
     const size_t lds = table_doubles * sizeof(double);
     // 32 right-hand sides per launch: one 32 x 32 accumulator block per
     // component stays in registers without spilling
+    if (table_deg != PB_TAB_DEG && table_deg != PB_TAB_DEG_MIXED)
+        return pb_fail(c, PB_ERR_INVALID, "radial table of an unsupported degree");
+    auto kern = table_deg == PB_TAB_DEG ? k_pair_mrhs<1, PB_TAB_DEG> : k_pair_mrhs<1, PB_TAB_DEG_MIXED>;
     for (int c0 = 0; c0 < RP; c0 += 32)
-        hipLaunchKernelGGL((k_pair_mrhs<1>), dim3(blocks), dim3(64 * kMWaves), lds, s, (const int2 *)c->tiles.ptr,
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kMWaves), lds, s, (const int2 *)c->tiles.ptr,
                            ntiles, (const int *)c->cell_start.ptr, (const double4 *)c->spos.ptr,
                            (const float *)c->mrhs_f.ptr, RP, c0, R, (const double4 *)c->tpos.ptr,
                            (const int *)c->tperm.ptr, nt, cells, box, kp, u);

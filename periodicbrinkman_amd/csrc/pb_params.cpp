@@ -150,21 +150,26 @@ void cheb_to_monomial(const long double *cheb, int deg, long double *mono)
 
 } // namespace
 
-double pb_build_table(pb_kparams *k, double z0, double z1, std::vector<double> &coef)
+double pb_build_table(pb_kparams *k, double z0, double z1, std::vector<double> &coef, int deg, double tol)
 {
     const long double a = k->pd.a, d = k->pd.d;
-    const int deg = PB_TAB_DEG;
-    // Chebyshev nodes and the cosine matrix of the coefficient transform
+    if (deg < 1 || deg > PB_TAB_MAXDEG) deg = PB_TAB_DEG;
+    // Chebyshev nodes and the cosine matrix of the coefficient transform, per degree
     struct cheb_consts {
-        long double node[PB_TAB_DEG + 1], cosm[PB_TAB_DEG + 1][PB_TAB_DEG + 1];
-        cheb_consts()
+        long double node[PB_TAB_MAXDEG + 1], cosm[PB_TAB_MAXDEG + 1][PB_TAB_MAXDEG + 1];
+        explicit cheb_consts(int n)
         {
-            for (int j = 0; j <= PB_TAB_DEG; j++) node[j] = cosl(kPi * (j + 0.5L) / (PB_TAB_DEG + 1));
-            for (int q = 0; q <= PB_TAB_DEG; q++)
-                for (int j = 0; j <= PB_TAB_DEG; j++) cosm[q][j] = cosl(kPi * q * (j + 0.5L) / (PB_TAB_DEG + 1));
+            for (int j = 0; j <= n; j++) node[j] = cosl(kPi * (j + 0.5L) / (n + 1));
+            for (int q = 0; q <= n; q++)
+                for (int j = 0; j <= n; j++) cosm[q][j] = cosl(kPi * q * (j + 0.5L) / (n + 1));
         }
     };
-    static const cheb_consts CC;
+    static const cheb_consts *const CCs[PB_TAB_MAXDEG + 1] = {
+        nullptr, new cheb_consts(1), new cheb_consts(2), new cheb_consts(3), new cheb_consts(4),
+        new cheb_consts(5), new cheb_consts(6), new cheb_consts(7), new cheb_consts(8), new cheb_consts(9),
+        new cheb_consts(10), new cheb_consts(11), new cheb_consts(12), new cheb_consts(13), new cheb_consts(14),
+        new cheb_consts(15)};
+    const cheb_consts &CC = *CCs[deg];
     double err = 1;
     for (int per_unit = 8; per_unit <= 256; per_unit *= 2) {
         int n = (int)std::ceil((z1 - z0) * per_unit);
@@ -212,7 +217,7 @@ double pb_build_table(pb_kparams *k, double z0, double z1, std::vector<double> &
         }
         err = worst;
         if ((size_t)n * (deg + 1) * 2 > PB_TAB_LDS_DOUBLES) break; // too large: no table
-        if (worst <= 4e-15) {
+        if (worst <= tol) {
             k->tz0 = z0;
             k->tinv = per_unit;
             k->tn = n;

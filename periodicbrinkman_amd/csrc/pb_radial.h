@@ -200,6 +200,7 @@ __device__ __forceinline__ tabview tabview_of(const pb_kparams &K)
 
 // Table evaluation of diffH(d) for r >= K.rcommon: interval i of z = r/d and
 // Horner in t on the interleaved (H1, H2) coefficients held in LDS.
+template <int DEG>
 __device__ __forceinline__ void table_H(double r, const tabview &V, const double2 *__restrict__ tab,
                                         double &H1, double &H2)
 {
@@ -207,10 +208,10 @@ __device__ __forceinline__ void table_H(double r, const tabview &V, const double
     int i = (int)u;
     i = i < V.tn ? i : V.tn - 1;
     const double t = 2.0 * (u - (double)i) - 1.0;
-    const double2 *c = tab + i * (PB_TAB_DEG + 1);
-    double h1 = c[PB_TAB_DEG].x, h2 = c[PB_TAB_DEG].y;
+    const double2 *c = tab + i * (DEG + 1);
+    double h1 = c[DEG].x, h2 = c[DEG].y;
 #pragma unroll
-    for (int k = PB_TAB_DEG - 1; k >= 0; k--) {
+    for (int k = DEG - 1; k >= 0; k--) {
         h1 = fma(h1, t, c[k].x);
         h2 = fma(h2, t, c[k].y);
     }
@@ -228,6 +229,7 @@ __device__ __forceinline__ tabview32 tabview32_of(const pb_kparams &K)
     return tabview32{(float)K.pd.inv_d, (float)K.tz0, (float)K.tinv, K.tn};
 }
 
+template <int DEG>
 __device__ __forceinline__ void table_H32(float r, const tabview32 &V, const float2 *__restrict__ tab,
                                           float &H1, float &H2)
 {
@@ -235,10 +237,10 @@ __device__ __forceinline__ void table_H32(float r, const tabview32 &V, const flo
     int i = (int)u;
     i = i < V.tn ? i : V.tn - 1;
     const float t = 2.0f * (u - (float)i) - 1.0f;
-    const float2 *c = tab + i * (PB_TAB_DEG + 1);
-    float h1 = c[PB_TAB_DEG].x, h2 = c[PB_TAB_DEG].y;
+    const float2 *c = tab + i * (DEG + 1);
+    float h1 = c[DEG].x, h2 = c[DEG].y;
 #pragma unroll
-    for (int k = PB_TAB_DEG - 1; k >= 0; k--) {
+    for (int k = DEG - 1; k >= 0; k--) {
         h1 = fmaf(h1, t, c[k].x);
         h2 = fmaf(h2, t, c[k].y);
     }

@@ -114,7 +114,7 @@ struct batch_consts {
     double rcommon;
 };
 
-template <bool TABLE>
+template <bool TABLE, int DEG>
 __device__ __forceinline__ void batch64(wave_lds &W, const int lane, const int cnt, const int head,
                                         const int rh, int &rn, unsigned &nin, const double4 *__restrict__ spos,
                                         const double4 *__restrict__ sfrc, const pb_box &box, const double rc2,
@@ -149,7 +149,7 @@ __device__ __forceinline__ void batch64(wave_lds &W, const int lane, const int c
     if (on && !rare) {
         double H1, H2;
         if (TABLE)
-            pbdev::table_H(r, K.tv, tab, H1, H2);
+            pbdev::table_H<DEG>(r, K.tv, tab, H1, H2);
         else
             pbdev::diff_direct(r, r * K.core.inv_d, K.core, H1, H2);
         accumulate(W, t, H1, H2, d0, d1, d2, fs);
@@ -181,7 +181,7 @@ __device__ __forceinline__ void batch_rare(wave_lds &W, const int lane, const in
 }
 
 // FP32 batch of the far ring (r >= rfar: only diffH(d), in FP32).
-template <bool TABLE>
+template <bool TABLE, int DEG>
 __device__ __forceinline__ void batch32(wave_lds &W, const int lane, const int cnt, const int head, unsigned &nin,
                                         const double4 *__restrict__ spos,
                                         const double4 *__restrict__ sfrc, const pb_box &box, const double rc2,
@@ -210,7 +210,7 @@ __device__ __forceinline__ void batch32(wave_lds &W, const int lane, const int c
         const float r = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
         float H1, H2;
         if (TABLE)
-            pbdev::table_H32(r, K.tv32, tab32, H1, H2);
+            pbdev::table_H32<DEG>(r, K.tv32, tab32, H1, H2);
         else
             pbdev::diff_far32(r, K.pf, H1, H2);
         const float g = H2 * (d0 * f0 + d1 * f1 + d2 * f2);
@@ -220,7 +220,7 @@ __device__ __forceinline__ void batch32(wave_lds &W, const int lane, const int c
     }
 }
 
-template <bool MIXED, bool TABLE>
+template <bool MIXED, bool TABLE, int DEG>
 __device__ __forceinline__ void pair_body(
     const int2 *__restrict__ tiles, const int *__restrict__ ntiles_ptr,
     const int *__restrict__ cstart, const double4 *__restrict__ spos,
@@ -234,7 +234,7 @@ __device__ __forceinline__ void pair_body(
     const double rfar2 = kp.rfar2;
     __shared__ wave_lds lds[kWaves];
     extern __shared__ double dyn_lds[]; // the radial table: FP64 then (FP32 far field) FP32 copy
-    const int ntab = TABLE ? kp.tn * (PB_TAB_DEG + 1) * 2 : 0;
+    const int ntab = TABLE ? kp.tn * (DEG + 1) * 2 : 0;
     if (TABLE) { // every wavefront takes part before any may leave
         for (int k = threadIdx.x; k < ntab; k += blockDim.x) dyn_lds[k] = kp.tab64[k];
         if (MIXED) {
@@ -289,7 +289,7 @@ __device__ __forceinline__ void pair_body(
     auto flush = [&](const bool last) {
         while (qn >= 64 || (last && qn > 0)) {
             const int c = qn < 64 ? qn : 64;
-            batch64<TABLE>(W, lane, c, qh, rh, rn, n64, spos, sfrc, box, rc2, K, tab);
+            batch64<TABLE, DEG>(W, lane, c, qh, rh, rn, n64, spos, sfrc, box, rc2, K, tab);
             qh = (qh + c) & (kQ - 1);
             qn -= c;
         }
@@ -302,7 +302,7 @@ __device__ __forceinline__ void pair_body(
         if (MIXED) {
             while (fn >= 64 || (last && fn > 0)) {
                 const int c = fn < 64 ? fn : 64;
-                batch32<TABLE>(W, lane, c, fh, n32, spos, sfrc, box, rc2, K, tab32);
+                batch32<TABLE, DEG>(W, lane, c, fh, n32, spos, sfrc, box, rc2, K, tab32);
                 fh = (fh + c) & (kQ - 1);
                 fn -= c;
             }
@@ -458,22 +458,22 @@ __global__ void k_zero(long long n, double *__restrict__ a)
 #ifndef PB_PAIR_OCC
 #define PB_PAIR_OCC 4
 #endif
-template <bool MIXED>
+template <bool MIXED, int DEG>
 __global__ __launch_bounds__(64 * kWaves, PB_PAIR_OCC) void k_pair_table(PB_PAIR_PARAMS)
 {
-    pair_body<MIXED, true>(PB_PAIR_FWD);
+    pair_body<MIXED, true, DEG>(PB_PAIR_FWD);
 }
 
 template <bool MIXED>
 __global__ __launch_bounds__(64 * kWaves) void k_pair_direct(PB_PAIR_PARAMS)
 {
-    pair_body<MIXED, false>(PB_PAIR_FWD);
+    pair_body<MIXED, false, PB_TAB_DEG>(PB_PAIR_FWD);
 }
 
 } // namespace
 
 int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp,
-                size_t table_doubles, const pb_box &box, int ns, int nt, double *u, bool accumulate,
+                size_t table_doubles, int table_deg, const pb_box &box, int ns, int nt, double *u, bool accumulate,
                 bool far32)
 {
     (void)ns;
@@ -502,9 +502,14 @@ int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kp
         (const double4 *)c->sfrc.ptr, (const double4 *)c->tpos.ptr, (const int *)c->tperm.ptr, nsplit,    \
         cells, box, kp, u, (uint4 *)c->counters.ptr
     dim3 gd(blocks), bd(64 * kWaves);
-    if (mixed && ntab) hipLaunchKernelGGL((k_pair_table<true>), gd, bd, lds, s, PB_PAIR_ARGS);
+    if (ntab && table_deg != PB_TAB_DEG && table_deg != PB_TAB_DEG_MIXED)
+        return pb_fail(c, PB_ERR_INVALID, "radial table of an unsupported degree");
+    const bool lo = table_deg == PB_TAB_DEG_MIXED;
+    if (mixed && ntab && lo) hipLaunchKernelGGL((k_pair_table<true, PB_TAB_DEG_MIXED>), gd, bd, lds, s, PB_PAIR_ARGS);
+    else if (mixed && ntab) hipLaunchKernelGGL((k_pair_table<true, PB_TAB_DEG>), gd, bd, lds, s, PB_PAIR_ARGS);
     else if (mixed) hipLaunchKernelGGL((k_pair_direct<true>), gd, bd, 0, s, PB_PAIR_ARGS);
-    else if (ntab) hipLaunchKernelGGL((k_pair_table<false>), gd, bd, lds, s, PB_PAIR_ARGS);
+    else if (ntab && lo) hipLaunchKernelGGL((k_pair_table<false, PB_TAB_DEG_MIXED>), gd, bd, lds, s, PB_PAIR_ARGS);
+    else if (ntab) hipLaunchKernelGGL((k_pair_table<false, PB_TAB_DEG>), gd, bd, lds, s, PB_PAIR_ARGS);
     else hipLaunchKernelGGL((k_pair_direct<false>), gd, bd, 0, s, PB_PAIR_ARGS);
 #undef PB_PAIR_ARGS
     return pb_hip_check(c, hipGetLastError(), "pair kernel launch");
