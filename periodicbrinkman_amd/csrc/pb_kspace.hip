@@ -18,6 +18,12 @@ namespace {
 
 constexpr int kPmax = 32;
 
+// Three components packed without padding (the inverse transform's output:
+// the gather reads 12 or 24 bytes per window point instead of 16 or 32).
+template <typename T> struct trip {
+    T x, y, z;
+};
+
 template <typename T> struct cplx;
 template <> struct cplx<double> {
     using type = double2;
@@ -425,7 +431,7 @@ template <typename T, int P>
 __global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, int nblk8,
                                                             const double4 *__restrict__ tpos,
                                                             const int *__restrict__ tperm, pb_kgrid g,
-                                                            const typename cplx<T>::type4 *__restrict__ U4,
+                                                            const trip<T> *__restrict__ U3,
                                                             double *__restrict__ u, int accumulate)
 {
     constexpr int NQ = P * P, NR = (NQ + 63) / 64;
@@ -460,7 +466,7 @@ __global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, int nblk8,
 #pragma unroll
             for (int px = 0; px < P; px++) {
                 const T wq = wts[wave][0][px] * wyz;
-                const typename cplx<T>::type4 v = U4[(unsigned)(off[wave][0][px] + oyz)];
+                const trip<T> v = U3[(unsigned)(off[wave][0][px] + oyz)];
                 a0 += wq * v.x;
                 a1 += wq * v.y;
                 a2 += wq * v.z;
@@ -489,7 +495,7 @@ __global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, int nblk8,
 template <typename T>
 __global__ __launch_bounds__(256) void k_interp_any(int nt, const double4 *__restrict__ tpos,
                                                     const int *__restrict__ tperm, pb_kgrid g,
-                                                    const typename cplx<T>::type4 *__restrict__ U4,
+                                                    const trip<T> *__restrict__ U3,
                                                     double *__restrict__ u, int accumulate)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -511,7 +517,7 @@ __global__ __launch_bounds__(256) void k_interp_any(int nt, const double4 *__res
                 const double dxz = (double)(f0[2] + pz) * g.h[2] - yc[2];
                 const double wt = wxy * exp(-dxz * dxz / g.eta);
                 const int jz = ((f0[2] + pz) % g.n[2] + g.n[2]) % g.n[2];
-                const typename cplx<T>::type4 v = U4[((size_t)jx * g.n[1] + jy) * g.n[2] + jz];
+                const trip<T> v = U3[((size_t)jx * g.n[1] + jy) * g.n[2] + jz];
                 a0 += wt * (double)v.x;
                 a1 += wt * (double)v.y;
                 a2 += wt * (double)v.z;
@@ -533,7 +539,7 @@ __global__ __launch_bounds__(256) void k_interp_any(int nt, const double4 *__res
 
 template <typename T>
 void launch_interp(hipStream_t s, int nt, const pb_kgrid &g, const void *tpos, const void *tperm,
-                   const typename cplx<T>::type4 *U4, double *u, int acc)
+                   const trip<T> *U3, double *u, int acc)
 {
     const int nblk8 = ((nt + kInterpWaves - 1) / kInterpWaves + 7) & ~7;
     dim3 gd(nblk8), bd(64 * kInterpWaves);
@@ -541,7 +547,7 @@ void launch_interp(hipStream_t s, int nt, const pb_kgrid &g, const void *tpos, c
     const double4 *tp = (const double4 *)tpos;
     const int *pm = (const int *)tperm;
     bool same = g.P[0] == g.P[1] && g.P[1] == g.P[2];
-#define PB_INTERP(PP) hipLaunchKernelGGL((k_interp<T, PP>), gd, bd, 0, s, nt, nblk8, tp, pm, g, U4, u, acc)
+#define PB_INTERP(PP) hipLaunchKernelGGL((k_interp<T, PP>), gd, bd, 0, s, nt, nblk8, tp, pm, g, U3, u, acc)
     switch (same ? g.P[0] : -1) {
     case 4: PB_INTERP(4); break;
     case 6: PB_INTERP(6); break;
@@ -551,7 +557,7 @@ void launch_interp(hipStream_t s, int nt, const pb_kgrid &g, const void *tpos, c
     case 16: PB_INTERP(16); break;
     case 20: PB_INTERP(20); break;
     case 24: PB_INTERP(24); break;
-    default: hipLaunchKernelGGL(k_interp_any<T>, gd1, bd1, 0, s, nt, tp, pm, g, U4, u, acc); break;
+    default: hipLaunchKernelGGL(k_interp_any<T>, gd1, bd1, 0, s, nt, tp, pm, g, U3, u, acc); break;
     }
 #undef PB_INTERP
 }
@@ -561,10 +567,13 @@ void launch_interp(hipStream_t s, int nt, const pb_kgrid &g, const void *tpos, c
 // conflicts) are transformed by Stockham stages of radix 8, 4, 2 (largest
 // first).  A stage reads all its butterfly inputs into registers, waits for
 // the workgroup, and writes the outputs in place, so one LDS buffer suffices.
-// Element budget per workgroup: E complex values.
-template <typename T> struct kbudget;
-template <> struct kbudget<float> { static constexpr int E = 4096; };
-template <> struct kbudget<double> { static constexpr int E = 2048; };
+// Element budget per workgroup: E complex values
+// (FP32 per pass, measured on MI355X at 128^3: z passes 2048, y 4096, x with
+// the multiply 3072; FP64 2048 throughout.)
+template <typename T, int PASS> struct pbudget { static constexpr int E = 2048; };
+template <> struct pbudget<float, 0> { static constexpr int E = 2048; };
+template <> struct pbudget<float, 1> { static constexpr int E = 4096; };
+template <> struct pbudget<float, 2> { static constexpr int E = 3072; };
 
 template <typename C> __device__ __forceinline__ C cmul(C a, C b)
 {
@@ -709,7 +718,7 @@ __global__ __launch_bounds__(256) void k_fft_z_r2c(const typename cplx<T>::type4
 {
     using C = typename cplx<T>::type;
     constexpr int NP = N + 1, H = N / 2 + 1;
-    constexpr int PP = kbudget<T>::E / (3 * N) > 0 ? kbudget<T>::E / (3 * N) : 1;
+    constexpr int PP = pbudget<T, 0>::E / (3 * N) > 0 ? pbudget<T, 0>::E / (3 * N) : 1;
     constexpr int E = 3 * PP * N;
     __shared__ C buf[3 * PP * NP];
     __shared__ C tw[N];
@@ -773,7 +782,7 @@ __global__ __launch_bounds__(256) void k_fft_y(typename cplx<T>::type *__restric
 {
     using C = typename cplx<T>::type;
     constexpr int NP = N + 1;
-    constexpr int B = kbudget<T>::E / N > 0 ? kbudget<T>::E / N : 1;
+    constexpr int B = pbudget<T, 1>::E / N > 0 ? pbudget<T, 1>::E / N : 1;
     constexpr int E = B * N;
     __shared__ C buf[B * NP];
     __shared__ C tw[N];
@@ -828,7 +837,7 @@ __global__ __launch_bounds__(256) void k_fft_x_mult(typename cplx<T>::type *__re
 {
     using C = typename cplx<T>::type;
     constexpr int NP = N + 1;
-    constexpr int B = kbudget<T>::E / (3 * N) > 0 ? kbudget<T>::E / (3 * N) : 1;
+    constexpr int B = pbudget<T, 2>::E / (3 * N) > 0 ? pbudget<T, 2>::E / (3 * N) : 1;
     constexpr int E = 3 * B * N;
     __shared__ C buf[3 * B * NP];
     __shared__ C tw[N];
@@ -899,12 +908,12 @@ __global__ __launch_bounds__(256) void k_fft_x_mult(typename cplx<T>::type *__re
 // as ux_a + i uy_a, ux_b + i uy_b, uz_a + i uz_b into the real grid (T4).
 template <typename T, int N>
 __global__ __launch_bounds__(256) void k_fft_z_c2r(const typename cplx<T>::type *__restrict__ S,
-                                                   typename cplx<T>::type4 *__restrict__ U4, int n0,
+                                                   trip<T> *__restrict__ U3, int n0,
                                                    int n1, const typename cplx<T>::type *__restrict__ gtw)
 {
     using C = typename cplx<T>::type;
     constexpr int NP = N + 1, H = N / 2 + 1;
-    constexpr int PP = kbudget<T>::E / (3 * N) > 0 ? kbudget<T>::E / (3 * N) : 1;
+    constexpr int PP = pbudget<T, 0>::E / (3 * N) > 0 ? pbudget<T, 0>::E / (3 * N) : 1;
     constexpr int E = 3 * PP * N;
     __shared__ C buf[3 * PP * NP];
     __shared__ C tw[N];
@@ -958,12 +967,7 @@ __global__ __launch_bounds__(256) void k_fft_z_c2r(const typename cplx<T>::type 
         const int pr = pair0 + pp;
         if (pr >= npair) continue;
         const C a = buf[(3 * pp + side) * NP + z], c3 = buf[(3 * pp + 2) * NP + z];
-        typename cplx<T>::type4 o;
-        o.x = a.x;
-        o.y = a.y;
-        o.z = side == 0 ? c3.x : c3.y;
-        o.w = (T)0;
-        U4[(size_t)(2 * pr + side) * N + z] = o;
+        U3[(size_t)(2 * pr + side) * N + z] = trip<T>{a.x, a.y, side == 0 ? c3.x : c3.y};
     }
 }
 
@@ -976,22 +980,22 @@ void launch_axis(hipStream_t s, int pass, const pb_kgrid &g, void *grid4, void *
     const int h2 = g.n[2] / 2 + 1;
     const C *t = (const C *)tw;
     if (pass == 0 || pass == 4) { // z
-        constexpr int PP = kbudget<T>::E / (3 * N) > 0 ? kbudget<T>::E / (3 * N) : 1;
+        constexpr int PP = pbudget<T, 0>::E / (3 * N) > 0 ? pbudget<T, 0>::E / (3 * N) : 1;
         const int npair = g.n[0] * (g.n[1] / 2);
         dim3 gd((npair + PP - 1) / PP);
         if (pass == 0)
             hipLaunchKernelGGL((k_fft_z_r2c<T, N>), gd, dim3(256), 0, s, (const C4 *)grid4, (C *)spec,
                                g.n[0], g.n[1], t);
         else
-            hipLaunchKernelGGL((k_fft_z_c2r<T, N>), gd, dim3(256), 0, s, (const C *)spec, (C4 *)grid4,
+            hipLaunchKernelGGL((k_fft_z_c2r<T, N>), gd, dim3(256), 0, s, (const C *)spec, (trip<T> *)grid4,
                                g.n[0], g.n[1], t);
     } else if (pass == 1 || pass == 3) { // y
-        constexpr int B = kbudget<T>::E / N > 0 ? kbudget<T>::E / N : 1;
+        constexpr int B = pbudget<T, 1>::E / N > 0 ? pbudget<T, 1>::E / N : 1;
         const long long npen = 3LL * g.n[0] * h2;
         hipLaunchKernelGGL((k_fft_y<T, N>), dim3((unsigned)((npen + B - 1) / B)), dim3(256), 0, s, (C *)spec,
                            g.n[0], h2, pass == 3 ? 1 : 0, t);
     } else { // x with the multiply
-        constexpr int B = kbudget<T>::E / (3 * N) > 0 ? kbudget<T>::E / (3 * N) : 1;
+        constexpr int B = pbudget<T, 2>::E / (3 * N) > 0 ? pbudget<T, 2>::E / (3 * N) : 1;
         const long long npen = (long long)g.n[1] * h2;
         hipLaunchKernelGGL((k_fft_x_mult<T, N>), dim3((unsigned)((npen + B - 1) / B)), dim3(256), 0, s,
                            (C *)spec, g.n[1], h2, km, t);
@@ -1062,8 +1066,8 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
             return pb_fail(c, PB_ERR_INVALID, "window points P must be in [1, min(32, grid)]");
     }
     const size_t h2 = (size_t)g.n[2] / 2 + 1;
-    // real grid: (fx, fy, fz, 0) per point after the spread, (ux, uy, uz, 0)
-    // after the inverse transform
+    // real grid: (fx, fy, fz, 0) per point after the spread, packed
+    // (ux, uy, uz) after the inverse transform
     if ((st = pb_reserve(c, &c->grid, (size_t)g.ng * sizeof(C4)))) return st;
     if ((st = pb_reserve(c, &c->spec, 3 * (size_t)g.n[0] * g.n[1] * h2 * sizeof(C)))) return st;
     if ((st = upload_twiddles<T>(c, s))) return st;
@@ -1092,7 +1096,7 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
         if ((st = fft_pass<T>(s, pass, g, grid4, spec, km, tws))) return pb_fail(c, st, "FFT size");
     }
     pb_mark(c, s, 5);
-    if (nt > 0) launch_interp<T>(s, nt, g, c->tpos.ptr, c->tperm.ptr, (const C4 *)grid4, u, accumulate ? 1 : 0);
+    if (nt > 0) launch_interp<T>(s, nt, g, c->tpos.ptr, c->tperm.ptr, (const trip<T> *)grid4, u, accumulate ? 1 : 0);
     if (c->timing != 2) pb_mark(c, s, 6); // (mode 2: the pair kernel records boundary 6 itself)
     return pb_hip_check(c, hipGetLastError(), "k-space launch");
 }
