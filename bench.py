@@ -28,7 +28,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 NS = NT = 100_000
-ALPHA, EPS, GRID, P, TOL = 10.0, 1e-3, (128, 128, 128), 12, 1e-6
+ALPHA, EPS, GRID, P = 10.0, 1e-3, (128, 128, 128), 12
+# The accuracy requirement is rel. L2 <= 1e-6 against the FP64 oracle.  The
+# tuner's error model (pb_tune: rc = d sqrt(ln(1/tol) + 1), d from the grid's
+# Nyquist and window) is conservative here: tol 1e-6 gives 2.3e-7 (FP64 form)
+# and 3.1e-7 (mixed), tol 2e-6 gives 3.5e-7 and 4.1e-7 with an rc 4% smaller.
+# The benchmark takes tol 2e-6; the measured errors are reported and checked.
+ACCURACY, TOL = 1e-6, 2e-6
 # Algorithmic work of the pair kernel: 200 FLOP per (target, source image)
 # inside rc -- about 80 arithmetic operations (geometry, radial functions,
 # 3x3 apply) plus six exp/erfc at 20 FLOP each.
@@ -49,8 +55,10 @@ def relL2(a, b):
 def timed_steps(ctx, p, xs, fs, ys, u, steps, warmup):
     """ms per step over `steps` timed steps, the pair kernel's ms per launch
     from HIP events around it in those same steps (two events per step: each
-    event costs the stream about 6 us), and the phase breakdown from a few
-    further steps with every phase boundary recorded."""
+    event costs the stream about 6 us; the pair kernel runs concurrently with
+    the k-space pipeline there), and the phase breakdown from a few further
+    steps with every phase boundary recorded (phases run one after another
+    then: "pair_alone" is the pair kernel without the concurrent k space)."""
     import torch
     for _ in range(warmup):
         ctx.velocity(p, xs, fs, ys, u)
@@ -67,6 +75,7 @@ def timed_steps(ctx, p, xs, fs, ys, u, steps, warmup):
         ctx.velocity(p, xs, fs, ys, u)
     phases = ctx.last_timings()
     ctx.set_timing(False)
+    phases["pair_alone"] = phases["pair"]
     phases["pair"] = pair_ms
     return (t1 - t0) * 1e3 / steps, phases
 
@@ -216,6 +225,8 @@ def main():
                    "ns": NS, "nt": NT, "d": p.d, "rc": p.rc, "grid": list(GRID), "P": P,
                    "m": p.m},
         "rel_l2_mixed": err_mixed,
+        "meets_accuracy": bool(err_mixed <= ACCURACY),
+        "tuner_tol": TOL,
         "rel_l2_fp64": err_fp64,
         "reference_split_check": ref_split,
         "rel_l2_plain_fp64": err_plain,
@@ -239,8 +250,11 @@ def main():
                              "(MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s); "
                              "%d FLOP per pair-image inside rc, %d pair-images per launch, "
                              "%.4f ms per launch (HIP events on the launch stream around each "
-                             "launch of the timed steps); "
-                             "FP64 pair fraction %.3f" % (FLOP_PER_PAIR, pairs, ph_mixed["pair"], f64)},
+                             "launch of the timed steps, where it runs concurrently with the "
+                             "k-space pipeline on another stream; alone it takes %.4f ms, "
+                             "frac %.3f); FP64 pair fraction %.3f"
+                             % (FLOP_PER_PAIR, pairs, ph_mixed["pair"], ph_mixed["pair_alone"],
+                                FLOP_PER_PAIR * pairs / (ph_mixed["pair_alone"] / 1e3) / 1e12 / peak, f64)},
     }
     if not args.no_cpu:
         log("cpu baseline")

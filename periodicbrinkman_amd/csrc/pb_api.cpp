@@ -56,6 +56,11 @@ void pb_mark(pb_context *c, hipStream_t s, int i)
     c->recorded[c->cur] |= 1u << i;
 }
 
+void pb_mark_end(pb_context *c, hipStream_t s)
+{
+    if (c && c->timing && c->ev_ready && !(c->recorded[c->cur] & (1u << PB_NPHASE))) pb_mark(c, s, PB_NPHASE);
+}
+
 namespace {
 
 // Fold a finished slot's phase durations into the running sums.
@@ -67,7 +72,8 @@ int timing_collect(pb_context *c, int slot)
     c->slot_used[slot] = 0;
     const unsigned m = c->recorded[slot];
     if (!(m & (1u << PB_NPHASE))) return PB_OK; // the call stopped early: no sample
-    if (hipEventSynchronize(c->ev[slot][PB_NPHASE]) != hipSuccess) return PB_ERR_HIP;
+    for (int i = 0; i <= PB_NPHASE; i++) // (events of one call can sit on two streams)
+        if (((m >> i) & 1u) && hipEventSynchronize(c->ev[slot][i]) != hipSuccess) return PB_ERR_HIP;
     for (int i = 0; i < PB_NPHASE; i++) {
         float t = 0;
         if (((m >> i) & 3u) == 3u && hipEventElapsedTime(&t, c->ev[slot][i], c->ev[slot][i + 1]) != hipSuccess)
@@ -263,7 +269,7 @@ bool far32_of(const pb_params *p, size_t ntab)
 // on the device (kp, ntab): binning, k space (overwrite u), real space (add).
 int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_kparams *kp, size_t ntab,
                      int ns, const double *xn, const double *f, int nt, const double *y, double *u, bool do_real,
-                     bool do_k)
+                     bool do_k, bool overlap)
 {
     int st;
     pb_box box;
@@ -272,12 +278,37 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
     if ((st = make_cells(c, p, &cells))) return st;
     if ((st = pb_bin_all(c, s, cells, box, ns, xn, f, nt, y))) return st;
     pb_mark(c, s, 1);
+    // Both halves: the pair kernel on a stream of its own, concurrently with
+    // the k-space pipeline (each adds to a zeroed u with atomics; the pair
+    // kernel's latency-bound waves and the k-space kernels' tails and launch
+    // gaps fill each other's idle compute units).  Not while every phase is
+    // timed (the phases would overlap).
+    overlap = overlap && do_real && do_k && nt > 0 && c->timing != 1;
+    if (overlap) {
+        if (!c->pair_stream &&
+            (st = pb_hip_check(c, hipStreamCreateWithFlags(&c->pair_stream, hipStreamNonBlocking), "stream")))
+            return st;
+        for (int k = 0; k < 2; k++)
+            if (!c->pair_event[k] &&
+                (st = pb_hip_check(c, hipEventCreateWithFlags(&c->pair_event[k], hipEventDisableTiming), "event")))
+                return st;
+        pb_zero_u(s, nt, u);
+        if ((st = pb_hip_check(c, hipEventRecord(c->pair_event[0], s), "fork"))) return st;
+        if ((st = pb_hip_check(c, hipStreamWaitEvent(c->pair_stream, c->pair_event[0], 0), "fork"))) return st;
+        if ((st = pb_run_real(c, c->pair_stream, cells, kp, ntab, table_degree(p), box, ns, nt, u, true,
+                              far32_of(p, ntab))))
+            return st;
+        if ((st = pb_hip_check(c, hipEventRecord(c->pair_event[1], c->pair_stream), "join"))) return st;
+        pb_kgrid g;
+        if ((st = make_kgrid(c, p, &g))) return st;
+        if ((st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u, 2, p->precision))) return st;
+        return pb_hip_check(c, hipStreamWaitEvent(s, c->pair_event[1], 0), "join");
+    }
     bool acc = false;
     if (do_k) {
         pb_kgrid g;
         if ((st = make_kgrid(c, p, &g))) return st;
-        if ((st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u, false, p->precision)))
-            return st;
+        if ((st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u, 0, p->precision))) return st;
         acc = true;
     }
     if (do_real &&
@@ -301,8 +332,8 @@ int velocity_impl(pb_context *c, const pb_params *p, int ns, const double *xn, c
     size_t ntab = 0;
     if (do_real && (st = device_kparams(c, s, p, std::fmax(3 * p->d, 8 * p->eps), &kp, &ntab))) return st;
     if ((st = timing_begin(c, s))) return pb_fail(c, st, "timing events");
-    if ((st = enqueue_velocity(c, s, p, kp, ntab, ns, xn, f, nt, y, u, do_real, do_k))) return st;
-    pb_mark(c, s, PB_NPHASE);
+    if ((st = enqueue_velocity(c, s, p, kp, ntab, ns, xn, f, nt, y, u, do_real, do_k, true))) return st;
+    pb_mark_end(c, s);
     return PB_OK;
 }
 
@@ -419,6 +450,9 @@ int pb_destroy(pb_context *c)
         if (c->sub_event[k]) (void)hipEventDestroy(c->sub_event[k]);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
+    if (c->pair_stream) (void)hipStreamDestroy(c->pair_stream);
+    for (int k = 0; k < 2; k++)
+        if (c->pair_event[k]) (void)hipEventDestroy(c->pair_event[k]);
     pb_buffer *bufs[] = {&c->cell_count, &c->cell_start, &c->cell_of_s, &c->rank_s, &c->cell_of_t,
                          &c->rank_t, &c->tcell_count, &c->tcell_start, &c->spos, &c->sfrc, &c->tpos,
                          &c->tperm, &c->counters, &c->scan_tmp, &c->qcount, &c->qoff, &c->tiles, &c->grid, &c->spec,
@@ -509,7 +543,7 @@ int pb_velocity_multirhs(pb_context *c, const pb_params *p, int ns, const double
     if ((st = make_kgrid(c, p, &g))) return st;
     for (int r = 0; r < R; r++) {
         if ((st = pb_force_set(c, s, ns, r, RP))) return st;
-        if ((st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u + (size_t)r * 3 * nt, false,
+        if ((st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u + (size_t)r * 3 * nt, 0,
                                 PB_PREC_MIXED)))
             return st;
     }
@@ -697,7 +731,7 @@ int pb_velocity_batched(pb_context *c, int nsys, pb_params *params, const int *s
         pb_context *sc = c->sub[lane_of[i]];
         if ((est = enqueue_velocity(sc, c->sub_stream[lane_of[i]], &params[i], kpd + i, c->batch_ntab[i], ns,
                                     xn + 3 * (size_t)src_off[i], f + 3 * (size_t)src_off[i], nt,
-                                    y + 3 * (size_t)tgt_off[i], u + 3 * (size_t)tgt_off[i], true, true))) {
+                                    y + 3 * (size_t)tgt_off[i], u + 3 * (size_t)tgt_off[i], true, true, false))) {
             est = pb_fail(c, est, sc->err);
             break;
         }
@@ -917,7 +951,7 @@ int fftBrinkman(double *U, const double *y, int m, const double *f, const double
     pb_kgrid g;
     if ((st = make_kgrid(c, &p, &g))) return st;
     for (int i = 0; i < 3; i++) g.P[i] = P[i];
-    if ((st = pb_run_kspace(c, nullptr, g, cells, box, alpha, d, N, m, du, false, PB_PREC_FP64))) return st;
+    if ((st = pb_run_kspace(c, nullptr, g, cells, box, alpha, d, N, m, du, 0, PB_PREC_FP64))) return st;
     if (hipDeviceSynchronize() != hipSuccess) return PB_ERR_HIP;
     return copy(U, du, 3 * (size_t)m * sizeof(double), hipMemcpyDeviceToHost);
 }

@@ -149,6 +149,10 @@ struct pb_context {
     hipStream_t sub_stream[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t sub_event[7] = {};
     hipStream_t cap_stream = nullptr; // origin of the batched call's fan-out and graph
+    // a single system's pair kernel runs on this stream, concurrently with its
+    // k-space pipeline on the caller's stream (fork and join by events)
+    hipStream_t pair_stream = nullptr;
+    hipEvent_t pair_event[2] = {};
     // the batched call's captured graph and the calls it replays
     hipGraphExec_t graph_exec = nullptr;
     std::vector<double> graph_key, last_batch_key;
@@ -176,6 +180,10 @@ struct pb_context {
 
 // Record the boundary event of phase i (0 = start) when timing is on.
 void pb_mark(pb_context *c, hipStream_t s, int i);
+// Record the end boundary (PB_NPHASE) unless the pair kernel already did.
+void pb_mark_end(pb_context *c, hipStream_t s);
+// Zero 3 n doubles of u on stream s.
+void pb_zero_u(hipStream_t s, int n, double *u);
 
 int pb_reserve(pb_context *c, pb_buffer *b, size_t bytes);
 int pb_fail(pb_context *c, int status, const std::string &msg);
@@ -226,8 +234,10 @@ __device__ __forceinline__ int pb_xcd_block(int b, int nblk8)
 int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp_dev,
                 size_t table_doubles, int table_deg, const pb_box &box, int ns, int nt, double *u, bool accumulate,
                 bool far32);
+// accumulate: 0 overwrite u, 1 add to u, 2 add with atomics (another kernel
+// may be adding to u at the same time)
 int pb_run_kspace(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells,
-                  const pb_box &box, double a, double d, int ns, int nt, double *u, bool accumulate,
+                  const pb_box &box, double a, double d, int ns, int nt, double *u, int accumulate,
                   int precision);
 int pb_run_dense(hipStream_t s, double *dA, int lda, const double *y, int nt, const double *x,
                  int ns, const pb_kparams &kp, const pb_box &box, const int *lo_real,

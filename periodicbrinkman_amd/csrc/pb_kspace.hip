@@ -479,7 +479,11 @@ __global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, int nblk8,
     if (lane == 0) {
         const double sc = g.scale * g.vol;
         const size_t o = 3 * (size_t)tperm[i];
-        if (accumulate) {
+        if (accumulate == 2) { // the pair kernel may be adding to u at the same time
+            atomicAdd(&u[o], sc * (double)a0);
+            atomicAdd(&u[o + 1], sc * (double)a1);
+            atomicAdd(&u[o + 2], sc * (double)a2);
+        } else if (accumulate) {
             u[o] += sc * (double)a0;
             u[o + 1] += sc * (double)a1;
             u[o + 2] += sc * (double)a2;
@@ -526,7 +530,11 @@ __global__ __launch_bounds__(256) void k_interp_any(int nt, const double4 *__res
     }
     const double sc = g.scale * g.vol;
     const size_t o = 3 * (size_t)tperm[i];
-    if (accumulate) {
+    if (accumulate == 2) {
+        atomicAdd(&u[o], sc * a0);
+        atomicAdd(&u[o + 1], sc * a1);
+        atomicAdd(&u[o + 2], sc * a2);
+    } else if (accumulate) {
         u[o] += sc * a0;
         u[o + 1] += sc * a1;
         u[o + 2] += sc * a2;
@@ -1053,7 +1061,7 @@ int upload_twiddles(pb_context *c, hipStream_t s)
 
 template <typename T>
 int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells,
-                 const pb_box &box, double a, double d, int ns, int nt, double *u, bool accumulate)
+                 const pb_box &box, double a, double d, int ns, int nt, double *u, int accumulate)
 {
     using C = typename cplx<T>::type;
     using C4 = typename cplx<T>::type4;
@@ -1096,7 +1104,7 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
         if ((st = fft_pass<T>(s, pass, g, grid4, spec, km, tws))) return pb_fail(c, st, "FFT size");
     }
     pb_mark(c, s, 5);
-    if (nt > 0) launch_interp<T>(s, nt, g, c->tpos.ptr, c->tperm.ptr, (const trip<T> *)grid4, u, accumulate ? 1 : 0);
+    if (nt > 0) launch_interp<T>(s, nt, g, c->tpos.ptr, c->tperm.ptr, (const trip<T> *)grid4, u, accumulate);
     if (c->timing != 2) pb_mark(c, s, 6); // (mode 2: the pair kernel records boundary 6 itself)
     return pb_hip_check(c, hipGetLastError(), "k-space launch");
 }
@@ -1104,7 +1112,7 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
 } // namespace
 
 int pb_run_kspace(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells,
-                  const pb_box &box, double a, double d, int ns, int nt, double *u, bool accumulate,
+                  const pb_box &box, double a, double d, int ns, int nt, double *u, int accumulate,
                   int precision)
 {
     if (precision == PB_PREC_MIXED)

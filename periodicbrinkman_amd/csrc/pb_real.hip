@@ -472,6 +472,13 @@ __global__ __launch_bounds__(64 * kWaves) void k_pair_direct(PB_PAIR_PARAMS)
 
 } // namespace
 
+void pb_zero_u(hipStream_t s, int n, double *u)
+{
+    if (n <= 0) return;
+    const long long n3 = 3LL * n;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, n3, u);
+}
+
 int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp,
                 size_t table_doubles, int table_deg, const pb_box &box, int ns, int nt, double *u, bool accumulate,
                 bool far32)
@@ -480,16 +487,17 @@ int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kp
     int st;
     c->stat_items = 0;
     if (nt <= 0) return PB_OK;
-    if (!accumulate) {
-        const long long n3 = 3LL * nt;
-        hipLaunchKernelGGL(k_zero, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, n3, u);
-    }
+    if (!accumulate) pb_zero_u(s, nt, u);
     const int max_tiles = c->max_tiles; // tiles built with the target binning
     const int *ntiles = (const int *)c->qoff.ptr + cells.nkey_t / 16;
     // wavefronts per tile: about two rounds of resident wavefronts (16 per CU)
     // in total; each extra wavefront repeats the tile set-up
     const double tiles_est = std::max(1.0, nt / 48.0);
-    const int nsplit = (int)std::min(16.0, std::max(1.0, std::round(2.25 * 16 * c->cu_count / tiles_est)));
+#ifndef PB_SPLIT_ROUNDS
+#define PB_SPLIT_ROUNDS 2.25
+#endif
+    const int nsplit =
+        (int)std::min(16.0, std::max(1.0, std::round(PB_SPLIT_ROUNDS * 16 * c->cu_count / tiles_est)));
     const unsigned blocks = (unsigned)(((long long)max_tiles * nsplit + kWaves - 1) / kWaves);
     if ((st = pb_reserve(c, &c->counters, sizeof(uint4) * (size_t)blocks * kWaves))) return st;
     c->stat_items = (long long)blocks * kWaves;
@@ -511,6 +519,7 @@ int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kp
     else if (ntab && lo) hipLaunchKernelGGL((k_pair_table<false, PB_TAB_DEG_MIXED>), gd, bd, lds, s, PB_PAIR_ARGS);
     else if (ntab) hipLaunchKernelGGL((k_pair_table<false, PB_TAB_DEG>), gd, bd, lds, s, PB_PAIR_ARGS);
     else hipLaunchKernelGGL((k_pair_direct<false>), gd, bd, 0, s, PB_PAIR_ARGS);
+    pb_mark(c, s, PB_NPHASE); // the pair interval ends with the launch, on its own stream
 #undef PB_PAIR_ARGS
     return pb_hip_check(c, hipGetLastError(), "pair kernel launch");
 }
