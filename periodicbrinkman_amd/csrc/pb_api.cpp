@@ -195,7 +195,8 @@ const pb_kparams &cached_kparams(double a, double d, double e, double rfar)
 // pointers of kp are left for the caller.
 // Degree of the radial table for a precision: the FP64 form keeps the table
 // at FP64 rounding (4e-15), the mixed form needs 1e-10 and takes degree 6
-// (112 bytes of coefficients per pair instead of 208, 14 FMAs instead of 26).
+// (112 bytes of coefficients per pair instead of 208, 14 FMAs instead of 26)
+// where that fits the LDS budget (large alpha d needs short intervals), else 12.
 int table_degree(const pb_params *p)
 {
     return p->precision == PB_PREC_MIXED ? PB_TAB_DEG_MIXED : PB_TAB_DEG;
@@ -210,8 +211,9 @@ void host_kparams(const pb_params *p, double rfar, pb_kparams *kp, std::vector<d
     coef.clear();
     if (p->kernel != PB_KERNEL_DIRECT && p->rc > kp->rcommon) {
         const bool mixed = p->precision == PB_PREC_MIXED;
-        pb_build_table(kp, kp->rcommon / p->d, p->rc / p->d * (1 + 1e-9), coef, table_degree(p),
-                       mixed ? 1e-10 : 4e-15);
+        const double z0 = kp->rcommon / p->d, z1 = p->rc / p->d * (1 + 1e-9);
+        pb_build_table(kp, z0, z1, coef, table_degree(p), mixed ? 1e-10 : 4e-15);
+        if (kp->tn == 0 && mixed) pb_build_table(kp, z0, z1, coef, PB_TAB_DEG, 1e-10);
         if (kp->tn == 0) coef.clear();
     }
 }
@@ -245,6 +247,7 @@ int device_kparams(pb_context *c, hipStream_t s, const pb_params *p, double rfar
                 return st;
             c->table_doubles = n;
         }
+        c->table_deg = kp.tdeg;
         if ((st = pb_hip_check(c, hipMemcpyAsync(c->kparams.ptr, &kp, sizeof(kp), hipMemcpyHostToDevice, s),
                                "kparams upload")))
             return st;
@@ -268,8 +271,8 @@ bool far32_of(const pb_params *p, size_t ntab)
 // The device work of one evaluation on stream s with kernel constants already
 // on the device (kp, ntab): binning, k space (overwrite u), real space (add).
 int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_kparams *kp, size_t ntab,
-                     int ns, const double *xn, const double *f, int nt, const double *y, double *u, bool do_real,
-                     bool do_k, bool overlap)
+                     int tdeg, int ns, const double *xn, const double *f, int nt, const double *y, double *u,
+                     bool do_real, bool do_k, bool overlap)
 {
     int st;
     pb_box box;
@@ -295,7 +298,7 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
         pb_zero_u(s, nt, u);
         if ((st = pb_hip_check(c, hipEventRecord(c->pair_event[0], s), "fork"))) return st;
         if ((st = pb_hip_check(c, hipStreamWaitEvent(c->pair_stream, c->pair_event[0], 0), "fork"))) return st;
-        if ((st = pb_run_real(c, c->pair_stream, cells, kp, ntab, table_degree(p), box, ns, nt, u, true,
+        if ((st = pb_run_real(c, c->pair_stream, cells, kp, ntab, tdeg, box, ns, nt, u, true,
                               far32_of(p, ntab))))
             return st;
         if ((st = pb_hip_check(c, hipEventRecord(c->pair_event[1], c->pair_stream), "join"))) return st;
@@ -312,7 +315,7 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
         acc = true;
     }
     if (do_real &&
-        (st = pb_run_real(c, s, cells, kp, ntab, table_degree(p), box, ns, nt, u, acc, far32_of(p, ntab))))
+        (st = pb_run_real(c, s, cells, kp, ntab, tdeg, box, ns, nt, u, acc, far32_of(p, ntab))))
         return st;
     return PB_OK;
 }
@@ -332,7 +335,8 @@ int velocity_impl(pb_context *c, const pb_params *p, int ns, const double *xn, c
     size_t ntab = 0;
     if (do_real && (st = device_kparams(c, s, p, std::fmax(3 * p->d, 8 * p->eps), &kp, &ntab))) return st;
     if ((st = timing_begin(c, s))) return pb_fail(c, st, "timing events");
-    if ((st = enqueue_velocity(c, s, p, kp, ntab, ns, xn, f, nt, y, u, do_real, do_k, true))) return st;
+    if ((st = enqueue_velocity(c, s, p, kp, ntab, c->table_deg, ns, xn, f, nt, y, u, do_real, do_k, true)))
+        return st;
     pb_mark_end(c, s);
     return PB_OK;
 }
@@ -553,7 +557,7 @@ int pb_velocity_multirhs(pb_context *c, const pb_params *p, int ns, const double
     const pb_kparams *kp = nullptr;
     size_t ntab = 0;
     if ((st = device_kparams(c, s, &pt, rfar, &kp, &ntab))) return st;
-    return pb_run_mrhs_real(c, s, cells, kp, ntab, table_degree(&pt), box, ns, nt, R, RP, u);
+    return pb_run_mrhs_real(c, s, cells, kp, ntab, c->table_deg, box, ns, nt, R, RP, u);
 }
 
 int pb_velocity_batched(pb_context *c, int nsys, pb_params *params, const int *src_off, const double *xn,
@@ -649,7 +653,11 @@ int pb_velocity_batched(pb_context *c, int nsys, pb_params *params, const int *s
         if ((st = pb_hip_check(c, hipStreamSynchronize(s), "batch upload"))) return st;
         c->batch_key = key;
         c->batch_ntab.assign(nsys, 0);
-        for (int i = 0; i < nsys; i++) c->batch_ntab[i] = toff[i + 1] - toff[i];
+        c->batch_tdeg.assign(nsys, 0);
+        for (int i = 0; i < nsys; i++) {
+            c->batch_ntab[i] = toff[i + 1] - toff[i];
+            c->batch_tdeg[i] = kps[i].tdeg;
+        }
     }
     // the systems go to 4 streams, each with its own scratch; largest first,
     // each to the least loaded stream (by grid points plus particles)
@@ -729,7 +737,8 @@ int pb_velocity_batched(pb_context *c, int nsys, pb_params *params, const int *s
     for (int i : order) {
         const int ns = src_off[i + 1] - src_off[i], nt = tgt_off[i + 1] - tgt_off[i];
         pb_context *sc = c->sub[lane_of[i]];
-        if ((est = enqueue_velocity(sc, c->sub_stream[lane_of[i]], &params[i], kpd + i, c->batch_ntab[i], ns,
+        if ((est = enqueue_velocity(sc, c->sub_stream[lane_of[i]], &params[i], kpd + i, c->batch_ntab[i],
+                                    c->batch_tdeg[i], ns,
                                     xn + 3 * (size_t)src_off[i], f + 3 * (size_t)src_off[i], nt,
                                     y + 3 * (size_t)tgt_off[i], u + 3 * (size_t)tgt_off[i], true, true, false))) {
             est = pb_fail(c, est, sc->err);

@@ -143,6 +143,7 @@ struct pb_context {
     pb_buffer batch_kp, batch_tab;
     std::vector<double> batch_key;
     std::vector<size_t> batch_ntab;
+    std::vector<int> batch_tdeg;
     // batched systems run on PB_BATCH_LANES streams of their own, each with
     // its own scratch (a sub-context), so small systems overlap on the device
     pb_context *sub[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -163,6 +164,7 @@ struct pb_context {
     pb_buffer kparams, table;
     double kp_key[7] = {-1, -1, -1, -1, -1, -1, -1};
     size_t table_doubles = 0;
+    int table_deg = PB_TAB_DEG; // degree of the table built with kparams
     // per-work-item statistics of the last real-space pass (uint4 each in
     // counters): FP64 pairs, FP32 pairs, candidates tested, candidates read
     long long stat_items = 0;
