@@ -461,7 +461,9 @@ int pb_destroy(pb_context *c)
                          &c->rank_t, &c->tcell_count, &c->tcell_start, &c->spos, &c->sfrc, &c->tpos,
                          &c->tperm, &c->counters, &c->scan_tmp, &c->qcount, &c->qoff, &c->tiles, &c->grid, &c->spec,
                          &c->sperm, &c->mrhs_f,
-                         &c->twid[0], &c->twid[1], &c->batch_kp, &c->batch_tab, &c->dense_tmp, &c->kparams, &c->table};
+                         &c->twid[0], &c->twid[1], &c->batch_kp, &c->batch_tab, &c->dense_tmp, &c->kparams, &c->table,
+                         &c->gb_int, &c->gb_d4, &c->gb_tiles, &c->gb_grid, &c->gb_spec, &c->gb_sys, &c->gb_map,
+                         &c->gb_stats};
     for (pb_buffer *b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     if (c->ev_ready)
@@ -559,6 +561,276 @@ int pb_velocity_multirhs(pb_context *c, const pb_params *p, int ns, const double
     if ((st = device_kparams(c, s, &pt, rfar, &kp, &ntab))) return st;
     return pb_run_mrhs_real(c, s, cells, kp, ntab, c->table_deg, box, ns, nt, R, RP, u);
 }
+
+namespace {
+
+// ---- grouped batched systems ------------------------------------------------
+// Systems whose binning fits the fused path (one scan workgroup) run as one
+// group: a launch per phase and class (kernel template) covers all of them,
+// each launched block finding its system and its block of that system in a
+// map, so a call of hundreds of systems is a few dozen launches that fill the
+// device instead of a dozen small launches per system.
+
+size_t pad16(size_t n) { return (n + 15) & ~(size_t)15; }
+size_t pad256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Scratch layout, descriptors, maps and launch list of the group idx; kept
+// while key (systems, parameters, arrays) is unchanged.
+int group_prepare(pb_context *c, hipStream_t s, const std::vector<int> &idx, const pb_params *params,
+                  const int *src_off, const int *tgt_off, const double *xn, const double *f, const double *y,
+                  double *u, const std::vector<double> &key)
+{
+    if (key == c->group_key) return PB_OK;
+    c->group_key.clear();
+    c->group_launches.clear();
+    int st;
+    const int n = (int)idx.size();
+    std::vector<pb_sys> sys(n);
+    std::vector<size_t> mt(n);
+    size_t ncount = 0, nint = 0, nd4 = 0, ntile = 0, ngrid = 0, nspec = 0;
+    for (int q = 0; q < n; q++) {
+        const int i = idx[q];
+        const pb_params &p = params[i];
+        pb_sys &S = sys[q];
+        std::memset(&S, 0, sizeof S);
+        pb_fill_box(&S.box, p.L);
+        if ((st = make_cells(c, &p, &S.cells))) return st;
+        if ((st = make_kgrid(c, &p, &S.g))) return st;
+        for (int ax = 0; ax < 3; ax++) {
+            const int m = S.g.n[ax];
+            if (m < 4 || (m & (m - 1)) || m > 512)
+                return pb_fail(c, PB_ERR_UNSUPPORTED, "FFT grid sizes must be powers of two in [4, 512]");
+            if (S.g.P[ax] < 1 || S.g.P[ax] > 32 || S.g.P[ax] > m)
+                return pb_fail(c, PB_ERR_INVALID, "window points P must be in [1, min(32, grid)]");
+        }
+        S.km = pb_make_kmult(S.g, S.box, p.alpha, p.d);
+        S.ns = src_off[i + 1] - src_off[i];
+        S.nt = tgt_off[i + 1] - tgt_off[i];
+        S.xn = S.ns ? xn + 3 * (size_t)src_off[i] : nullptr;
+        S.f = S.ns && f ? f + 3 * (size_t)src_off[i] : nullptr;
+        S.y = S.nt ? y + 3 * (size_t)tgt_off[i] : nullptr;
+        S.u = S.nt ? u + 3 * (size_t)tgt_off[i] : nullptr;
+        S.kp = (const pb_kparams *)c->batch_kp.ptr + i;
+        const int nc = S.cells.ncell, nk = S.cells.nkey_t;
+        ncount += pad16(nc + 1) + pad16(nk + 1);
+        nint += pad16(nc + 1) + 3 * pad16(S.ns) + pad16(nk + 1) + 3 * pad16(S.nt) + 16;
+        nd4 += 2 * pad16(S.ns) + pad16(S.nt);
+        mt[q] = (size_t)std::min(nk / 16, S.nt) + (size_t)S.nt / 64 + 1;
+        ntile += pad16(mt[q]);
+        const size_t cb = p.precision == PB_PREC_MIXED ? 8 : 16; // complex bytes
+        ngrid += pad256((size_t)S.g.ng * 2 * cb);
+        nspec += pad256(3 * (size_t)S.g.n[0] * S.g.n[1] * (S.g.n[2] / 2 + 1) * cb);
+    }
+    if ((st = pb_reserve(c, &c->gb_int, sizeof(int) * (ncount + nint)))) return st;
+    if ((st = pb_reserve(c, &c->gb_d4, sizeof(double4) * std::max<size_t>(nd4, 1)))) return st;
+    if ((st = pb_reserve(c, &c->gb_tiles, sizeof(int2) * ntile))) return st;
+    if ((st = pb_reserve(c, &c->gb_grid, ngrid))) return st;
+    if ((st = pb_reserve(c, &c->gb_spec, nspec))) return st;
+    {
+        int *cnt = (int *)c->gb_int.ptr, *ip = cnt + ncount;
+        double4 *dp = (double4 *)c->gb_d4.ptr;
+        int2 *tp = (int2 *)c->gb_tiles.ptr;
+        char *gp = (char *)c->gb_grid.ptr, *sp = (char *)c->gb_spec.ptr;
+        for (int q = 0; q < n; q++) {
+            pb_sys &S = sys[q];
+            const int nc = S.cells.ncell, nk = S.cells.nkey_t;
+            S.scount = cnt, cnt += pad16(nc + 1);
+            S.tcount = cnt, cnt += pad16(nk + 1);
+            S.sstart = ip, ip += pad16(nc + 1);
+            S.skey = ip, ip += pad16(S.ns);
+            S.srank = ip, ip += pad16(S.ns);
+            S.sperm = ip, ip += pad16(S.ns);
+            S.tstart = ip, ip += pad16(nk + 1);
+            S.tkey = ip, ip += pad16(S.nt);
+            S.trank = ip, ip += pad16(S.nt);
+            S.tperm = ip, ip += pad16(S.nt);
+            S.ntiles = ip, ip += 16;
+            S.spos = dp, dp += pad16(S.ns);
+            S.sfrc = dp, dp += pad16(S.ns);
+            S.tpos = dp, dp += pad16(S.nt);
+            S.tiles = tp, tp += pad16(mt[q]);
+            const size_t cb = params[idx[q]].precision == PB_PREC_MIXED ? 8 : 16;
+            S.grid4 = gp, gp += pad256((size_t)S.g.ng * 2 * cb);
+            S.spec = sp, sp += pad256(3 * (size_t)S.g.n[0] * S.g.n[1] * (S.g.n[2] / 2 + 1) * cb);
+        }
+    }
+    // launches in phase order, each with its map
+    std::vector<int2> map;
+    auto add = [&](int kind, int a, int b, int cc, size_t lds, const std::vector<int2> &m) {
+        if (m.empty()) return;
+        c->group_launches.push_back({kind, a, b, cc, lds, map.size(), (int)m.size()});
+        map.insert(map.end(), m.begin(), m.end());
+    };
+    {
+        std::vector<int2> m, one;
+        for (int q = 0; q < n; q++) {
+            const long long npts = (long long)sys[q].ns + sys[q].nt;
+            for (int b = 0; (long long)b * 256 < npts; b++) m.push_back(make_int2(q, b));
+            one.push_back(make_int2(q, 0));
+        }
+        add(0, 0, 0, 0, 0, m);
+        add(1, 0, 0, 0, 0, one);
+        add(2, 0, 0, 0, 0, m);
+    }
+    // pair kernel: classes by (FP32 far field, table degree; 0 = closed forms)
+    long long items = 0;
+    {
+        std::vector<std::pair<int, int>> classes;
+        std::vector<int> cls(n, -1);
+        for (int q = 0; q < n; q++) {
+            const int i = idx[q];
+            if (sys[q].nt == 0) continue;
+            const int tdeg = c->batch_ntab[i] ? c->batch_tdeg[i] : 0;
+            const std::pair<int, int> k(far32_of(&params[i], c->batch_ntab[i]) ? 1 : 0, tdeg);
+            auto it = std::find(classes.begin(), classes.end(), k);
+            cls[q] = (int)(it - classes.begin());
+            if (it == classes.end()) classes.push_back(k);
+        }
+        std::vector<size_t> stat_off(n, 0);
+        for (int k = 0; k < (int)classes.size(); k++) {
+            double tiles_est = 0;
+            size_t lds = 0;
+            for (int q = 0; q < n; q++)
+                if (cls[q] == k) {
+                    tiles_est += std::max(1.0, sys[q].nt / 48.0);
+                    const size_t nt_ = c->batch_ntab[idx[q]];
+                    lds = std::max(lds, nt_ * (sizeof(double) + (classes[k].first ? sizeof(float) : 0)));
+                }
+            // wavefronts per tile: about two rounds of resident wavefronts in total
+            const int nsplit = (int)std::min(16.0, std::max(1.0, std::round(2.25 * 16 * c->cu_count / tiles_est)));
+            std::vector<int2> m;
+            for (int q = 0; q < n; q++)
+                if (cls[q] == k) {
+                    const long long blocks = ((long long)mt[q] * nsplit + pb_pair_waves - 1) / pb_pair_waves;
+                    stat_off[q] = (size_t)items;
+                    items += blocks * pb_pair_waves;
+                    for (long long b = 0; b < blocks; b++) m.push_back(make_int2(q, (int)b));
+                }
+            add(3, classes[k].first, classes[k].second, nsplit, lds, m);
+        }
+        if ((st = pb_reserve(c, &c->gb_stats, sizeof(uint4) * std::max<long long>(items, 1)))) return st;
+        for (int q = 0; q < n; q++) sys[q].stats = (uint4 *)c->gb_stats.ptr + stat_off[q];
+    }
+    c->group_items = items;
+    // k space: spread, the five FFT passes, gather; classes by precision
+    // (and transformed length, window)
+    auto prec_of = [&](int q) { return params[idx[q]].precision == PB_PREC_MIXED ? PB_PREC_MIXED : PB_PREC_FP64; };
+    const int precs[2] = {PB_PREC_MIXED, PB_PREC_FP64};
+    for (int pr : precs) {
+        std::vector<int2> m;
+        for (int q = 0; q < n; q++)
+            if (prec_of(q) == pr) {
+                const pb_kgrid &g = sys[q].g;
+                const int nreg = ((g.n[0] + 7) / 8) * ((g.n[1] + 7) / 8) * ((g.n[2] + 7) / 8);
+                for (int b = 0; b < nreg; b++) m.push_back(make_int2(q, b));
+            }
+        add(4, pr, 0, 0, 0, m);
+    }
+    for (int pass = 0; pass < 5; pass++) {
+        const int ax = (pass == 0 || pass == 4) ? 2 : (pass == 2 ? 0 : 1);
+        for (int pr : precs)
+            for (int len = 4; len <= 512; len *= 2) {
+                std::vector<int2> m;
+                for (int q = 0; q < n; q++)
+                    if (prec_of(q) == pr && sys[q].g.n[ax] == len) {
+                        const int blocks = pb_group_fft_blocks(pr, pass, sys[q].g);
+                        for (int b = 0; b < blocks; b++) m.push_back(make_int2(q, b));
+                    }
+                add(5, pr, pass, len, 0, m);
+            }
+    }
+    {
+        const int Ps[9] = {4, 6, 8, 10, 12, 16, 20, 24, 0};
+        for (int pr : precs)
+            for (int P : Ps) {
+                std::vector<int2> m;
+                for (int q = 0; q < n; q++) {
+                    const pb_kgrid &g = sys[q].g;
+                    const bool same = g.P[0] == g.P[1] && g.P[1] == g.P[2];
+                    const int Pq = same && std::find(Ps, Ps + 8, g.P[0]) != Ps + 8 ? g.P[0] : 0;
+                    if (prec_of(q) != pr || Pq != P || sys[q].nt == 0) continue;
+                    const int per = P ? pb_interp_waves : 256;
+                    for (int b = 0; b * per < sys[q].nt; b++) m.push_back(make_int2(q, b));
+                }
+                if (P) // the gather renumbers blocks per XCD: a multiple of 8
+                    while (m.size() % 8) m.push_back(make_int2(-1, 0));
+                add(6, pr, P, 0, 0, m);
+            }
+    }
+    c->group_counts = ncount;
+    if ((st = pb_reserve(c, &c->gb_sys, sizeof(pb_sys) * n))) return st;
+    if ((st = pb_reserve(c, &c->gb_map, sizeof(int2) * std::max<size_t>(map.size(), 1)))) return st;
+    if ((st = pb_hip_check(c, hipMemcpyAsync(c->gb_sys.ptr, sys.data(), sizeof(pb_sys) * n, hipMemcpyHostToDevice, s),
+                           "group upload")))
+        return st;
+    if (!map.empty() &&
+        (st = pb_hip_check(c, hipMemcpyAsync(c->gb_map.ptr, map.data(), sizeof(int2) * map.size(),
+                                             hipMemcpyHostToDevice, s), "group upload")))
+        return st;
+    for (int pr : precs) {
+        const void *tw;
+        bool used = false;
+        for (int q = 0; q < n; q++) used = used || prec_of(q) == pr;
+        if (used && (st = pb_twiddles(c, s, pr, &tw))) return st;
+    }
+    if ((st = pb_hip_check(c, hipStreamSynchronize(s), "group upload"))) return st;
+    c->group_key = key;
+    return PB_OK;
+}
+
+// The group's launches on stream g (pair kernels forked to c->pair_stream and
+// joined back); u of the group's targets must be zero.
+int group_enqueue(pb_context *c, hipStream_t g)
+{
+    int st;
+    const pb_sys *sys = (const pb_sys *)c->gb_sys.ptr;
+    const int2 *map = (const int2 *)c->gb_map.ptr;
+    pb_fill_zero(g, (long long)c->group_counts, (int *)c->gb_int.ptr);
+    bool forked = false;
+    for (const auto &L : c->group_launches) {
+        const int2 *m = map + L.map;
+        switch (L.kind) {
+        case 0: pb_group_bin_count(g, sys, m, L.nblk); break;
+        case 1: pb_group_bin_scan(g, sys, m, L.nblk); break;
+        case 2: pb_group_bin_scatter(g, sys, m, L.nblk); break;
+        case 3:
+            if (!forked) {
+                if (!c->pair_stream &&
+                    (st = pb_hip_check(c, hipStreamCreateWithFlags(&c->pair_stream, hipStreamNonBlocking),
+                                       "stream")))
+                    return st;
+                for (int k = 0; k < 2; k++)
+                    if (!c->pair_event[k] &&
+                        (st = pb_hip_check(c, hipEventCreateWithFlags(&c->pair_event[k], hipEventDisableTiming),
+                                           "event")))
+                        return st;
+                if ((st = pb_hip_check(c, hipEventRecord(c->pair_event[0], g), "fork"))) return st;
+                if ((st = pb_hip_check(c, hipStreamWaitEvent(c->pair_stream, c->pair_event[0], 0), "fork")))
+                    return st;
+                forked = true;
+            }
+            if ((st = pb_group_pair(c->pair_stream, sys, m, L.nblk, L.c, L.a != 0, L.b, L.lds)))
+                return pb_fail(c, st, "grouped pair kernel launch");
+            break;
+        case 4: pb_group_spread(g, sys, m, L.nblk, L.a); break;
+        case 5: {
+            const void *tw = c->twid[L.a == PB_PREC_MIXED ? 1 : 0].ptr;
+            if ((st = pb_group_fft(g, sys, m, L.nblk, L.a, L.b, L.c, tw))) return pb_fail(c, st, "FFT size");
+            break;
+        }
+        default:
+            if ((st = pb_group_interp(g, sys, m, L.nblk, L.a, L.b))) return pb_fail(c, st, "gather window");
+            break;
+        }
+    }
+    if (forked) {
+        if ((st = pb_hip_check(c, hipEventRecord(c->pair_event[1], c->pair_stream), "join"))) return st;
+        if ((st = pb_hip_check(c, hipStreamWaitEvent(g, c->pair_event[1], 0), "join"))) return st;
+    }
+    return pb_hip_check(c, hipGetLastError(), "grouped launch");
+}
+
+} // namespace
 
 int pb_velocity_batched(pb_context *c, int nsys, pb_params *params, const int *src_off, const double *xn,
                         const double *f, const int *tgt_off, const double *y, double *u, void *stream)
@@ -676,8 +948,14 @@ int pb_velocity_batched(pb_context *c, int nsys, pb_params *params, const int *s
         return (double)params[i].grid[0] * params[i].grid[1] * params[i].grid[2] + 4.0 * (src_off[i + 1] - src_off[i]) +
                4.0 * (tgt_off[i + 1] - tgt_off[i]);
     };
-    std::vector<int> order(nsys);
-    for (int i = 0; i < nsys; i++) order[i] = i;
+    // systems whose binning fits one scan workgroup form the group (one launch
+    // per phase for all of them); the others run one by one on the streams
+    std::vector<int> grouped, order;
+    for (int i = 0; i < nsys; i++) {
+        pb_cells cl;
+        if ((st = make_cells(c, &params[i], &cl))) return st;
+        (pb_bin_fused_ok(cl) ? grouped : order).push_back(i);
+    }
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
     double load[K] = {0, 0, 0, 0};
     std::vector<int> lane_of(nsys);
@@ -707,6 +985,10 @@ int pb_velocity_batched(pb_context *c, int nsys, pb_params *params, const int *s
     const double ptrs[5] = {(double)(uintptr_t)xn, (double)(uintptr_t)f, (double)(uintptr_t)y,
                             (double)(uintptr_t)u, (double)(uintptr_t)s};
     gkey.insert(gkey.end(), ptrs, ptrs + 5);
+    if (!grouped.empty() && (st = group_prepare(c, g, grouped, params, src_off, tgt_off, xn, f, y, u, gkey)))
+        return st;
+    c->stat_group = !grouped.empty();
+    c->stat_items = grouped.empty() ? 0 : c->group_items;
     if ((st = pb_hip_check(c, hipEventRecord(c->sub_event[K], s), "order"))) return st;
     if ((st = pb_hip_check(c, hipStreamWaitEvent(g, c->sub_event[K], 0), "order"))) return st;
     auto finish = [&]() -> int {
@@ -727,6 +1009,8 @@ int pb_velocity_batched(pb_context *c, int nsys, pb_params *params, const int *s
     }
     if (capture && (st = pb_hip_check(c, hipStreamBeginCapture(g, hipStreamCaptureModeThreadLocal), "capture")))
         return st;
+    // the group adds into zeroed u; the systems on the streams overwrite theirs
+    if (!grouped.empty()) pb_zero_u(g, tgt_off[nsys], u);
     // fork
     if ((st = pb_hip_check(c, hipEventRecord(c->sub_event[K + 1], g), "fork"))) return st;
     for (int k = 0; k < K; k++)
@@ -745,6 +1029,7 @@ int pb_velocity_batched(pb_context *c, int nsys, pb_params *params, const int *s
             break;
         }
     }
+    if (!est && !grouped.empty()) est = group_enqueue(c, g);
     // join (also after a failure, so a capture can be closed)
     for (int k = 0; k < K; k++) {
         if ((st = pb_hip_check(c, hipEventRecord(c->sub_event[k], c->sub_stream[k]), "join"))) return st;
@@ -776,9 +1061,10 @@ int pb_last_pair_stats(pb_context *c, long long out[4])
 {
     if (!c || !out) return PB_ERR_INVALID;
     out[0] = out[1] = out[2] = out[3] = 0;
-    if (!c->counters.ptr || c->stat_items <= 0) return PB_OK;
+    const pb_buffer &b = c->stat_group ? c->gb_stats : c->counters;
+    if (!b.ptr || c->stat_items <= 0) return PB_OK;
     std::vector<unsigned> h(4 * (size_t)c->stat_items);
-    int st = pb_hip_check(c, hipMemcpy(h.data(), c->counters.ptr, h.size() * sizeof(unsigned),
+    int st = pb_hip_check(c, hipMemcpy(h.data(), b.ptr, h.size() * sizeof(unsigned),
                                        hipMemcpyDeviceToHost), "counters");
     for (size_t i = 0; i < h.size(); i++) out[i & 3] += h[i];
     return st;

@@ -189,12 +189,11 @@ __device__ __forceinline__ int cell_key(const double *__restrict__ pos, size_t i
 
 // threads [0, ns): sources (z-major cell keys); [ns, ns + nt): targets
 // (brick-Morton keys)
-__global__ void k_bin_count2(int ns, const double *__restrict__ xs, int nt, const double *__restrict__ yt,
-                             pb_box box, pb_cells cells, int *__restrict__ scount, int *__restrict__ skey,
-                             int *__restrict__ srank, int *__restrict__ tcount, int *__restrict__ tkey,
-                             int *__restrict__ trank)
+__device__ __forceinline__ void bin_count2(int i, int ns, const double *__restrict__ xs, int nt,
+                                           const double *__restrict__ yt, const pb_box &box, const pb_cells &cells,
+                                           int *__restrict__ scount, int *__restrict__ skey, int *__restrict__ srank,
+                                           int *__restrict__ tcount, int *__restrict__ tkey, int *__restrict__ trank)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < ns) {
         const int key = cell_key(xs, (size_t)i, box, cells, false);
         skey[i] = key;
@@ -207,14 +206,24 @@ __global__ void k_bin_count2(int ns, const double *__restrict__ xs, int nt, cons
     }
 }
 
-__global__ void k_bin_scatter2(int ns, const double *__restrict__ xs, const double *__restrict__ fs, int nt,
-                               const double *__restrict__ yt, pb_box box, const int *__restrict__ sstart,
-                               const int *__restrict__ skey, const int *__restrict__ srank,
-                               double4 *__restrict__ spos, double4 *__restrict__ sfrc, int *__restrict__ sperm,
-                               const int *__restrict__ tstart, const int *__restrict__ tkey,
-                               const int *__restrict__ trank, double4 *__restrict__ tpos, int *__restrict__ tperm)
+__global__ void k_bin_count2(int ns, const double *__restrict__ xs, int nt, const double *__restrict__ yt,
+                             pb_box box, pb_cells cells, int *__restrict__ scount, int *__restrict__ skey,
+                             int *__restrict__ srank, int *__restrict__ tcount, int *__restrict__ tkey,
+                             int *__restrict__ trank)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bin_count2(blockIdx.x * blockDim.x + threadIdx.x, ns, xs, nt, yt, box, cells, scount, skey, srank, tcount, tkey,
+               trank);
+}
+
+__device__ __forceinline__ void bin_scatter2(int i, int ns, const double *__restrict__ xs,
+                                             const double *__restrict__ fs, int nt, const double *__restrict__ yt,
+                                             const pb_box &box, const int *__restrict__ sstart,
+                                             const int *__restrict__ skey, const int *__restrict__ srank,
+                                             double4 *__restrict__ spos, double4 *__restrict__ sfrc,
+                                             int *__restrict__ sperm, const int *__restrict__ tstart,
+                                             const int *__restrict__ tkey, const int *__restrict__ trank,
+                                             double4 *__restrict__ tpos, int *__restrict__ tperm)
+{
     const bool src = i < ns;
     const int j = src ? i : i - ns;
     if (!src && j >= nt) return;
@@ -231,6 +240,17 @@ __global__ void k_bin_scatter2(int ns, const double *__restrict__ xs, const doub
         tpos[dst] = make_double4(w[0], w[1], w[2], 0.0);
         tperm[dst] = j;
     }
+}
+
+__global__ void k_bin_scatter2(int ns, const double *__restrict__ xs, const double *__restrict__ fs, int nt,
+                               const double *__restrict__ yt, pb_box box, const int *__restrict__ sstart,
+                               const int *__restrict__ skey, const int *__restrict__ srank,
+                               double4 *__restrict__ spos, double4 *__restrict__ sfrc, int *__restrict__ sperm,
+                               const int *__restrict__ tstart, const int *__restrict__ tkey,
+                               const int *__restrict__ trank, double4 *__restrict__ tpos, int *__restrict__ tperm)
+{
+    bin_scatter2(blockIdx.x * blockDim.x + threadIdx.x, ns, xs, fs, nt, yt, box, sstart, skey, srank, spos, sfrc,
+                 sperm, tstart, tkey, trank, tpos, tperm);
 }
 
 // Exclusive scan across the 1024 threads of the workgroup; total to every thread.
@@ -335,6 +355,37 @@ __global__ __launch_bounds__(1024) void k_bin_scan2(const int *__restrict__ scou
     scan_counts(tcount, nkey_t, tstart, wsum, tiles, ntiles);
 }
 
+
+// ---- grouped binning: every system of a batched call in one launch each ----
+__global__ __launch_bounds__(256) void k_gbin_count(const pb_sys *__restrict__ sys, const int2 *__restrict__ map)
+{
+    const int2 m = map[blockIdx.x];
+    const pb_sys &S = sys[m.x];
+    bin_count2(m.y * 256 + (int)threadIdx.x, S.ns, S.xn, S.nt, S.y, S.box, S.cells, S.scount, S.skey, S.srank,
+               S.tcount, S.tkey, S.trank);
+}
+
+__global__ __launch_bounds__(256) void k_gbin_scatter(const pb_sys *__restrict__ sys, const int2 *__restrict__ map)
+{
+    const int2 m = map[blockIdx.x];
+    const pb_sys &S = sys[m.x];
+    bin_scatter2(m.y * 256 + (int)threadIdx.x, S.ns, S.xn, S.f, S.nt, S.y, S.box, S.sstart, S.skey, S.srank, S.spos,
+                 S.f ? S.sfrc : nullptr, S.sperm, S.tstart, S.tkey, S.trank, S.tpos, S.tperm);
+}
+
+__global__ __launch_bounds__(1024) void k_gbin_scan(const pb_sys *__restrict__ sys, const int2 *__restrict__ map)
+{
+    __shared__ int wsum[16];
+    const pb_sys &S = sys[map[blockIdx.x].x];
+    scan_counts(S.scount, S.cells.ncell, S.sstart, wsum, nullptr, nullptr);
+    scan_counts(S.tcount, S.cells.nkey_t, S.tstart, wsum, S.tiles, S.ntiles);
+}
+
+__global__ void k_fill_zero(long long n, int *__restrict__ a)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
 } // namespace
 
 // Exclusive scan of n ints from in to out (out[n] = total) on stream s.
@@ -414,7 +465,7 @@ int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_
 int pb_bin_all(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_box &box, int ns, const double *xn,
                const double *f, int nt, const double *y)
 {
-    if ((long long)cells.ncell + cells.nkey_t > kFusedKeys || cells.nkey_t % 64 != 0) {
+    if (!pb_bin_fused_ok(cells)) {
         int st;
         if ((st = pb_bin_points(c, s, cells, box, ns, xn, f, true))) return st;
         return pb_bin_points(c, s, cells, box, nt, y, nullptr, false);
@@ -422,7 +473,7 @@ int pb_bin_all(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_box
     int st;
     const size_t nsn = (size_t)(ns > 0 ? ns : 1), ntn = (size_t)(nt > 0 ? nt : 1);
     const int ncell = cells.ncell, nkt = cells.nkey_t, nq = nkt / 16;
-    const size_t mt = (size_t)nq + (size_t)nt / 64 + 1;
+    const size_t mt = (size_t)std::min(nq, nt) + (size_t)nt / 64 + 1; // non-empty quarters <= nt
     if ((st = pb_reserve(c, &c->cell_count, sizeof(int) * (ncell + 1)))) return st;
     if ((st = pb_reserve(c, &c->cell_start, sizeof(int) * (ncell + 1)))) return st;
     if ((st = pb_reserve(c, &c->cell_of_s, sizeof(int) * nsn))) return st;
@@ -457,4 +508,29 @@ int pb_bin_all(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_box
                            (const int *)c->rank_t.ptr, (double4 *)c->tpos.ptr, (int *)c->tperm.ptr);
     c->max_tiles = (int)mt;
     return pb_hip_check(c, hipGetLastError(), "binning launch");
+}
+
+void pb_group_bin_count(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk)
+{
+    if (nblk > 0) hipLaunchKernelGGL(k_gbin_count, dim3(nblk), dim3(256), 0, s, sys, map);
+}
+
+void pb_group_bin_scan(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk)
+{
+    if (nblk > 0) hipLaunchKernelGGL(k_gbin_scan, dim3(nblk), dim3(1024), 0, s, sys, map);
+}
+
+void pb_group_bin_scatter(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk)
+{
+    if (nblk > 0) hipLaunchKernelGGL(k_gbin_scatter, dim3(nblk), dim3(256), 0, s, sys, map);
+}
+
+void pb_fill_zero(hipStream_t s, long long n, int *a)
+{
+    if (n > 0) hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a);
+}
+
+bool pb_bin_fused_ok(const pb_cells &cells)
+{
+    return (long long)cells.ncell + cells.nkey_t <= kFusedKeys && cells.nkey_t % 64 == 0;
 }

@@ -105,6 +105,41 @@ struct pb_kgrid {
     double vol;     // h0 h1 h2 (gather quadrature weight)
 };
 
+// Constants of the Fourier multiply of one system.
+struct pb_kmult {
+    double k[3];     // 2 pi / L per axis
+    double c;        // (1 - e^{-a^2 d^2/4}) / a^2
+    double a2;       // a^2
+    double inv_ng;   // 1 / (n0 n1 n2)
+    double beta;     // d^2/4 - eta/2: e^{-beta k^2} removes the split and the two window convolutions
+    int n[3];
+};
+pb_kmult pb_make_kmult(const pb_kgrid &g, const pb_box &box, double a, double d);
+
+// One system of a grouped batched call (device copy): its share of the
+// caller's arrays, its geometry and constants, and its share of the group's
+// scratch.  A grouped launch covers every system of a class (same kernel
+// template) in one grid; block b of the launch works on system map[b].x,
+// block map[b].y of that system's own partition.
+struct pb_sys {
+    const double *xn, *f, *y;
+    double *u;
+    int ns, nt;
+    pb_box box;
+    pb_cells cells;
+    pb_kgrid g;
+    pb_kmult km;
+    const pb_kparams *kp;
+    int *scount, *sstart, *skey, *srank, *sperm; // source binning (sstart = cell_start)
+    double4 *spos, *sfrc;
+    int *tcount, *tstart, *tkey, *trank, *tperm; // target binning
+    double4 *tpos;
+    int2 *tiles;
+    int *ntiles;
+    void *grid4, *spec; // real grid (T4 / packed trip) and half spectrum
+    uint4 *stats;       // pair-kernel statistics, one per wavefront item
+};
+
 // ---- host helpers (pb_params.cpp) ----------------------------------------
 // Build the table of diffH(d) on z in [z0, z1] with polynomials of degree deg
 // whose verified error (relative to each interval's largest |H|) is <= tol:
@@ -154,6 +189,22 @@ struct pb_context {
     // k-space pipeline on the caller's stream (fork and join by events)
     hipStream_t pair_stream = nullptr;
     hipEvent_t pair_event[2] = {};
+    // grouped batched systems: scratch of every system (ints, double4 points,
+    // tiles, grids, spectra), device descriptors and block maps, and the
+    // launch list built for the group layout in group_key
+    pb_buffer gb_int, gb_d4, gb_tiles, gb_grid, gb_spec, gb_sys, gb_map, gb_stats;
+    std::vector<double> group_key;
+    struct group_launch {
+        int kind;      // 0 count, 1 scan, 2 scatter, 3 pair, 4 spread, 5 FFT, 6 gather
+        int a, b, c;   // pair: far32, table degree, nsplit; spread: precision; FFT: precision, pass, n;
+                       // gather: precision, P
+        size_t lds;    // pair: dynamic LDS bytes
+        size_t map;    // offset of the launch's map entries
+        int nblk;
+    };
+    std::vector<group_launch> group_launches;
+    size_t group_counts = 0;    // ints to zero at the start of gb_int (the count arrays)
+    long long group_items = 0;  // pair statistics items
     // the batched call's captured graph and the calls it replays
     hipGraphExec_t graph_exec = nullptr;
     std::vector<double> graph_key, last_batch_key;
@@ -168,6 +219,7 @@ struct pb_context {
     // per-work-item statistics of the last real-space pass (uint4 each in
     // counters): FP64 pairs, FP32 pairs, candidates tested, candidates read
     long long stat_items = 0;
+    bool stat_group = false; // the last pass was a grouped batched call (statistics in gb_stats)
     // phase timing (pb_set_timing): a ring of event sets, one per call, so a
     // timed loop of up to PB_TSLOTS calls needs no host synchronization
     int timing = 0;
@@ -231,6 +283,36 @@ __device__ __forceinline__ int pb_xcd_block(int b, int nblk8)
 {
     return (b & 7) * (nblk8 >> 3) + (b >> 3);
 }
+// ---- grouped launches (one per phase and class, for many systems) ----------
+// sys: device descriptors; map: (system, block) per launched block; nblk:
+// launched blocks.  Binning: count and scatter take one block of 256 points
+// (sources then targets) per map entry, the scan one workgroup per system.
+void pb_group_bin_count(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk);
+void pb_group_bin_scan(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk);
+void pb_group_bin_scatter(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk);
+void pb_fill_zero(hipStream_t s, long long n, int *a);
+// Pair kernel of one class (far32, table degree; tdeg 0: closed forms) with
+// nsplit wavefronts per tile and lds bytes of dynamic LDS (the largest table).
+int pb_group_pair(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, int nsplit, bool far32,
+                  int tdeg, size_t lds);
+// k space of one precision (mixed: FP32 grids): spread (one block per 8^3
+// region), the FFT passes (pass 0..4 as in the single-system pipeline, for
+// systems whose transformed axis has length n), the gather (window P on
+// every axis, or P = 0 for per-axis windows; map over blocks of
+// 4 targets, XCD-renumbered: nblk a multiple of 8, entries (-1, 0) idle).
+void pb_group_spread(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, int precision);
+int pb_group_fft(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, int precision, int pass, int n,
+                 const void *twiddles);
+int pb_group_interp(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, int precision, int P);
+// blocks of one system in each grouped launch
+int pb_group_fft_blocks(int precision, int pass, const pb_kgrid &g);
+constexpr int pb_interp_waves = 4; // targets per gather block
+constexpr int pb_pair_waves = 4;   // wavefronts per pair-kernel workgroup
+// the fused binning (one scan workgroup) applies to this key space
+bool pb_bin_fused_ok(const pb_cells &cells);
+// FFT twiddles of a precision (uploaded once per context)
+int pb_twiddles(pb_context *c, hipStream_t s, int precision, const void **tw);
+
 // far32: pairs beyond rfar in FP32 (table copy when table_doubles > 0, else
 // the FP32 closed form); otherwise every pair in FP64.
 int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp_dev,

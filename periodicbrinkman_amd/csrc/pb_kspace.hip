@@ -273,16 +273,15 @@ __device__ __forceinline__ void spread_process(spread_lds<T> &S, int wave, int l
 }
 
 template <typename T>
-__global__ __launch_bounds__(64 * kSpreadWaves) void k_spread_regions(
-    const int *__restrict__ cstart, const double4 *__restrict__ spos,
-    const double4 *__restrict__ sfrc, pb_cells cells, pb_kgrid g,
+__device__ __forceinline__ void spread_region(
+    const int rid, const int *__restrict__ cstart, const double4 *__restrict__ spos,
+    const double4 *__restrict__ sfrc, const pb_cells &cells, const pb_kgrid &g,
     typename cplx<T>::type4 *__restrict__ grid4)
 {
     constexpr int CAP = spread_cfg<T>::CAP;
     __shared__ spread_lds<T> S;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int nr0 = (g.n[0] + 7) / 8, nr1 = (g.n[1] + 7) / 8, nr2 = (g.n[2] + 7) / 8;
-    const int rid = blockIdx.x;
     if (rid >= nr0 * nr1 * nr2) return; // whole workgroup, before any barrier
     const int rr[3] = {rid / (nr1 * nr2), (rid / nr2) % nr1, rid % nr2};
     int gA[3], gB[3], ulo[3], uhi[3];
@@ -410,6 +409,24 @@ __global__ __launch_bounds__(64 * kSpreadWaves) void k_spread_regions(
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(64 * kSpreadWaves) void k_spread_regions(
+    const int *__restrict__ cstart, const double4 *__restrict__ spos,
+    const double4 *__restrict__ sfrc, pb_cells cells, pb_kgrid g,
+    typename cplx<T>::type4 *__restrict__ grid4)
+{
+    spread_region<T>((int)blockIdx.x, cstart, spos, sfrc, cells, g, grid4);
+}
+
+template <typename T>
+__global__ __launch_bounds__(64 * kSpreadWaves) void k_gspread(const pb_sys *__restrict__ sys,
+                                                               const int2 *__restrict__ map)
+{
+    const int2 m = map[blockIdx.x];
+    const pb_sys &S = sys[m.x];
+    spread_region<T>(m.y, S.sstart, S.spos, S.sfrc, S.cells, S.g, (typename cplx<T>::type4 *)S.grid4);
+}
+
 // ---- interpolation (gather) --------------------------------------------------
 // u(y) = h^3 (pi eta)^{-3/2} sum_window e^{-|x_g - y|^2/eta} G(x_g), G = (ux,
 // uy, uz) per grid point.  One wavefront per target (targets in their sorted,
@@ -419,6 +436,7 @@ __global__ __launch_bounds__(64 * kSpreadWaves) void k_spread_regions(
 // x-planes (weights and offsets the same for the whole wavefront): one vector
 // load and three FMAs per point.  The partial sums meet in a wave reduction.
 constexpr int kInterpWaves = 4;
+static_assert(kInterpWaves == pb_interp_waves, "gather workgroup size");
 
 template <typename T> __device__ __forceinline__ T wave_sum(T v)
 {
@@ -428,17 +446,15 @@ template <typename T> __device__ __forceinline__ T wave_sum(T v)
 }
 
 template <typename T, int P>
-__global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, int nblk8,
-                                                            const double4 *__restrict__ tpos,
-                                                            const int *__restrict__ tperm, pb_kgrid g,
-                                                            const trip<T> *__restrict__ U3,
-                                                            double *__restrict__ u, int accumulate)
+__device__ __forceinline__ void interp_target(const int i, int nt, const double4 *__restrict__ tpos,
+                                              const int *__restrict__ tperm, const pb_kgrid &g,
+                                              const trip<T> *__restrict__ U3, double *__restrict__ u,
+                                              int accumulate)
 {
     constexpr int NQ = P * P, NR = (NQ + 63) / 64;
     __shared__ T wts[kInterpWaves][3][P];
     __shared__ int off[kInterpWaves][3][P]; // wrapped index times its stride
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = pb_xcd_block(blockIdx.x, nblk8) * kInterpWaves + wave;
     if (i >= nt) return; // whole wavefront; no block barriers below
     const double4 y = tpos[i];
     for (int k = lane; k < 3 * P; k += 64) {
@@ -495,14 +511,37 @@ __global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, int nblk8,
     }
 }
 
+template <typename T, int P>
+__global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, int nblk8,
+                                                            const double4 *__restrict__ tpos,
+                                                            const int *__restrict__ tperm, pb_kgrid g,
+                                                            const trip<T> *__restrict__ U3,
+                                                            double *__restrict__ u, int accumulate)
+{
+    interp_target<T, P>(pb_xcd_block(blockIdx.x, nblk8) * kInterpWaves + (int)(threadIdx.x >> 6), nt, tpos, tperm,
+                        g, U3, u, accumulate);
+}
+
+// Grouped: blocks renumbered per XCD before the map lookup; the gather adds
+// to u with atomics (the pair kernel may run at the same time).
+template <typename T, int P>
+__global__ __launch_bounds__(64 * kInterpWaves) void k_ginterp(const pb_sys *__restrict__ sys,
+                                                             const int2 *__restrict__ map, int nblk8)
+{
+    const int2 m = map[pb_xcd_block(blockIdx.x, nblk8)];
+    if (m.x < 0) return;
+    const pb_sys &S = sys[m.x];
+    interp_target<T, P>(m.y * kInterpWaves + (int)(threadIdx.x >> 6), S.nt, S.tpos, S.tperm, S.g,
+                        (const trip<T> *)S.grid4, S.u, 2);
+}
+
 // Per-axis window sizes that differ (fftBrinkman's P[3]) take this path.
 template <typename T>
-__global__ __launch_bounds__(256) void k_interp_any(int nt, const double4 *__restrict__ tpos,
-                                                    const int *__restrict__ tperm, pb_kgrid g,
-                                                    const trip<T> *__restrict__ U3,
-                                                    double *__restrict__ u, int accumulate)
+__device__ __forceinline__ void interp_any_target(const int i, int nt, const double4 *__restrict__ tpos,
+                                                  const int *__restrict__ tperm, const pb_kgrid &g,
+                                                  const trip<T> *__restrict__ U3, double *__restrict__ u,
+                                                  int accumulate)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
     const double4 y = tpos[i];
     const double yc[3] = {y.x, y.y, y.z};
@@ -543,6 +582,24 @@ __global__ __launch_bounds__(256) void k_interp_any(int nt, const double4 *__res
         u[o + 1] = sc * a1;
         u[o + 2] = sc * a2;
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_interp_any(int nt, const double4 *__restrict__ tpos,
+                                                    const int *__restrict__ tperm, pb_kgrid g,
+                                                    const trip<T> *__restrict__ U3,
+                                                    double *__restrict__ u, int accumulate)
+{
+    interp_any_target<T>((int)(blockIdx.x * blockDim.x + threadIdx.x), nt, tpos, tperm, g, U3, u, accumulate);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_ginterp_any(const pb_sys *__restrict__ sys, const int2 *__restrict__ map)
+{
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const pb_sys &S = sys[m.x];
+    interp_any_target<T>(m.y * 256 + (int)threadIdx.x, S.nt, S.tpos, S.tperm, S.g, (const trip<T> *)S.grid4, S.u, 2);
 }
 
 template <typename T>
@@ -720,9 +777,9 @@ __device__ __forceinline__ void load_twiddles(C *tw, const C *__restrict__ gtw)
 // pencils fx_a + i fy_a, fx_b + i fy_b, fz_a + i fz_b; the half spectra
 // (kz = 0 .. N/2) of the six real pencils are written to S.
 template <typename T, int N>
-__global__ __launch_bounds__(256) void k_fft_z_r2c(const typename cplx<T>::type4 *__restrict__ R4,
-                                                   typename cplx<T>::type *__restrict__ S, int n0,
-                                                   int n1, const typename cplx<T>::type *__restrict__ gtw)
+__device__ __forceinline__ void fft_z_r2c(const int blk, const typename cplx<T>::type4 *__restrict__ R4,
+                                          typename cplx<T>::type *__restrict__ S, int n0, int n1,
+                                          const typename cplx<T>::type *__restrict__ gtw)
 {
     using C = typename cplx<T>::type;
     constexpr int NP = N + 1, H = N / 2 + 1;
@@ -732,7 +789,7 @@ __global__ __launch_bounds__(256) void k_fft_z_r2c(const typename cplx<T>::type4
     __shared__ C tw[N];
     load_twiddles<C, N>(tw, gtw);
     const int npair = n0 * (n1 / 2);
-    const int pair0 = blockIdx.x * PP;
+    const int pair0 = blk * PP;
     for (int idx = threadIdx.x; idx < 2 * PP * N; idx += 256) {
         const int pp = idx / (2 * N), r = idx % (2 * N), side = r / N, z = r % N;
         const int pr = pair0 + pp;
@@ -785,8 +842,8 @@ __global__ __launch_bounds__(256) void k_fft_z_r2c(const typename cplx<T>::type4
 // y pass (forward or inverse) on S: pencils along i1 of length N = n1 for
 // every (field, i0, kz), B consecutive (field, i0, kz) per workgroup.
 template <typename T, int N>
-__global__ __launch_bounds__(256) void k_fft_y(typename cplx<T>::type *__restrict__ S, int n0, int h2,
-                                               int inverse, const typename cplx<T>::type *__restrict__ gtw)
+__device__ __forceinline__ void fft_y(const int blk, typename cplx<T>::type *__restrict__ S, int n0, int h2,
+                                      int inverse, const typename cplx<T>::type *__restrict__ gtw)
 {
     using C = typename cplx<T>::type;
     constexpr int NP = N + 1;
@@ -796,7 +853,7 @@ __global__ __launch_bounds__(256) void k_fft_y(typename cplx<T>::type *__restric
     __shared__ C tw[N];
     load_twiddles<C, N>(tw, gtw);
     const long long npen = 3LL * n0 * h2;
-    const long long q0 = (long long)blockIdx.x * B;
+    const long long q0 = (long long)blk * B;
     // pencil q = (f i0) * h2 + kz; element i1 at ((f i0) * N + i1) * h2 + kz
     for (int idx = threadIdx.x; idx < E; idx += 256) {
         const int i1 = idx / B, g = idx % B;
@@ -821,14 +878,7 @@ __global__ __launch_bounds__(256) void k_fft_y(typename cplx<T>::type *__restric
     }
 }
 
-struct kmult {
-    double k[3];     // 2 pi / L per axis
-    double c;        // (1 - e^{-a^2 d^2/4}) / a^2
-    double a2;       // a^2
-    double inv_ng;   // 1 / (n0 n1 n2)
-    double beta;     // d^2/4 - eta/2: e^{-beta k^2} removes the split and the two window convolutions
-    int n[3];
-};
+using kmult = pb_kmult;
 
 // x pass with the Brinkman projection between the forward and inverse
 // transforms: pencils along i0 of the three fields for B consecutive
@@ -840,8 +890,8 @@ struct kmult {
 // which equals the real part of the full complex transform (the
 // reference's convention).  k = 0 is set to zero.
 template <typename T, int N>
-__global__ __launch_bounds__(256) void k_fft_x_mult(typename cplx<T>::type *__restrict__ S, int n1, int h2,
-                                                    kmult km, const typename cplx<T>::type *__restrict__ gtw)
+__device__ __forceinline__ void fft_x_mult(const int blk, typename cplx<T>::type *__restrict__ S, int n1, int h2,
+                                           const kmult &km, const typename cplx<T>::type *__restrict__ gtw)
 {
     using C = typename cplx<T>::type;
     constexpr int NP = N + 1;
@@ -851,7 +901,7 @@ __global__ __launch_bounds__(256) void k_fft_x_mult(typename cplx<T>::type *__re
     __shared__ C tw[N];
     load_twiddles<C, N>(tw, gtw);
     const long long npen = (long long)n1 * h2;
-    const long long q0 = (long long)blockIdx.x * B;
+    const long long q0 = (long long)blk * B;
     const size_t fs = (size_t)N * npen;
     // element (field f, i0) of pencil q = i1 * h2 + kz at (f * N + i0) * npen + q
     for (int idx = threadIdx.x; idx < E; idx += 256) {
@@ -915,9 +965,9 @@ __global__ __launch_bounds__(256) void k_fft_x_mult(typename cplx<T>::type *__re
 // and b = (i0, i1 + 1) are extended by U(N - k) = conj U(k) and transformed
 // as ux_a + i uy_a, ux_b + i uy_b, uz_a + i uz_b into the real grid (T4).
 template <typename T, int N>
-__global__ __launch_bounds__(256) void k_fft_z_c2r(const typename cplx<T>::type *__restrict__ S,
-                                                   trip<T> *__restrict__ U3, int n0,
-                                                   int n1, const typename cplx<T>::type *__restrict__ gtw)
+__device__ __forceinline__ void fft_z_c2r(const int blk, const typename cplx<T>::type *__restrict__ S,
+                                          trip<T> *__restrict__ U3, int n0, int n1,
+                                          const typename cplx<T>::type *__restrict__ gtw)
 {
     using C = typename cplx<T>::type;
     constexpr int NP = N + 1, H = N / 2 + 1;
@@ -927,7 +977,7 @@ __global__ __launch_bounds__(256) void k_fft_z_c2r(const typename cplx<T>::type 
     __shared__ C tw[N];
     load_twiddles<C, N>(tw, gtw);
     const int npair = n0 * (n1 / 2);
-    const int pair0 = blockIdx.x * PP;
+    const int pair0 = blk * PP;
     const size_t fs = (size_t)n0 * n1 * H;
     for (int idx = threadIdx.x; idx < PP * 2 * N; idx += 256) {
         const int pp = idx / (2 * N), r = idx % (2 * N), side = r / N, k = r % N;
@@ -977,6 +1027,57 @@ __global__ __launch_bounds__(256) void k_fft_z_c2r(const typename cplx<T>::type 
         const C a = buf[(3 * pp + side) * NP + z], c3 = buf[(3 * pp + 2) * NP + z];
         U3[(size_t)(2 * pr + side) * N + z] = trip<T>{a.x, a.y, side == 0 ? c3.x : c3.y};
     }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_fft_z_r2c(const typename cplx<T>::type4 *__restrict__ R4,
+                                                   typename cplx<T>::type *__restrict__ S, int n0,
+                                                   int n1, const typename cplx<T>::type *__restrict__ gtw)
+{
+    fft_z_r2c<T, N>((int)blockIdx.x, R4, S, n0, n1, gtw);
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_fft_y(typename cplx<T>::type *__restrict__ S, int n0, int h2,
+                                               int inverse, const typename cplx<T>::type *__restrict__ gtw)
+{
+    fft_y<T, N>((int)blockIdx.x, S, n0, h2, inverse, gtw);
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_fft_x_mult(typename cplx<T>::type *__restrict__ S, int n1, int h2,
+                                                    kmult km, const typename cplx<T>::type *__restrict__ gtw)
+{
+    fft_x_mult<T, N>((int)blockIdx.x, S, n1, h2, km, gtw);
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_fft_z_c2r(const typename cplx<T>::type *__restrict__ S,
+                                                   trip<T> *__restrict__ U3, int n0,
+                                                   int n1, const typename cplx<T>::type *__restrict__ gtw)
+{
+    fft_z_c2r<T, N>((int)blockIdx.x, S, U3, n0, n1, gtw);
+}
+
+// Grouped FFT pass PASS (0 z forward, 1 y forward, 2 x with the multiply,
+// 3 y inverse, 4 z inverse) for the systems whose transformed axis has
+// length N; tw: the twiddles of length N.
+template <typename T, int N, int PASS>
+__global__ __launch_bounds__(256) void k_gfft(const pb_sys *__restrict__ sys, const int2 *__restrict__ map,
+                                              const typename cplx<T>::type *__restrict__ tw)
+{
+    using C = typename cplx<T>::type;
+    const int2 m = map[blockIdx.x];
+    const pb_sys &S = sys[m.x];
+    const int h2 = S.g.n[2] / 2 + 1;
+    if constexpr (PASS == 0)
+        fft_z_r2c<T, N>(m.y, (const typename cplx<T>::type4 *)S.grid4, (C *)S.spec, S.g.n[0], S.g.n[1], tw);
+    else if constexpr (PASS == 1 || PASS == 3)
+        fft_y<T, N>(m.y, (C *)S.spec, S.g.n[0], h2, PASS == 3, tw);
+    else if constexpr (PASS == 2)
+        fft_x_mult<T, N>(m.y, (C *)S.spec, S.g.n[1], h2, S.km, tw);
+    else
+        fft_z_c2r<T, N>(m.y, (const C *)S.spec, (trip<T> *)S.grid4, S.g.n[0], S.g.n[1], tw);
 }
 
 template <typename T, int N>
@@ -1079,15 +1180,7 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
     if ((st = pb_reserve(c, &c->grid, (size_t)g.ng * sizeof(C4)))) return st;
     if ((st = pb_reserve(c, &c->spec, 3 * (size_t)g.n[0] * g.n[1] * h2 * sizeof(C)))) return st;
     if ((st = upload_twiddles<T>(c, s))) return st;
-    kmult km;
-    for (int ax = 0; ax < 3; ax++) {
-        km.k[ax] = 2 * M_PI / box.L[ax];
-        km.n[ax] = g.n[ax];
-    }
-    km.c = -std::expm1(-a * a * d * d / 4) / (a * a);
-    km.a2 = a * a;
-    km.inv_ng = 1.0 / (double)g.ng;
-    km.beta = d * d / 4 - g.eta / 2;
+    const kmult km = pb_make_kmult(g, box, a, d);
     void *grid4 = c->grid.ptr, *spec = c->spec.ptr;
     {
         const long long nreg = (long long)((g.n[0] + 7) / 8) * ((g.n[1] + 7) / 8) * ((g.n[2] + 7) / 8);
@@ -1118,4 +1211,127 @@ int pb_run_kspace(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cell
     if (precision == PB_PREC_MIXED)
         return run_kspace_t<float>(c, s, g, cells, box, a, d, ns, nt, u, accumulate);
     return run_kspace_t<double>(c, s, g, cells, box, a, d, ns, nt, u, accumulate);
+}
+
+pb_kmult pb_make_kmult(const pb_kgrid &g, const pb_box &box, double a, double d)
+{
+    pb_kmult km;
+    for (int ax = 0; ax < 3; ax++) {
+        km.k[ax] = 2 * M_PI / box.L[ax];
+        km.n[ax] = g.n[ax];
+    }
+    km.c = -std::expm1(-a * a * d * d / 4) / (a * a);
+    km.a2 = a * a;
+    km.inv_ng = 1.0 / (double)g.ng;
+    km.beta = d * d / 4 - g.eta / 2;
+    return km;
+}
+
+int pb_twiddles(pb_context *c, hipStream_t s, int precision, const void **tw)
+{
+    int st = precision == PB_PREC_MIXED ? upload_twiddles<float>(c, s) : upload_twiddles<double>(c, s);
+    *tw = c->twid[precision == PB_PREC_MIXED ? 1 : 0].ptr;
+    return st;
+}
+
+namespace {
+template <typename T>
+int fft_blocks_t(int pass, const pb_kgrid &g)
+{
+    const long long h2 = g.n[2] / 2 + 1;
+    long long work, per;
+    if (pass == 0 || pass == 4) {
+        per = std::max(1, pbudget<T, 0>::E / (3 * g.n[2]));
+        work = (long long)g.n[0] * (g.n[1] / 2);
+    } else if (pass == 1 || pass == 3) {
+        per = std::max(1, pbudget<T, 1>::E / g.n[1]);
+        work = 3LL * g.n[0] * h2;
+    } else {
+        per = std::max(1, pbudget<T, 2>::E / (3 * g.n[0]));
+        work = (long long)g.n[1] * h2;
+    }
+    return (int)((work + per - 1) / per);
+}
+
+template <typename T, int N>
+void launch_gfft(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, int pass, const void *tw)
+{
+    using C = typename cplx<T>::type;
+    const C *t = (const C *)tw + (N - 4); // table of length N
+    dim3 gd(nblk), bd(256);
+    switch (pass) {
+    case 0: hipLaunchKernelGGL((k_gfft<T, N, 0>), gd, bd, 0, s, sys, map, t); break;
+    case 1: hipLaunchKernelGGL((k_gfft<T, N, 1>), gd, bd, 0, s, sys, map, t); break;
+    case 2: hipLaunchKernelGGL((k_gfft<T, N, 2>), gd, bd, 0, s, sys, map, t); break;
+    case 3: hipLaunchKernelGGL((k_gfft<T, N, 3>), gd, bd, 0, s, sys, map, t); break;
+    default: hipLaunchKernelGGL((k_gfft<T, N, 4>), gd, bd, 0, s, sys, map, t); break;
+    }
+}
+
+template <typename T>
+int group_fft_t(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, int pass, int n, const void *tw)
+{
+    switch (n) {
+    case 4: launch_gfft<T, 4>(s, sys, map, nblk, pass, tw); break;
+    case 8: launch_gfft<T, 8>(s, sys, map, nblk, pass, tw); break;
+    case 16: launch_gfft<T, 16>(s, sys, map, nblk, pass, tw); break;
+    case 32: launch_gfft<T, 32>(s, sys, map, nblk, pass, tw); break;
+    case 64: launch_gfft<T, 64>(s, sys, map, nblk, pass, tw); break;
+    case 128: launch_gfft<T, 128>(s, sys, map, nblk, pass, tw); break;
+    case 256: launch_gfft<T, 256>(s, sys, map, nblk, pass, tw); break;
+    case 512: launch_gfft<T, 512>(s, sys, map, nblk, pass, tw); break;
+    default: return PB_ERR_UNSUPPORTED;
+    }
+    return PB_OK;
+}
+
+template <typename T>
+int group_interp_t(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, int P)
+{
+    dim3 gd(nblk), bd(64 * kInterpWaves);
+#define PB_GINTERP(PP) hipLaunchKernelGGL((k_ginterp<T, PP>), gd, bd, 0, s, sys, map, nblk)
+    switch (P) {
+    case 4: PB_GINTERP(4); break;
+    case 6: PB_GINTERP(6); break;
+    case 8: PB_GINTERP(8); break;
+    case 10: PB_GINTERP(10); break;
+    case 12: PB_GINTERP(12); break;
+    case 16: PB_GINTERP(16); break;
+    case 20: PB_GINTERP(20); break;
+    case 24: PB_GINTERP(24); break;
+    case 0: hipLaunchKernelGGL(k_ginterp_any<T>, gd, dim3(256), 0, s, sys, map); break;
+    default: return PB_ERR_UNSUPPORTED;
+    }
+#undef PB_GINTERP
+    return PB_OK;
+}
+} // namespace
+
+int pb_group_fft_blocks(int precision, int pass, const pb_kgrid &g)
+{
+    return precision == PB_PREC_MIXED ? fft_blocks_t<float>(pass, g) : fft_blocks_t<double>(pass, g);
+}
+
+void pb_group_spread(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, int precision)
+{
+    if (nblk <= 0) return;
+    if (precision == PB_PREC_MIXED)
+        hipLaunchKernelGGL(k_gspread<float>, dim3(nblk), dim3(64 * kSpreadWaves), 0, s, sys, map);
+    else
+        hipLaunchKernelGGL(k_gspread<double>, dim3(nblk), dim3(64 * kSpreadWaves), 0, s, sys, map);
+}
+
+int pb_group_fft(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, int precision, int pass, int n,
+                 const void *twiddles)
+{
+    if (nblk <= 0) return PB_OK;
+    return precision == PB_PREC_MIXED ? group_fft_t<float>(s, sys, map, nblk, pass, n, twiddles)
+                                      : group_fft_t<double>(s, sys, map, nblk, pass, n, twiddles);
+}
+
+int pb_group_interp(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, int precision, int P)
+{
+    if (nblk <= 0) return PB_OK;
+    return precision == PB_PREC_MIXED ? group_interp_t<float>(s, sys, map, nblk, P)
+                                      : group_interp_t<double>(s, sys, map, nblk, P);
 }

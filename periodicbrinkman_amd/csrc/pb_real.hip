@@ -32,6 +32,7 @@
 namespace {
 
 constexpr int kWaves = 4; // wavefronts per workgroup, each independent
+static_assert(kWaves == pb_pair_waves, "pair workgroup size");
 constexpr int kQ = 128;   // ring capacity (power of two)
 
 
@@ -222,7 +223,7 @@ __device__ __forceinline__ void batch32(wave_lds &W, const int lane, const int c
 
 template <bool MIXED, bool TABLE, int DEG>
 __device__ __forceinline__ void pair_body(
-    const int2 *__restrict__ tiles, const int *__restrict__ ntiles_ptr,
+    const int blk, const int2 *__restrict__ tiles, const int *__restrict__ ntiles_ptr,
     const int *__restrict__ cstart, const double4 *__restrict__ spos,
     const double4 *__restrict__ sfrc, const double4 *__restrict__ tpos,
     const int *__restrict__ tperm, int nsplit, pb_cells cells, pb_box box,
@@ -247,7 +248,7 @@ __device__ __forceinline__ void pair_body(
     const float2 *tab32 = reinterpret_cast<const float2 *>(dyn_lds + ntab);
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const long long item = (long long)blockIdx.x * kWaves + wave;
+    const long long item = (long long)blk * kWaves + wave;
     const int tile = (int)(item / nsplit), split = (int)(item % nsplit);
     if (tile >= *ntiles_ptr) { // whole wavefront leaves; no block barriers below
         if (lane == 0) stats[item] = make_uint4(0u, 0u, 0u, 0u);
@@ -451,7 +452,7 @@ __global__ void k_zero(long long n, double *__restrict__ a)
         const double4 *__restrict__ spos, const double4 *__restrict__ sfrc,                              \
         const double4 *__restrict__ tpos, const int *__restrict__ tperm, int nsplit, pb_cells cells,      \
         pb_box box, const pb_kparams *__restrict__ kpp, double *__restrict__ u, uint4 *__restrict__ stats
-#define PB_PAIR_FWD tiles, ntiles_ptr, cstart, spos, sfrc, tpos, tperm, nsplit, cells, box, kpp, u, stats
+#define PB_PAIR_FWD (int)blockIdx.x, tiles, ntiles_ptr, cstart, spos, sfrc, tpos, tperm, nsplit, cells, box, kpp, u, stats
 
 // The table path fits 4 wavefronts per SIMD (build-time PB_PAIR_OCC); the
 // closed forms need more registers and run with the compiler's own choice.
@@ -470,7 +471,44 @@ __global__ __launch_bounds__(64 * kWaves) void k_pair_direct(PB_PAIR_PARAMS)
     pair_body<MIXED, false, PB_TAB_DEG>(PB_PAIR_FWD);
 }
 
+// Grouped: every system of a class, block b on system map[b].x.
+#define PB_GPAIR_FWD                                                                                    \
+    m.y, S.tiles, S.ntiles, S.sstart, S.spos, S.sfrc, S.tpos, S.tperm, nsplit, S.cells, S.box, S.kp, S.u, S.stats
+template <bool MIXED, int DEG>
+__global__ __launch_bounds__(64 * kWaves, PB_PAIR_OCC) void k_gpair_table(const pb_sys *__restrict__ sys,
+                                                                          const int2 *__restrict__ map, int nsplit)
+{
+    const int2 m = map[blockIdx.x];
+    const pb_sys &S = sys[m.x];
+    pair_body<MIXED, true, DEG>(PB_GPAIR_FWD);
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(64 * kWaves) void k_gpair_direct(const pb_sys *__restrict__ sys,
+                                                              const int2 *__restrict__ map, int nsplit)
+{
+    const int2 m = map[blockIdx.x];
+    const pb_sys &S = sys[m.x];
+    pair_body<MIXED, false, PB_TAB_DEG>(PB_GPAIR_FWD);
+}
+#undef PB_GPAIR_FWD
+
 } // namespace
+
+int pb_group_pair(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, int nsplit, bool far32, int tdeg,
+                  size_t lds)
+{
+    if (nblk <= 0) return PB_OK;
+    dim3 gd(nblk), bd(64 * kWaves);
+    if (tdeg == PB_TAB_DEG_MIXED && far32) hipLaunchKernelGGL((k_gpair_table<true, PB_TAB_DEG_MIXED>), gd, bd, lds, s, sys, map, nsplit);
+    else if (tdeg == PB_TAB_DEG && far32) hipLaunchKernelGGL((k_gpair_table<true, PB_TAB_DEG>), gd, bd, lds, s, sys, map, nsplit);
+    else if (tdeg == PB_TAB_DEG_MIXED) hipLaunchKernelGGL((k_gpair_table<false, PB_TAB_DEG_MIXED>), gd, bd, lds, s, sys, map, nsplit);
+    else if (tdeg == PB_TAB_DEG) hipLaunchKernelGGL((k_gpair_table<false, PB_TAB_DEG>), gd, bd, lds, s, sys, map, nsplit);
+    else if (tdeg == 0 && far32) hipLaunchKernelGGL((k_gpair_direct<true>), gd, bd, 0, s, sys, map, nsplit);
+    else if (tdeg == 0) hipLaunchKernelGGL((k_gpair_direct<false>), gd, bd, 0, s, sys, map, nsplit);
+    else return PB_ERR_INVALID;
+    return hipGetLastError() == hipSuccess ? PB_OK : PB_ERR_HIP;
+}
 
 void pb_zero_u(hipStream_t s, int n, double *u)
 {
@@ -486,6 +524,7 @@ int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kp
     (void)ns;
     int st;
     c->stat_items = 0;
+    c->stat_group = false;
     if (nt <= 0) return PB_OK;
     if (!accumulate) pb_zero_u(s, nt, u);
     const int max_tiles = c->max_tiles; // tiles built with the target binning

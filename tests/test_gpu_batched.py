@@ -89,3 +89,46 @@ def test_batched_graph_replays_follow_the_inputs():
         assert relL2(got, scale * ref) < 1e-12, (k, scale)
     del junk
     ctx.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-6)])
+def test_batched_grouped_and_single_systems_mixed(precision, tol):
+    # systems whose cell lists fit one scan workgroup run grouped (one launch
+    # per phase for all of them); the middle system's fine cell grid (rc small
+    # against L: ncell + target keys > 2^18) sends it through the per-system
+    # path.  Every system against the oracle and the single-system call; the
+    # pair statistics of the call cover the grouped systems.
+    import torch
+    rng = np.random.default_rng(5)
+    sizes = [(600, 500), (400, 300), (700, 900), (64, 1)]
+    params, xs, fs, ys = [], [], [], []
+    for i, (ns, nt) in enumerate(sizes):
+        L = (1.0, 1.0, 1.0)
+        x, f, y = pb.uniform_system(pb.config_seed(500 + i), ns, nt, L)
+        if i == 1:
+            q = pb.Params(L=L, alpha=4.0, eps=1e-3, d=0.03, rc=0.06, grid=(64, 64, 64), P=8,
+                          m=float(np.sqrt(np.pi * 8)), precision=precision)
+        else:
+            q = pb.Params(L=L, alpha=float(rng.uniform(1, 30)), eps=1e-3, precision=precision,
+                          tol=1e-8 if precision == "fp64" else 1e-7)
+        params.append(q)
+        xs.append(x), fs.append(f), ys.append(y)
+    so = np.concatenate([[0], np.cumsum([s[0] for s in sizes])])
+    to = np.concatenate([[0], np.cumsum([s[1] for s in sizes])])
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+    ctx = pb.Context(0)
+    X, F, Y = dev(np.concatenate(xs)), dev(np.concatenate(fs)), dev(np.concatenate(ys))
+    u = ctx.velocity_batched(params, X, F, so, Y, to).cpu().numpy()
+    stats = ctx.last_pair_stats()
+    u2 = ctx.velocity_batched(params, X, F, so, Y, to).cpu().numpy()  # captured
+    u3 = ctx.velocity_batched(params, X, F, so, Y, to).cpu().numpy()  # replayed
+    for i, q in enumerate(params):
+        ui = u[to[i]:to[i + 1]]
+        want = po.velocity_ewald(ys[i], xs[i], fs[i], q.L, q.alpha, q.d, q.eps, q.rc, q.grid, (q.P,) * 3, q.m)
+        assert relL2(ui, want) < tol, (i, relL2(ui, want))
+        single = ctx.velocity(q, dev(xs[i]), dev(fs[i]), dev(ys[i])).cpu().numpy()
+        assert relL2(ui, single) < (1e-12 if precision == "fp64" else 1e-6), i
+    assert relL2(u2, u) < (1e-12 if precision == "fp64" else 1e-6)
+    assert relL2(u3, u) < (1e-12 if precision == "fp64" else 1e-6)
+    assert stats[0] + stats[1] > 0
+    ctx.close()
