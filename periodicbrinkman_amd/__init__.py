@@ -455,6 +455,28 @@ def clustered_system(seed, n, nclusters=64, sigma=0.02, L=(1.0, 1.0, 1.0)):
     return x, f
 
 
+def batched_systems(seed, nsys=512, nmin=100, nmax=20000):
+    """Independent periodic systems of uneven size, targets = sources.  One
+    stream (seed) draws, per system in turn, N = floor(exp(U(ln nmin,
+    ln nmax))), the cubic box side L ~ U[0.5, 2) and alpha = exp(U(0, ln 50));
+    eps = 1e-3 L.  System i's points are uniform_system(seed + ((i + 1) << 32),
+    N, 0, (L, L, L)).  Returns (systems, x, f, offsets): systems a list of
+    dicts (n, L, alpha, eps), x and f the concatenated (sum N, 3) arrays."""
+    r = Rng(seed)
+    systems, xs, fs = [], [], []
+    for i in range(nsys):
+        u = r.uniform(3)
+        n = int(np.floor(np.exp(np.log(nmin) + u[0] * (np.log(nmax) - np.log(nmin)))))
+        L = 0.5 + 1.5 * float(u[1])
+        alpha = float(np.exp(u[2] * np.log(50.0)))
+        x, f, _ = uniform_system(seed + ((i + 1) << 32), n, 0, (L, L, L))
+        systems.append({"n": n, "L": L, "alpha": alpha, "eps": 1e-3 * L})
+        xs.append(x)
+        fs.append(f)
+    off = np.concatenate([[0], np.cumsum([s["n"] for s in systems])]).astype(np.int64)
+    return systems, np.concatenate(xs), np.concatenate(fs), off
+
+
 # ---- the reference driver ---------------------------------------------------------
 def read_param_file(path):
     """Parameters of the reference driver's parameter file (pb_read_param_file):

@@ -5,7 +5,8 @@
 // sorted array.  Targets are ordered by a brick-Morton key (bricks of 4x4x4
 // cells in z-major order, Morton order inside a brick), so the targets of a
 // quarter brick (2x2x4 cells, 16 consecutive keys) are contiguous; the pair
-// kernel's tiles are runs of <= 64 of them (pb_build_tiles).
+// kernel's tiles are runs of <= 64 of them, or a whole half brick or brick
+// where that holds <= 64 targets (quarter_tiles).
 //
 // Positions are wrapped into [0, L) with the same operation sequence as the
 // oracle (v - floor(v/L)*L, no contraction) so both place a point on a cell
@@ -140,12 +141,43 @@ __global__ void k_fill_int(int n, int *__restrict__ a, int v)
     if (i < n) a[i] = v;
 }
 
+// Target tiles from the counts of a brick's quarters (qi = quarter within the
+// brick, c its count, h the count of its half brick, B of the brick): a brick
+// with <= 64 targets is one tile (4x4x4 cells), else a half brick with <= 64
+// (2x4x4), else each quarter (2x2x4) in runs of <= 64.  Sparse targets thus
+// share one candidate walk per tile instead of a few lanes each.  Returns the
+// tiles that quarter qi writes and their span (targets from the quarter's
+// first one).
+__device__ __forceinline__ int quarter_tiles(int qi, int c, int h, int B, int &span)
+{
+    if (B <= 64) {
+        span = B;
+        return qi == 0 && B > 0;
+    }
+    if (h <= 64) {
+        span = h;
+        return (qi & 1) == 0 && h > 0;
+    }
+    span = c;
+    return (c + 63) >> 6;
+}
+
+__device__ __forceinline__ int quarter_tiles_at(const int *__restrict__ tstart, int q, int &first, int &span)
+{
+    const int b = q >> 2, hq = q >> 1;
+    first = tstart[16 * q];
+    const int c = tstart[16 * q + 16] - first;
+    const int h = tstart[32 * hq + 32] - tstart[32 * hq];
+    const int B = tstart[64 * b + 64] - tstart[64 * b];
+    return quarter_tiles(q & 3, c, h, B, span);
+}
+
 __global__ void k_quarter_tiles(int nq, const int *__restrict__ tstart, int *__restrict__ qcount)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq) return;
-    const int cnt = tstart[16 * q + 16] - tstart[16 * q];
-    qcount[q] = (cnt + 63) >> 6;
+    int first, span;
+    qcount[q] = quarter_tiles_at(tstart, q, first, span);
 }
 
 __global__ void k_write_tiles(int nq, const int *__restrict__ tstart, const int *__restrict__ qoff,
@@ -153,9 +185,10 @@ __global__ void k_write_tiles(int nq, const int *__restrict__ tstart, const int 
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq) return;
-    const int first = tstart[16 * q], cnt = tstart[16 * q + 16] - first;
-    int o = qoff[q];
-    for (int i = 0; 64 * i < cnt; i++) tiles[o + i] = make_int2(first + 64 * i, min(64, cnt - 64 * i));
+    int first, span;
+    const int nt = quarter_tiles_at(tstart, q, first, span);
+    const int o = qoff[q];
+    for (int i = 0; i < nt; i++) tiles[o + i] = make_int2(first + 64 * i, min(64, span - 64 * i));
 }
 
 // ---- fused binning of sources and targets (small key spaces) -----------------
@@ -280,7 +313,7 @@ __device__ __forceinline__ int scan1024(int v, int *wsum, int &total)
 // Exclusive scan of n counts into start[0..n] (start[n] = total), 16
 // consecutive counts per thread and 16384 per round.  With tiles != null the
 // counts are target keys (n a multiple of 64): each thread's 16 keys are one
-// quarter brick, whose targets become ceil(count / 64) tiles of <= 64.
+// quarter brick, whose targets become tiles by quarter_tiles.
 __device__ void scan_counts(const int *__restrict__ count, int n, int *__restrict__ start, int *wsum,
                             int2 *__restrict__ tiles, int *__restrict__ ntiles)
 {
@@ -331,11 +364,14 @@ __device__ void scan_counts(const int *__restrict__ count, int n, int *__restric
             }
         }
         carry += tot;
-        if (tiles) {
-            const int nq = (sum + 63) >> 6;
+        if (tiles) { // the 4 quarters of a brick are 4 consecutive lanes
+            const int h = sum + __shfl_xor(sum, 1, 64);
+            const int B = h + __shfl_xor(h, 2, 64);
+            int span;
+            const int nq = quarter_tiles((int)threadIdx.x & 3, sum, h, B, span);
             int ttot;
             int o = scan1024(nq, wsum, ttot) + tcarry;
-            for (int k = 0; k < nq; k++) tiles[o + k] = make_int2(first + 64 * k, min(64, sum - 64 * k));
+            for (int k = 0; k < nq; k++) tiles[o + k] = make_int2(first + 64 * k, min(64, span - 64 * k));
             tcarry += ttot;
         }
     }

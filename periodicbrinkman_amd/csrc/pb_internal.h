@@ -7,7 +7,8 @@
 //                    c = (c0 * nc1 + c1) * nc2 + c2 (z-major)
 //   sorted targets   tpos[t] = (x, y, z, 0), tperm[t] = original index, in
 //                    brick-Morton order; tiles[k] = (first, count <= 64)
-//                    within one quarter brick (2x2x4 cells)
+//                    within one quarter brick (2x2x4 cells), or a whole half
+//                    brick or brick where that holds <= 64 targets
 //   k-space grids    real grid: (fx, fy, fz, 0) then (ux, uy, uz, 0) per point,
 //                    point (i0, i1, i2) at (i0*n1 + i1)*n2 + i2 (T4);
 //                    half spectrum: 3 fields of n0 x n1 x (n2/2 + 1) complex,
@@ -256,7 +257,8 @@ int pb_run_mrhs_real(pb_context *c, hipStream_t s, const pb_cells &cells, const 
                      size_t table_doubles, int table_deg, const pb_box &box, int ns, int nt, int R, int RP,
                      double *u);
 
-// Target tiles: runs of <= 64 sorted targets inside one quarter brick; the
+// Target tiles: runs of <= 64 sorted targets inside one quarter brick (or a
+// sparse half brick or brick); the
 // tile count is left on the device, *max_tiles bounds it.  Built by
 // pb_bin_points for targets.
 int pb_build_tiles(pb_context *c, hipStream_t s, const pb_cells &cells, int nt, int *max_tiles);

@@ -6,7 +6,8 @@
 // realShells :564-693, kernelReal :219-314).
 //
 // Work item = (tile, split): a tile is <= 64 targets of one quarter brick
-// (2x2x4 cells, one target per lane); its bounding box B decides the
+// (2x2x4 cells; half a brick or a brick where targets are sparse), one
+// target per lane; its bounding box B decides the
 // candidate sources: the (x, y) cell columns within rc of B, each trimmed in
 // z to sqrt(rc^2 - gap^2), which are contiguous runs of the z-major sorted
 // sources.  The columns are dealt round-robin to `nsplit` wavefronts, which
