@@ -170,9 +170,13 @@ int pb_velocity_kspace(pb_context *ctx, const pb_params *p, int ns, const double
  * (box, alpha, eps, precision, kernel, tol); a system whose d or grid is
  * unset is tuned by pb_autotune, which writes the choice back into
  * params[i].  The kernel constants and tables of all systems are built and
- * uploaded once, then every system is queued on `stream` without further
- * host synchronization.  u is OVERWRITTEN.  No reference counterpart: the
- * reference evaluates one system per call. */
+ * uploaded once; systems with moderate cell grids then run together, one
+ * launch per phase and kernel class for all of them, the others one by one
+ * on internal streams, all ordered after `stream` without further host
+ * synchronization.  The second of two identical calls (systems, parameters,
+ * arrays, stream) captures a hipGraph that later identical calls replay.
+ * u is OVERWRITTEN.  No reference counterpart: the reference evaluates one
+ * system per call. */
 int pb_velocity_batched(pb_context *ctx, int nsys, pb_params *params, const int *src_off, const double *xn,
                         const double *f, const int *tgt_off, const double *y, double *u, void *stream);
 
