@@ -22,6 +22,11 @@ replays a captured graph; the mixed form against the FP64 (table) and plain
 FP64 (closed forms) forms of the same systems, errors against an FP64
 evaluation of every system at tuner tolerance 1e-11.
 
+Clustered leg ("clustered_form"): 1e6 sources in 64 Gaussian clusters (sigma
+0.02), targets = sources, alpha 25, eps 5e-4, 256^3, P 16 (seed 0x5EED0002).
+Multi-RHS leg ("multirhs_form"): 2e5 uniform particles with 64 FP32 force
+sets, targets = sources, 128^3, P 12 (seed 0x5EED0004).
+
 Prints ONE JSON line (rank 0, N = 1 only).
 """
 import argparse
@@ -44,6 +49,7 @@ ALPHA, EPS, GRID, P = 10.0, 1e-3, (128, 128, 128), 12
 # The benchmark takes tol 2e-6; the measured errors are reported and checked.
 ACCURACY, TOL = 1e-6, 2e-6
 BATCH_TOL = 1e-6  # tuner tolerance of the batched leg (its measured error is reported)
+CLUSTER_TOL = 1e-6  # tuner tolerance of the clustered leg
 # Algorithmic work of the pair kernel: 200 FLOP per (target, source image)
 # inside rc -- about 80 arithmetic operations (geometry, radial functions,
 # 3x3 apply) plus six exp/erfc at 20 FLOP each.
@@ -143,6 +149,98 @@ def batched_leg(ctx, steps, warmup):
     }
 
 
+def clustered_leg(ctx, steps):
+    """The clustered large system: 1e6 sources in 64 Gaussian clusters, targets = sources."""
+    import torch
+
+    import periodicbrinkman_amd as pb
+
+    n = 1_000_000
+    x, f = pb.clustered_system(pb.config_seed(2), n, 64, 0.02)
+    X = torch.as_tensor(x, device="cuda:0")
+    F = torch.as_tensor(f, device="cuda:0")
+    U = torch.empty_like(X)
+    base = pb.Params(alpha=25.0, eps=5e-4, grid=(256, 256, 256), P=16, tol=CLUSTER_TOL)
+
+    def run(p):
+        ctx.velocity(p, X, F, X, U)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.velocity(p, X, F, X, U)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps, U.cpu().numpy().copy()
+
+    pm = base.copy(precision="mixed").tune(n, n)
+    ms_m, u_m = run(pm)
+    stats = ctx.last_pair_stats()
+    log(f"clustered mixed: {ms_m:.2f} ms/step, pairs {stats[0] + stats[1]}")
+    ms_64, u_64 = run(base.copy(precision="fp64").tune(n, n))
+    log(f"clustered fp64: {ms_64:.2f} ms/step")
+    ms_pl, u_pl = run(base.copy(precision="fp64", kernel="direct").tune(n, n))
+    log(f"clustered plain fp64: {ms_pl:.2f} ms/step")
+    pr = pb.Params(alpha=25.0, eps=5e-4, grid=(256, 256, 256), P=16, tol=1e-10, precision="fp64").tune(n, n)
+    u_ref = ctx.velocity(pr, X, F, X).cpu().numpy()
+    return {
+        "workload": "1e6 sources in 64 Gaussian clusters (sigma 0.02, centres uniform, wrapped), targets = "
+                    "sources, alpha 25, eps 5e-4, 256^3 grid, P 16, tuner tol %g" % CLUSTER_TOL,
+        "value": n / (ms_m / 1e3), "unit": "velocity evals/s", "ms_per_step": ms_m, "steps": steps,
+        "d": pm.d, "rc": pm.rc, "pairs": stats[0] + stats[1],
+        "rel_l2_mixed": relL2(u_m, u_ref), "meets_accuracy": bool(relL2(u_m, u_ref) <= ACCURACY),
+        "rel_l2_fp64": relL2(u_64, u_ref),
+        "fp64_form": {"value": n / (ms_64 / 1e3), "ms_per_step": ms_64},
+        "plain_fp64_form": {"value": n / (ms_pl / 1e3), "ms_per_step": ms_pl},
+        "speedup_mixed_vs_plain_fp64": ms_pl / ms_m,
+        "note": "errors against an FP64 evaluation at tuner tol 1e-10 (a smaller split d)",
+    }
+
+
+def multirhs_leg(ctx, steps):
+    """Many right-hand sides: 2e5 particles, 64 FP32 force sets, targets = sources."""
+    import torch
+
+    import periodicbrinkman_amd as pb
+
+    n, R = 200_000, 64
+    x, _, _ = pb.uniform_system(pb.config_seed(4), n, 0)
+    r = pb.Rng(pb.config_seed(4) + (1 << 32))
+    Fh = r.normal(3 * n * R).reshape(R, n, 3).astype(np.float32)
+    X = torch.as_tensor(x, device="cuda:0")
+    F = torch.as_tensor(Fh, device="cuda:0")
+    p = pb.Params(alpha=10.0, eps=1e-3, grid=(128, 128, 128), P=12, tol=TOL, precision="mixed").tune(n, n)
+    U = ctx.velocity_multirhs(p, X, F, X)
+    U = ctx.velocity_multirhs(p, X, F, X, U)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.velocity_multirhs(p, X, F, X, U)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    log(f"multi-RHS: {ms:.2f} ms/call")
+    u = U.cpu().numpy()
+    # the same sets one call each (mixed single-set form), and FP64 references for three sets
+    F64 = [torch.as_tensor(Fh[k].astype(np.float64), device="cuda:0") for k in range(R)]
+    V = torch.empty_like(X)
+    ctx.velocity(p, X, F64[0], X, V)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(R):
+        ctx.velocity(p, X, F64[k], X, V)
+    torch.cuda.synchronize()
+    ms_single = (time.perf_counter() - t0) * 1e3
+    pr = pb.Params(alpha=10.0, eps=1e-3, grid=(128, 128, 128), P=20, tol=1e-11, precision="fp64").tune(n, n)
+    errs = [relL2(u[k], ctx.velocity(pr, X, F64[k], X).cpu().numpy()) for k in (0, R // 2, R - 1)]
+    return {
+        "workload": "2e5 uniform particles, 64 force sets (FP32, N(0,1)), targets = sources, alpha 10, "
+                    "eps 1e-3, 128^3 grid, P 12, tuner tol %g" % TOL,
+        "value": n * R / (ms / 1e3), "unit": "velocity evals/s (targets x force sets)", "ms_per_call": ms,
+        "calls": steps, "rel_l2_sets_0_32_63": errs, "meets_accuracy": bool(max(errs) <= ACCURACY),
+        "single_set_calls_ms": ms_single, "speedup_vs_single_set_calls": ms_single / ms,
+        "note": "real space as the MFMA contraction of FP32 kernel tiles with the 64 sets; errors of three "
+                "sets against FP64 (P 20, tuner tol 1e-11)",
+    }
+
+
 def cpu_baseline(x, f, y, p, n_real_targets, threads=1):
     """The reference algorithm (FP64 oracle) on the host: real space over all
     source pairs x 27 images (realShells) on a target subsample, extrapolated
@@ -201,6 +299,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle check")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-systems leg")
+    ap.add_argument("--no-more", action="store_true", help="skip the clustered and multi-RHS legs")
     ap.add_argument("--cpu-targets", type=int, default=8,
                     help="targets in the CPU real-space sample (all 1e5 sources x 27 images each)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -322,6 +421,9 @@ def main():
     }
     if not args.no_batched:
         result["batched_form"] = batched_leg(ctx, args.steps, args.warmup)
+    if not args.no_more:
+        result["clustered_form"] = clustered_leg(ctx, 3)
+        result["multirhs_form"] = multirhs_leg(ctx, 3)
     if not args.no_cpu:
         log("cpu baseline")
         cps, t_real, t_k = cpu_baseline(x, f, y, p64, args.cpu_targets)
