@@ -463,7 +463,7 @@ int pb_destroy(pb_context *c)
                          &c->sperm, &c->mrhs_f,
                          &c->twid[0], &c->twid[1], &c->batch_kp, &c->batch_tab, &c->dense_tmp, &c->kparams, &c->table,
                          &c->gb_int, &c->gb_d4, &c->gb_tiles, &c->gb_grid, &c->gb_spec, &c->gb_sys, &c->gb_map,
-                         &c->gb_stats};
+                         &c->gb_stats, &c->mr_sys, &c->mr_map, &c->mr_frc, &c->mr_grid, &c->mr_spec};
     for (pb_buffer *b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     if (c->ev_ready)
@@ -525,6 +525,102 @@ int pb_velocity_kspace(pb_context *ctx, const pb_params *p, int ns, const double
     return velocity_impl(ctx, p, ns, xn, f, nt, y, u, stream, false, true);
 }
 
+namespace {
+
+size_t pad16(size_t n) { return (n + 15) & ~(size_t)15; }
+size_t pad256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// k space of R force sets (FP32 pipeline) on the binned particle set: up to
+// 64 sets per grouped launch (fewer when their grids would pass 4 GB), each
+// set a system of the group with the shared cells, sorted points and k grid
+// and its own forces, grid, spectrum and block of u (overwritten).
+int mrhs_kspace(pb_context *c, hipStream_t s, const pb_params *p, const pb_cells &cells, const pb_box &box,
+                const pb_kgrid &g, int ns, int nt, int R, int RP, double *u)
+{
+    int st;
+    const size_t gbytes = pad256((size_t)g.ng * 16);                                   // float4 per point
+    const size_t sbytes = pad256(3 * (size_t)g.n[0] * g.n[1] * (g.n[2] / 2 + 1) * 8); // complex float
+    const size_t fbytes = (size_t)std::max(ns, 1) * sizeof(double4);
+    const int CH = (int)std::max<size_t>(1, std::min<size_t>(64, ((size_t)4 << 30) / (gbytes + sbytes + fbytes)));
+    const int nc = std::min(R, CH);
+    const void *tw;
+    if ((st = pb_twiddles(c, s, PB_PREC_MIXED, &tw))) return st;
+    if ((st = pb_reserve(c, &c->mr_grid, gbytes * nc))) return st;
+    if ((st = pb_reserve(c, &c->mr_spec, sbytes * nc))) return st;
+    if ((st = pb_reserve(c, &c->mr_frc, fbytes * nc))) return st;
+    // one descriptor per set, uploaded once per call
+    pb_sys base;
+    std::memset(&base, 0, sizeof base);
+    base.ns = ns;
+    base.nt = nt;
+    base.box = box;
+    base.cells = cells;
+    base.g = g;
+    base.km = pb_make_kmult(g, box, p->alpha, p->d);
+    base.sstart = (int *)c->cell_start.ptr;
+    base.spos = (double4 *)c->spos.ptr;
+    base.tpos = (double4 *)c->tpos.ptr;
+    base.tperm = (int *)c->tperm.ptr;
+    base.acc_mode = 0;
+    std::vector<pb_sys> sys(R, base);
+    for (int r = 0; r < R; r++) {
+        const int q = r % CH;
+        sys[r].sfrc = (double4 *)((char *)c->mr_frc.ptr + fbytes * q);
+        sys[r].grid4 = (char *)c->mr_grid.ptr + gbytes * q;
+        sys[r].spec = (char *)c->mr_spec.ptr + sbytes * q;
+        sys[r].u = u + (size_t)r * 3 * nt;
+    }
+    if ((st = pb_reserve(c, &c->mr_sys, sizeof(pb_sys) * R))) return st;
+    if ((st = pb_hip_check(c, hipMemcpyAsync(c->mr_sys.ptr, sys.data(), sizeof(pb_sys) * R, hipMemcpyHostToDevice, s),
+                           "descriptor upload")))
+        return st;
+    if ((st = pb_hip_check(c, hipStreamSynchronize(s), "descriptor upload"))) return st;
+    // maps: spread, the five FFT passes, the gather (XCD-renumbered: a multiple of 8)
+    const int nreg = ((g.n[0] + 7) / 8) * ((g.n[1] + 7) / 8) * ((g.n[2] + 7) / 8);
+    int per[7];
+    per[0] = nreg;
+    for (int pass = 0; pass < 5; pass++) per[1 + pass] = pb_group_fft_blocks(PB_PREC_MIXED, pass, g);
+    const bool same = g.P[0] == g.P[1] && g.P[1] == g.P[2];
+    const int Ps[8] = {4, 6, 8, 10, 12, 16, 20, 24};
+    const int P = same && std::find(Ps, Ps + 8, g.P[0]) != Ps + 8 ? g.P[0] : 0; // 0: per-axis windows
+    per[6] = P ? (nt + pb_interp_waves - 1) / pb_interp_waves : (nt + 255) / 256;
+    size_t off[8] = {0};
+    for (int k = 0; k < 7; k++) {
+        size_t n = (size_t)per[k] * nc;
+        if (k == 6) n = 8 * (size_t)nc * ((per[6] + 7) / 8);
+        off[k + 1] = off[k] + n;
+    }
+    if ((st = pb_reserve(c, &c->mr_map, sizeof(int2) * off[7]))) return st;
+    int2 *map = (int2 *)c->mr_map.ptr;
+    int filled = -1; // chunk size the maps were filled for
+    long long ngather = 0;
+    for (int r0 = 0; r0 < R; r0 += CH) {
+        const int n = std::min(CH, R - r0);
+        if (n != filled) {
+            for (int k = 0; k < 6; k++) {
+                const long long valid = (long long)per[k] * n;
+                pb_fill_map(s, map + off[k], valid, per[k], valid);
+            }
+            ngather = P ? pb_fill_map_xcd(s, map + off[6], n, per[6]) : (long long)per[6] * n;
+            if (!P) pb_fill_map(s, map + off[6], ngather, per[6], ngather);
+            filled = n;
+        }
+        const pb_sys *sd = (const pb_sys *)c->mr_sys.ptr + r0;
+        if ((st = pb_force_sets(c, s, ns, r0, n, RP, (double4 *)c->mr_frc.ptr))) return st;
+        pb_group_spread(s, sd, map + off[0], per[0] * n, PB_PREC_MIXED);
+        for (int pass = 0; pass < 5; pass++) {
+            const int ax = (pass == 0 || pass == 4) ? 2 : (pass == 2 ? 0 : 1);
+            if ((st = pb_group_fft(s, sd, map + off[1 + pass], per[1 + pass] * n, PB_PREC_MIXED, pass, g.n[ax], tw)))
+                return pb_fail(c, st, "FFT size");
+        }
+        if ((st = pb_group_interp(s, sd, map + off[6], (int)ngather, PB_PREC_MIXED, P)))
+            return pb_fail(c, st, "gather window");
+    }
+    return pb_hip_check(c, hipGetLastError(), "multi-RHS k-space launch");
+}
+
+} // namespace
+
 int pb_velocity_multirhs(pb_context *c, const pb_params *p, int ns, const double *xn, int R,
                          const float *f, int nt, const double *y, double *u, void *stream)
 {
@@ -547,12 +643,7 @@ int pb_velocity_multirhs(pb_context *c, const pb_params *p, int ns, const double
     if (nt == 0) return PB_OK;
     pb_kgrid g;
     if ((st = make_kgrid(c, p, &g))) return st;
-    for (int r = 0; r < R; r++) {
-        if ((st = pb_force_set(c, s, ns, r, RP))) return st;
-        if ((st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u + (size_t)r * 3 * nt, 0,
-                                PB_PREC_MIXED)))
-            return st;
-    }
+    if ((st = mrhs_kspace(c, s, p, cells, box, g, ns, nt, R, RP, u))) return st;
     double rfar = std::fmax(3 * p->d, 8 * p->eps);
     pb_params pt = *p;
     pt.kernel = PB_KERNEL_TABLE;
@@ -570,9 +661,6 @@ namespace {
 // each launched block finding its system and its block of that system in a
 // map, so a call of hundreds of systems is a few dozen launches that fill the
 // device instead of a dozen small launches per system.
-
-size_t pad16(size_t n) { return (n + 15) & ~(size_t)15; }
-size_t pad256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 // Scratch layout, descriptors, maps and launch list of the group idx; kept
 // while key (systems, parameters, arrays) is unchanged.
@@ -611,6 +699,7 @@ int group_prepare(pb_context *c, hipStream_t s, const std::vector<int> &idx, con
         S.y = S.nt ? y + 3 * (size_t)tgt_off[i] : nullptr;
         S.u = S.nt ? u + 3 * (size_t)tgt_off[i] : nullptr;
         S.kp = (const pb_kparams *)c->batch_kp.ptr + i;
+        S.acc_mode = 2;
         const int nc = S.cells.ncell, nk = S.cells.nkey_t;
         ncount += pad16(nc + 1) + pad16(nk + 1);
         nint += pad16(nc + 1) + 3 * pad16(S.ns) + pad16(nk + 1) + 3 * pad16(S.nt) + 16;

@@ -417,6 +417,29 @@ __global__ __launch_bounds__(1024) void k_gbin_scan(const pb_sys *__restrict__ s
     scan_counts(S.tcount, S.cells.nkey_t, S.tstart, wsum, S.tiles, S.ntiles);
 }
 
+__global__ void k_fill_map(int2 *__restrict__ map, long long n, int per, long long valid)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) map[i] = i < valid ? make_int2((int)(i / per), (int)(i % per)) : make_int2(-1, 0);
+}
+
+// Gather map for systems of per blocks each, read after the XCD renumbering
+// of pb_xcd_block (n = 8 nsys per8 entries, per8 = ceil(per / 8)): at each
+// step of the dispatch the 8 XCDs take the 8 eighths of one system, so the
+// systems pass through the device one after another (one grid in the shared
+// cache at a time) while each XCD's L2 sees a contiguous run of targets.
+__global__ void k_fill_map_xcd(int2 *__restrict__ map, int nsys, int per)
+{
+    const int per8 = (per + 7) / 8;
+    const long long n = 8LL * nsys * per8, L = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= n) return;
+    const long long steps = (long long)nsys * per8;
+    const int x = (int)(L / steps);
+    const long long k = L % steps;
+    const int sys = (int)(k / per8), b = x * per8 + (int)(k % per8);
+    map[L] = b < per ? make_int2(sys, b) : make_int2(-1, 0);
+}
+
 __global__ void k_fill_zero(long long n, int *__restrict__ a)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -569,4 +592,16 @@ void pb_fill_zero(hipStream_t s, long long n, int *a)
 bool pb_bin_fused_ok(const pb_cells &cells)
 {
     return (long long)cells.ncell + cells.nkey_t <= kFusedKeys && cells.nkey_t % 64 == 0;
+}
+
+void pb_fill_map(hipStream_t s, int2 *map, long long n, int per, long long valid)
+{
+    if (n > 0) hipLaunchKernelGGL(k_fill_map, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, map, n, per, valid);
+}
+
+long long pb_fill_map_xcd(hipStream_t s, int2 *map, int nsys, int per)
+{
+    const long long n = 8LL * nsys * ((per + 7) / 8);
+    if (n > 0) hipLaunchKernelGGL(k_fill_map_xcd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, map, nsys, per);
+    return n;
 }

@@ -139,6 +139,7 @@ struct pb_sys {
     int *ntiles;
     void *grid4, *spec; // real grid (T4 / packed trip) and half spectrum
     uint4 *stats;       // pair-kernel statistics, one per wavefront item
+    int acc_mode;       // the gather: 0 overwrites u, 2 adds with atomics
 };
 
 // ---- host helpers (pb_params.cpp) ----------------------------------------
@@ -205,6 +206,8 @@ struct pb_context {
     };
     std::vector<group_launch> group_launches;
     size_t group_counts = 0;    // ints to zero at the start of gb_int (the count arrays)
+    // many right-hand sides: k space of up to 64 force sets per grouped launch
+    pb_buffer mr_sys, mr_map, mr_frc, mr_grid, mr_spec;
     long long group_items = 0;  // pair statistics items
     // the batched call's captured graph and the calls it replays
     hipGraphExec_t graph_exec = nullptr;
@@ -253,6 +256,8 @@ int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_
 // (double, set r at r * 3 nt).
 int pb_sort_forces(pb_context *c, hipStream_t s, int ns, int R, int RP, const float *f);
 int pb_force_set(pb_context *c, hipStream_t s, int ns, int r, int RP);
+// frc[q * ns + j] = force set r0 + q of sorted source j, q < nr
+int pb_force_sets(pb_context *c, hipStream_t s, int ns, int r0, int nr, int RP, double4 *frc);
 int pb_run_mrhs_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp,
                      size_t table_doubles, int table_deg, const pb_box &box, int ns, int nt, int R, int RP,
                      double *u);
@@ -293,6 +298,11 @@ void pb_group_bin_count(hipStream_t s, const pb_sys *sys, const int2 *map, int n
 void pb_group_bin_scan(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk);
 void pb_group_bin_scatter(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk);
 void pb_fill_zero(hipStream_t s, long long n, int *a);
+// map[i] = (i / per, i % per) for i < valid, (-1, 0) for valid <= i < n
+void pb_fill_map(hipStream_t s, int2 *map, long long n, int per, long long valid);
+// gather map (read after the XCD renumbering) that passes nsys systems of
+// per blocks through all XCDs one system at a time; returns the entries
+long long pb_fill_map_xcd(hipStream_t s, int2 *map, int nsys, int per);
 // Pair kernel of one class (far32, table degree; tdeg 0: closed forms) with
 // nsplit wavefronts per tile and lds bytes of dynamic LDS (the largest table).
 int pb_group_pair(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, int nsplit, bool far32,

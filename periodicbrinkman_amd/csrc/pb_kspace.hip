@@ -523,7 +523,8 @@ __global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, int nblk8,
 }
 
 // Grouped: blocks renumbered per XCD before the map lookup; the gather adds
-// to u with atomics (the pair kernel may run at the same time).
+// to u with atomics (acc_mode 2: the pair kernel may run at the same time)
+// or overwrites it (0).
 template <typename T, int P>
 __global__ __launch_bounds__(64 * kInterpWaves) void k_ginterp(const pb_sys *__restrict__ sys,
                                                              const int2 *__restrict__ map, int nblk8)
@@ -532,7 +533,7 @@ __global__ __launch_bounds__(64 * kInterpWaves) void k_ginterp(const pb_sys *__r
     if (m.x < 0) return;
     const pb_sys &S = sys[m.x];
     interp_target<T, P>(m.y * kInterpWaves + (int)(threadIdx.x >> 6), S.nt, S.tpos, S.tperm, S.g,
-                        (const trip<T> *)S.grid4, S.u, 2);
+                        (const trip<T> *)S.grid4, S.u, S.acc_mode);
 }
 
 // Per-axis window sizes that differ (fftBrinkman's P[3]) take this path.
@@ -599,7 +600,8 @@ __global__ __launch_bounds__(256) void k_ginterp_any(const pb_sys *__restrict__ 
     const int2 m = map[blockIdx.x];
     if (m.x < 0) return;
     const pb_sys &S = sys[m.x];
-    interp_any_target<T>(m.y * 256 + (int)threadIdx.x, S.nt, S.tpos, S.tperm, S.g, (const trip<T> *)S.grid4, S.u, 2);
+    interp_any_target<T>(m.y * 256 + (int)threadIdx.x, S.nt, S.tpos, S.tperm, S.g, (const trip<T> *)S.grid4, S.u,
+                         S.acc_mode);
 }
 
 template <typename T>

@@ -277,7 +277,25 @@ __global__ void k_force_set(int ns, int r, int RP, const float *__restrict__ Fs,
     sfrc[j] = make_double4((double)p[0], (double)p[RP], (double)p[2 * RP], 0.0);
 }
 
+__global__ void k_force_sets(int ns, int r0, int nr, int RP, const float *__restrict__ Fs, double4 *__restrict__ frc)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)ns * nr) return;
+    const int r = (int)(q / ns), j = (int)(q % ns);
+    const float *p = Fs + (size_t)j * 3 * RP + r0 + r;
+    frc[q] = make_double4((double)p[0], (double)p[RP], (double)p[2 * RP], 0.0);
+}
+
 } // namespace
+
+int pb_force_sets(pb_context *c, hipStream_t s, int ns, int r0, int nr, int RP, double4 *frc)
+{
+    const long long n = (long long)ns * nr;
+    if (n > 0)
+        hipLaunchKernelGGL(k_force_sets, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ns, r0, nr, RP,
+                           (const float *)c->mrhs_f.ptr, frc);
+    return pb_hip_check(c, hipGetLastError(), "force sets launch");
+}
 
 int pb_sort_forces(pb_context *c, hipStream_t s, int ns, int R, int RP, const float *f)
 {
