@@ -304,6 +304,8 @@ def main():
                     help="targets in the CPU real-space sample (all 1e5 sources x 27 images each)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of known size (for HBM counter calibration)")
+    ap.add_argument("--only", choices=("batched", "clustered", "multirhs"),
+                    help="run one extra leg alone and print its JSON (for profiling it)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -315,6 +317,14 @@ def main():
     import torch
 
     import periodicbrinkman_amd as pb
+
+    if args.only:
+        ctx = pb.Context(0)
+        leg = {"batched": lambda: batched_leg(ctx, args.steps, args.warmup),
+               "clustered": lambda: clustered_leg(ctx, 3), "multirhs": lambda: multirhs_leg(ctx, 3)}[args.only]
+        print(json.dumps({args.only + "_form": leg()}))
+        ctx.close()
+        return
 
     x, f, y = pb.uniform_system(pb.config_seed(1), NS, NT)
     dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda:0")
