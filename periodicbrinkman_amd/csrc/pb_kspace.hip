@@ -854,29 +854,38 @@ __device__ __forceinline__ void fft_y(const int blk, typename cplx<T>::type *__r
     __shared__ C buf[B * NP];
     __shared__ C tw[N];
     load_twiddles<C, N>(tw, gtw);
-    const long long npen = 3LL * n0 * h2;
-    const long long q0 = (long long)blk * B;
-    // pencil q = (f i0) * h2 + kz; element i1 at ((f i0) * N + i1) * h2 + kz
-    for (int idx = threadIdx.x; idx < E; idx += 256) {
-        const int i1 = idx / B, g = idx % B;
-        const long long q = q0 + g;
+    const int npen = 3 * n0 * h2;
+    const int q0 = blk * B;
+    // pencil q = (f i0) * h2 + kz; element i1 at ((f i0) * N + i1) * h2 + kz.
+    // Element idx of the workgroup is (i1, g) = (idx / B, idx % B): a thread
+    // meets G distinct pencils g (one when B divides 256), whose offsets it
+    // computes once.
+    static_assert(E % 256 == 0 && (B % 256 == 0 || 256 % B == 0), "y-pass workgroup shape");
+    constexpr int G = B >= 256 ? B / 256 : 1;
+    size_t base[G];
+    bool ok[G];
+#pragma unroll
+    for (int c = 0; c < G; c++) {
+        const int g = B >= 256 ? (int)threadIdx.x + 256 * c : (int)threadIdx.x % B;
+        const int q = q0 + g;
+        ok[c] = q < npen;
+        const int fi = q / h2, kz = q - fi * h2;
+        base[c] = (size_t)fi * N * h2 + kz;
+    }
+#pragma unroll
+    for (int m = 0; m < E / 256; m++) {
+        const int idx = (int)threadIdx.x + 256 * m, i1 = idx / B, g = idx % B, c = B >= 256 ? m % G : 0;
         C v;
         v.x = v.y = (T)0;
-        if (q < npen) {
-            const long long fi = q / h2, kz = q % h2;
-            v = S[((size_t)fi * N + i1) * h2 + kz];
-        }
+        if (ok[c]) v = S[base[c] + (size_t)i1 * h2];
         buf[g * NP + i1] = v;
     }
     __syncthreads();
     fft_stages<C, N, 1, E>(buf, inverse != 0, tw);
-    for (int idx = threadIdx.x; idx < E; idx += 256) {
-        const int i1 = idx / B, g = idx % B;
-        const long long q = q0 + g;
-        if (q < npen) {
-            const long long fi = q / h2, kz = q % h2;
-            S[((size_t)fi * N + i1) * h2 + kz] = buf[g * NP + i1];
-        }
+#pragma unroll
+    for (int m = 0; m < E / 256; m++) {
+        const int idx = (int)threadIdx.x + 256 * m, i1 = idx / B, g = idx % B, c = B >= 256 ? m % G : 0;
+        if (ok[c]) S[base[c] + (size_t)i1 * h2] = buf[g * NP + i1];
     }
 }
 
@@ -920,7 +929,7 @@ __device__ __forceinline__ void fft_x_mult(const int blk, typename cplx<T>::type
         const int g = idx / N, i0 = idx % N;
         const long long q = q0 + g;
         if (q >= npen) continue;
-        const int i1 = (int)(q / h2), kz = (int)(q % h2);
+        const int i1 = (int)q / h2, kz = (int)q - i1 * h2;
         const int n2 = km.n[2];
         const int f0 = (2 * i0 >= N) ? i0 - N : i0;
         const int f1 = (2 * i1 >= n1) ? i1 - n1 : i1;
