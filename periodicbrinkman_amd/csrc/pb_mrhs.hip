@@ -22,6 +22,12 @@ namespace {
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kMWaves = 4;
+#ifndef MRHS_Q2
+#define MRHS_Q2 4 // sources per lane parity and contraction step, 64 sets per launch
+#endif
+#ifndef MRHS_Q1
+#define MRHS_Q1 8 // the same, 32 sets per launch
+#endif
 
 struct mrhs_lds {
     double4 ring[128]; // kept candidate sources: image position, index (as bits)
@@ -60,9 +66,9 @@ __device__ __forceinline__ int floordiv(int a, int b)
     return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
 }
 
-// Contract the 16 ring entries from head (cnt <= 16 of them valid) into acc.
-template <int NB, int DEG>
-__device__ __forceinline__ void contract16(const mrhs_lds &W, int head, int cnt, int lane, double y0, double y1,
+// Contract the 2Q ring entries from head (cnt <= 2Q of them valid) into acc.
+template <int Q, int NB, int DEG>
+__device__ __forceinline__ void contract(const mrhs_lds &W, int head, int cnt, int lane, double y0, double y1,
                                            double y2, bool active, double rc2, double rcommon,
                                            const pbdev::tabview &tv, const double2 *__restrict__ tab,
                                            const pb_kparams *kpp, const float *__restrict__ Fs, int RP, int c0,
@@ -70,10 +76,10 @@ __device__ __forceinline__ void contract16(const mrhs_lds &W, int head, int cnt,
 {
     const int t = lane & 31, kh = lane >> 5;
     (void)t;
-    float H1[8], H2[8], dd[8][3];
-    int src[8];
+    float H1[Q], H2[Q], dd[Q][3];
+    int src[Q];
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < Q; q++) {
         const int s = 2 * q + kh;
         const bool valid = s < cnt;
         const double4 sc = W.ring[(head + (valid ? s : 0)) & 127];
@@ -99,8 +105,8 @@ __device__ __forceinline__ void contract16(const mrhs_lds &W, int head, int cnt,
         src[q] = valid ? (int)__double_as_longlong(sc.w) : -1;
     }
 #pragma unroll
-    for (int kk = 0; kk < 24; kk++) {
-        const int b = kk >> 3, q = kk & 7;
+    for (int kk = 0; kk < 3 * Q; kk++) {
+        const int b = kk / Q, q = kk % Q;
         float bv[NB];
 #pragma unroll
         for (int nb = 0; nb < NB; nb++)
@@ -115,7 +121,7 @@ __device__ __forceinline__ void contract16(const mrhs_lds &W, int head, int cnt,
     }
 }
 
-template <int NB, int DEG>
+template <int Q, int NB, int DEG>
 __global__ __launch_bounds__(64 * kMWaves, 2) void k_pair_mrhs(
     const int2 *__restrict__ tiles, const int *__restrict__ ntiles_ptr, const int *__restrict__ cstart,
     const double4 *__restrict__ spos, const float *__restrict__ Fs, int RP, int c0, int R,
@@ -218,11 +224,11 @@ __global__ __launch_bounds__(64 * kMWaves, 2) void k_pair_mrhs(
                             make_double4(sx, sy, sz, __longlong_as_double((long long)(j + lane)));
                     pend += __popcll(km);
                     __builtin_amdgcn_wave_barrier();
-                    while (pend >= 16) {
-                        contract16<NB, DEG>(W, head, 16, lane, y0, y1, y2, active, rc2, rcommon, tv, tab, kpp, Fs, RP,
-                                       c0, acc);
-                        head = (head + 16) & 127;
-                        pend -= 16;
+                    while (pend >= 2 * Q) {
+                        contract<Q, NB, DEG>(W, head, 2 * Q, lane, y0, y1, y2, active, rc2, rcommon, tv, tab, kpp, Fs,
+                                             RP, c0, acc);
+                        head = (head + 2 * Q) & 127;
+                        pend -= 2 * Q;
                     }
                 }
                 if (czend >= cze) break;
@@ -236,7 +242,7 @@ __global__ __launch_bounds__(64 * kMWaves, 2) void k_pair_mrhs(
     }
     __builtin_amdgcn_wave_barrier();
     if (pend > 0)
-        contract16<NB, DEG>(W, head, pend, lane, y0, y1, y2, active, rc2, rcommon, tv, tab, kpp, Fs, RP, c0, acc);
+        contract<Q, NB, DEG>(W, head, pend, lane, y0, y1, y2, active, rc2, rcommon, tv, tab, kpp, Fs, RP, c0, acc);
     // C[i][j]: column j = lane & 31, row i = 8 (v / 4) + 4 (lane / 32) + v % 4
     const int kh = lane >> 5;
 #pragma unroll
@@ -326,15 +332,17 @@ int pb_run_mrhs_real(pb_context *c, hipStream_t s, const pb_cells &cells, const 
     const int *ntiles = (const int *)c->qoff.ptr + cells.nkey_t / 16;
     const unsigned blocks = (unsigned)((2LL * c->max_tiles + kMWaves - 1) / kMWaves);
     const size_t lds = table_doubles * sizeof(double);
-    // 32 right-hand sides per launch: one 32 x 32 accumulator block per
-    // component stays in registers without spilling
+    // 64 right-hand sides per launch (two 32 x 32 accumulator blocks per
+    // component, 8 sources per contraction step), a last 32 with one block
     if (table_deg != PB_TAB_DEG && table_deg != PB_TAB_DEG_MIXED)
         return pb_fail(c, PB_ERR_INVALID, "radial table of an unsupported degree");
-    auto kern = table_deg == PB_TAB_DEG ? k_pair_mrhs<1, PB_TAB_DEG> : k_pair_mrhs<1, PB_TAB_DEG_MIXED>;
-    for (int c0 = 0; c0 < RP; c0 += 32)
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kMWaves), lds, s, (const int2 *)c->tiles.ptr,
-                           ntiles, (const int *)c->cell_start.ptr, (const double4 *)c->spos.ptr,
-                           (const float *)c->mrhs_f.ptr, RP, c0, R, (const double4 *)c->tpos.ptr,
-                           (const int *)c->tperm.ptr, nt, cells, box, kp, u);
+    const bool hi = table_deg == PB_TAB_DEG;
+    auto k64 = hi ? k_pair_mrhs<MRHS_Q2, 2, PB_TAB_DEG> : k_pair_mrhs<MRHS_Q2, 2, PB_TAB_DEG_MIXED>;
+    auto k32 = hi ? k_pair_mrhs<MRHS_Q1, 1, PB_TAB_DEG> : k_pair_mrhs<MRHS_Q1, 1, PB_TAB_DEG_MIXED>;
+    for (int c0 = 0; c0 < RP; c0 += 64)
+        hipLaunchKernelGGL(RP - c0 >= 64 ? k64 : k32, dim3(blocks), dim3(64 * kMWaves), lds, s,
+                           (const int2 *)c->tiles.ptr, ntiles, (const int *)c->cell_start.ptr,
+                           (const double4 *)c->spos.ptr, (const float *)c->mrhs_f.ptr, RP, c0, R,
+                           (const double4 *)c->tpos.ptr, (const int *)c->tperm.ptr, nt, cells, box, kp, u);
     return pb_hip_check(c, hipGetLastError(), "multi-RHS pair launch");
 }

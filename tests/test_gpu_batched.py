@@ -132,3 +132,36 @@ def test_batched_grouped_and_single_systems_mixed(precision, tol):
     assert relL2(u3, u) < (1e-12 if precision == "fp64" else 1e-6)
     assert stats[0] + stats[1] > 0
     ctx.close()
+
+
+def test_batched_group_edge_systems():
+    # the group with systems that have no sources or no targets, FP64 and
+    # mixed systems side by side, and one target: every system against the
+    # oracle (a system without sources has zero velocity)
+    import torch
+    sizes = [(0, 50), (300, 0), (1, 1), (500, 400), (0, 0), (200, 1), (350, 280)]
+    precs = ["fp64", "mixed", "fp64", "mixed", "fp64", "fp64", "mixed"]
+    params, xs, fs, ys = [], [], [], []
+    for i, ((ns, nt), pr) in enumerate(zip(sizes, precs)):
+        L = (0.8 + 0.1 * i, 1.0, 1.2)
+        x, f, y = pb.uniform_system(pb.config_seed(700 + i), ns, nt, L)
+        params.append(pb.Params(L=L, alpha=3.0 + 2 * i, eps=1e-3, precision=pr,
+                                tol=1e-8 if pr == "fp64" else 1e-7))
+        xs.append(x.reshape(-1, 3)), fs.append(f.reshape(-1, 3)), ys.append(y.reshape(-1, 3))
+    so = np.concatenate([[0], np.cumsum([s[0] for s in sizes])])
+    to = np.concatenate([[0], np.cumsum([s[1] for s in sizes])])
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+    ctx = pb.Context(0)
+    X, F, Y = dev(np.concatenate(xs)), dev(np.concatenate(fs)), dev(np.concatenate(ys))
+    for call in range(3):  # eager, captured, replayed
+        u = ctx.velocity_batched(params, X, F, so, Y, to).cpu().numpy()
+        for i, q in enumerate(params):
+            ui = u[to[i]:to[i + 1]]
+            if sizes[i][1] == 0:
+                continue
+            if sizes[i][0] == 0:
+                assert np.all(ui == 0), i
+                continue
+            want = po.velocity_ewald(ys[i], xs[i], fs[i], q.L, q.alpha, q.d, q.eps, q.rc, q.grid, (q.P,) * 3, q.m)
+            assert relL2(ui, want) < (1e-10 if precs[i] == "fp64" else 1e-6), (call, i, relL2(ui, want))
+    ctx.close()
