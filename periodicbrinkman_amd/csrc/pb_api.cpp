@@ -583,7 +583,8 @@ int mrhs_kspace(pb_context *c, hipStream_t s, const pb_params *p, const pb_cells
     const bool same = g.P[0] == g.P[1] && g.P[1] == g.P[2];
     const int Ps[8] = {4, 6, 8, 10, 12, 16, 20, 24};
     const int P = same && std::find(Ps, Ps + 8, g.P[0]) != Ps + 8 ? g.P[0] : 0; // 0: per-axis windows
-    per[6] = P ? (nt + pb_interp_waves - 1) / pb_interp_waves : (nt + 255) / 256;
+    const int tpb = P ? pb_interp_waves * pb_interp_tw(P) : 256; // targets per gather block
+    per[6] = (nt + tpb - 1) / tpb;
     size_t off[8] = {0};
     for (int k = 0; k < 7; k++) {
         size_t n = (size_t)per[k] * nc;
@@ -838,7 +839,7 @@ int group_prepare(pb_context *c, hipStream_t s, const std::vector<int> &idx, con
                     const bool same = g.P[0] == g.P[1] && g.P[1] == g.P[2];
                     const int Pq = same && std::find(Ps, Ps + 8, g.P[0]) != Ps + 8 ? g.P[0] : 0;
                     if (prec_of(q) != pr || Pq != P || sys[q].nt == 0) continue;
-                    const int per = P ? pb_interp_waves : 256;
+                    const int per = P ? pb_interp_waves * pb_interp_tw(P) : 256;
                     for (int b = 0; b * per < sys[q].nt; b++) m.push_back(make_int2(q, b));
                 }
                 if (P) // the gather renumbers blocks per XCD: a multiple of 8

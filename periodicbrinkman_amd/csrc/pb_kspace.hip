@@ -445,68 +445,92 @@ template <typename T> __device__ __forceinline__ T wave_sum(T v)
     return v;
 }
 
+// The gather: one wavefront per TW = pb_interp_tw(P) consecutive targets.
+// Lane (target w, py, pz) of the TW P^2 window columns sums its P points along
+// x; with P^2 not a multiple of 64, TW = 4 targets fill the rounds of 64
+// lanes (P = 12: 9 rounds instead of 4 x 3 with a quarter of the lanes idle).
+// For P >= 8 a round spans at most two targets, so a lane's partial sums
+// go to one of two accumulators fixed at compile time.
 template <typename T, int P>
-__device__ __forceinline__ void interp_target(const int i, int nt, const double4 *__restrict__ tpos,
-                                              const int *__restrict__ tperm, const pb_kgrid &g,
-                                              const trip<T> *__restrict__ U3, double *__restrict__ u,
-                                              int accumulate)
+__device__ __forceinline__ void interp_wave(const int i0, int nt, const double4 *__restrict__ tpos,
+                                            const int *__restrict__ tperm, const pb_kgrid &g,
+                                            const trip<T> *__restrict__ U3, double *__restrict__ u, int accumulate)
 {
-    constexpr int NQ = P * P, NR = (NQ + 63) / 64;
-    __shared__ T wts[kInterpWaves][3][P];
-    __shared__ int off[kInterpWaves][3][P]; // wrapped index times its stride
+    constexpr int TW = pb_interp_tw(P), NQ = P * P, NR = (TW * NQ + 63) / 64;
+    static_assert(TW == 1 || NQ >= 64, "a round spans at most two targets");
+    __shared__ T wts[kInterpWaves][TW][3][P];
+    __shared__ int off[kInterpWaves][TW][3][P]; // wrapped index times its stride
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (i >= nt) return; // whole wavefront; no block barriers below
-    const double4 y = tpos[i];
-    for (int k = lane; k < 3 * P; k += 64) {
-        const int a = k / P, p = k % P;
+    if (i0 >= nt) return; // whole wavefront; no block barriers below
+    const int nv = min(TW, nt - i0); // targets of this wavefront (the others repeat the last)
+    for (int k = lane; k < TW * 3 * P; k += 64) {
+        const int w = k / (3 * P), a = (k / P) % 3, p = k % P;
+        const double4 y = tpos[i0 + min(w, nv - 1)];
         const double yc = a == 0 ? y.x : (a == 1 ? y.y : y.z);
         const int f0 = (int)floor(yc * g.inv_h[a] - 0.5 * P) + 1;
         const double dx = (double)(f0 + p) * g.h[a] - yc;
         if (sizeof(T) == 8)
-            wts[wave][a][p] = (T)exp(-dx * dx / g.eta);
+            wts[wave][w][a][p] = (T)exp(-dx * dx / g.eta);
         else
-            wts[wave][a][p] = (T)__expf(-(float)(dx * dx) * (float)(1.0 / g.eta));
+            wts[wave][w][a][p] = (T)__expf(-(float)(dx * dx) * (float)(1.0 / g.eta));
         int j = f0 + p;
         j = j < 0 ? j + g.n[a] : (j >= g.n[a] ? j - g.n[a] : j);
-        off[wave][a][p] = a == 0 ? j * g.n[1] * g.n[2] : (a == 1 ? j * g.n[2] : j);
+        off[wave][w][a][p] = a == 0 ? j * g.n[1] * g.n[2] : (a == 1 ? j * g.n[2] : j);
     }
     __builtin_amdgcn_wave_barrier();
-    T a0 = 0, a1 = 0, a2 = 0;
+    T acc[TW][3];
+#pragma unroll
+    for (int w = 0; w < TW; w++) acc[w][0] = acc[w][1] = acc[w][2] = (T)0;
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         const int q = r * 64 + lane;
-        if (NQ % 64 == 0 || q < NQ) {
-            const int py = q / P, pz = q % P;
-            const T wyz = wts[wave][1][py] * wts[wave][2][pz];
-            const int oyz = off[wave][1][py] + off[wave][2][pz];
+        const int wlo = (r * 64) / NQ;                   // first target of the round (compile time)
+        const int whi = wlo + 1 < TW ? wlo + 1 : wlo;    // the next one, if any
+        if ((TW * NQ) % 64 == 0 || q < TW * NQ) {
+            const bool hi = q >= (wlo + 1) * NQ;
+            const int w = hi ? whi : wlo, qq = q - w * NQ;
+            const int py = qq / P, pz = qq % P;
+            const T wyz = wts[wave][w][1][py] * wts[wave][w][2][pz];
+            const int oyz = off[wave][w][1][py] + off[wave][w][2][pz];
+            T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
             for (int px = 0; px < P; px++) {
-                const T wq = wts[wave][0][px] * wyz;
-                const trip<T> v = U3[(unsigned)(off[wave][0][px] + oyz)];
-                a0 += wq * v.x;
-                a1 += wq * v.y;
-                a2 += wq * v.z;
+                const T wq = wts[wave][w][0][px] * wyz;
+                const trip<T> v = U3[(unsigned)(off[wave][w][0][px] + oyz)];
+                s0 += wq * v.x;
+                s1 += wq * v.y;
+                s2 += wq * v.z;
+            }
+            if (hi) {
+                acc[whi][0] += s0;
+                acc[whi][1] += s1;
+                acc[whi][2] += s2;
+            } else {
+                acc[wlo][0] += s0;
+                acc[wlo][1] += s1;
+                acc[wlo][2] += s2;
             }
         }
     }
-    a0 = wave_sum(a0);
-    a1 = wave_sum(a1);
-    a2 = wave_sum(a2);
-    if (lane == 0) {
-        const double sc = g.scale * g.vol;
-        const size_t o = 3 * (size_t)tperm[i];
-        if (accumulate == 2) { // the pair kernel may be adding to u at the same time
-            atomicAdd(&u[o], sc * (double)a0);
-            atomicAdd(&u[o + 1], sc * (double)a1);
-            atomicAdd(&u[o + 2], sc * (double)a2);
-        } else if (accumulate) {
-            u[o] += sc * (double)a0;
-            u[o + 1] += sc * (double)a1;
-            u[o + 2] += sc * (double)a2;
-        } else {
-            u[o] = sc * (double)a0;
-            u[o + 1] = sc * (double)a1;
-            u[o + 2] = sc * (double)a2;
+    const double sc = g.scale * g.vol;
+#pragma unroll
+    for (int w = 0; w < TW; w++) {
+        const T a0 = wave_sum(acc[w][0]), a1 = wave_sum(acc[w][1]), a2 = wave_sum(acc[w][2]);
+        if (lane == 0 && w < nv) {
+            const size_t o = 3 * (size_t)tperm[i0 + w];
+            if (accumulate == 2) { // the pair kernel may be adding to u at the same time
+                atomicAdd(&u[o], sc * (double)a0);
+                atomicAdd(&u[o + 1], sc * (double)a1);
+                atomicAdd(&u[o + 2], sc * (double)a2);
+            } else if (accumulate) {
+                u[o] += sc * (double)a0;
+                u[o + 1] += sc * (double)a1;
+                u[o + 2] += sc * (double)a2;
+            } else {
+                u[o] = sc * (double)a0;
+                u[o + 1] = sc * (double)a1;
+                u[o + 2] = sc * (double)a2;
+            }
         }
     }
 }
@@ -518,8 +542,9 @@ __global__ __launch_bounds__(64 * kInterpWaves) void k_interp(int nt, int nblk8,
                                                             const trip<T> *__restrict__ U3,
                                                             double *__restrict__ u, int accumulate)
 {
-    interp_target<T, P>(pb_xcd_block(blockIdx.x, nblk8) * kInterpWaves + (int)(threadIdx.x >> 6), nt, tpos, tperm,
-                        g, U3, u, accumulate);
+    constexpr int TW = pb_interp_tw(P);
+    interp_wave<T, P>((pb_xcd_block(blockIdx.x, nblk8) * kInterpWaves + (int)(threadIdx.x >> 6)) * TW, nt, tpos,
+                      tperm, g, U3, u, accumulate);
 }
 
 // Grouped: blocks renumbered per XCD before the map lookup; the gather adds
@@ -529,11 +554,12 @@ template <typename T, int P>
 __global__ __launch_bounds__(64 * kInterpWaves) void k_ginterp(const pb_sys *__restrict__ sys,
                                                              const int2 *__restrict__ map, int nblk8)
 {
+    constexpr int TW = pb_interp_tw(P);
     const int2 m = map[pb_xcd_block(blockIdx.x, nblk8)];
     if (m.x < 0) return;
     const pb_sys &S = sys[m.x];
-    interp_target<T, P>(m.y * kInterpWaves + (int)(threadIdx.x >> 6), S.nt, S.tpos, S.tperm, S.g,
-                        (const trip<T> *)S.grid4, S.u, S.acc_mode);
+    interp_wave<T, P>((m.y * kInterpWaves + (int)(threadIdx.x >> 6)) * TW, S.nt, S.tpos, S.tperm, S.g,
+                      (const trip<T> *)S.grid4, S.u, S.acc_mode);
 }
 
 // Per-axis window sizes that differ (fftBrinkman's P[3]) take this path.
@@ -608,12 +634,13 @@ template <typename T>
 void launch_interp(hipStream_t s, int nt, const pb_kgrid &g, const void *tpos, const void *tperm,
                    const trip<T> *U3, double *u, int acc)
 {
-    const int nblk8 = ((nt + kInterpWaves - 1) / kInterpWaves + 7) & ~7;
+    const bool same = g.P[0] == g.P[1] && g.P[1] == g.P[2];
+    const int tpb = kInterpWaves * (same ? pb_interp_tw(g.P[0]) : 1); // targets per block
+    const int nblk8 = ((nt + tpb - 1) / tpb + 7) & ~7;
     dim3 gd(nblk8), bd(64 * kInterpWaves);
     dim3 gd1((nt + 255) / 256), bd1(256);
     const double4 *tp = (const double4 *)tpos;
     const int *pm = (const int *)tperm;
-    bool same = g.P[0] == g.P[1] && g.P[1] == g.P[2];
 #define PB_INTERP(PP) hipLaunchKernelGGL((k_interp<T, PP>), gd, bd, 0, s, nt, nblk8, tp, pm, g, U3, u, acc)
     switch (same ? g.P[0] : -1) {
     case 4: PB_INTERP(4); break;
