@@ -193,8 +193,9 @@ __global__ void k_write_tiles(int nq, const int *__restrict__ tstart, const int 
 
 // ---- fused binning of sources and targets (small key spaces) -----------------
 // Four launches instead of seventeen: one fill, one count and one scatter
-// over both point sets, and one workgroup that scans both count arrays and
-// writes the target tiles.  Used when ncell + nkey_t <= kFusedKeys.
+// over both point sets, and one scan launch of two workgroups (one per count
+// array; the target one also writes the tiles).  Used when ncell + nkey_t <=
+// kFusedKeys.
 constexpr int kFusedKeys = 1 << 18;
 
 __global__ void k_fill_int2(int n0, int *__restrict__ a0, int n1, int *__restrict__ a1)
@@ -318,9 +319,8 @@ __device__ void scan_counts(const int *__restrict__ count, int n, int *__restric
                             int2 *__restrict__ tiles, int *__restrict__ ntiles)
 {
     int carry = 0, tcarry = 0;
-    for (int base = 0; base < n; base += 16 * 1024) {
-        const int i0 = base + 16 * (int)threadIdx.x;
-        int v[16];
+    // the counts of the next round are loaded while this round is scanned
+    auto load16 = [&](int i0, int (&v)[16]) {
         if (i0 + 16 <= n) {
             const int4 *c4 = reinterpret_cast<const int4 *>(count + i0);
 #pragma unroll
@@ -335,6 +335,15 @@ __device__ void scan_counts(const int *__restrict__ count, int n, int *__restric
 #pragma unroll
             for (int q = 0; q < 16; q++) v[q] = i0 + q < n ? count[i0 + q] : 0;
         }
+    };
+    int vn[16];
+    load16(16 * (int)threadIdx.x, vn);
+    for (int base = 0; base < n; base += 16 * 1024) {
+        const int i0 = base + 16 * (int)threadIdx.x;
+        int v[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) v[q] = vn[q];
+        if (base + 16 * 1024 < n) load16(i0 + 16 * 1024, vn);
         int sum = 0;
 #pragma unroll
         for (int q = 0; q < 16; q++) sum += v[q];
@@ -387,8 +396,10 @@ __global__ __launch_bounds__(1024) void k_bin_scan2(const int *__restrict__ scou
                                                     int2 *__restrict__ tiles, int *__restrict__ ntiles)
 {
     __shared__ int wsum[16];
-    scan_counts(scount, ncell, sstart, wsum, nullptr, nullptr);
-    scan_counts(tcount, nkey_t, tstart, wsum, tiles, ntiles);
+    if (blockIdx.x == 0) // the two scans run side by side, one workgroup each
+        scan_counts(scount, ncell, sstart, wsum, nullptr, nullptr);
+    else
+        scan_counts(tcount, nkey_t, tstart, wsum, tiles, ntiles);
 }
 
 
@@ -556,7 +567,7 @@ int pb_bin_all(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_box
         hipLaunchKernelGGL(k_bin_count2, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, ns, xn, nt, y, box,
                            cells, scount, (int *)c->cell_of_s.ptr, (int *)c->rank_s.ptr, tcount,
                            (int *)c->cell_of_t.ptr, (int *)c->rank_t.ptr);
-    hipLaunchKernelGGL(k_bin_scan2, dim3(1), dim3(1024), 0, s, (const int *)scount, ncell, (int *)c->cell_start.ptr,
+    hipLaunchKernelGGL(k_bin_scan2, dim3(2), dim3(1024), 0, s, (const int *)scount, ncell, (int *)c->cell_start.ptr,
                        (const int *)tcount, nkt, (int *)c->tcell_start.ptr, (int2 *)c->tiles.ptr,
                        (int *)c->qoff.ptr + nq);
     if (npts > 0)
