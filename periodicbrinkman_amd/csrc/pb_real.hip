@@ -33,7 +33,7 @@
 namespace {
 
 constexpr int kWaves = 4; // wavefronts per workgroup, each independent
-static_assert(kWaves == pb_pair_waves, "pair workgroup size");
+static_assert(kWaves == pb_pair_waves && kWaves == 4, "pair workgroup size");
 constexpr int kQ = 128;   // ring capacity (power of two)
 
 
@@ -251,7 +251,8 @@ __device__ __forceinline__ void pair_body(
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long long item = (long long)blk * kWaves + wave;
     const int tile = (int)(item / nsplit), split = (int)(item % nsplit);
-    if (tile >= *ntiles_ptr) { // whole wavefront leaves; no block barriers below
+    if (tile >= *ntiles_ptr) { // whole wavefront leaves (with nsplit = kWaves the whole workgroup: the only
+                               // block barrier, at the end, is taken only then)
         if (lane == 0) stats[item] = make_uint4(0u, 0u, 0u, 0u);
         return;
     }
@@ -431,7 +432,18 @@ __device__ __forceinline__ void pair_body(
     }
     flush(true);
     __builtin_amdgcn_wave_barrier();
-    if (active && (n64 | n32)) {
+    if (nsplit == kWaves) {
+        // the workgroup's wavefronts share the tile (and left together above
+        // when there was none): one of them adds the four partial sums to u
+        __syncthreads();
+        if (wave == 0 && active) {
+            const size_t o = 3 * (size_t)tperm[t0 + lane];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                atomicAdd(&u[o + c], (lds[0].acc[lane][c] + lds[1].acc[lane][c]) +
+                                         (lds[2].acc[lane][c] + lds[3].acc[lane][c]));
+        }
+    } else if (active && (n64 | n32)) {
         const size_t o = 3 * (size_t)tperm[t0 + lane];
         atomicAdd(&u[o + 0], W.acc[lane][0]);
         atomicAdd(&u[o + 1], W.acc[lane][1]);
