@@ -608,7 +608,11 @@ int mrhs_kspace(pb_context *c, hipStream_t s, const pb_params *p, const pb_cells
         }
         const pb_sys *sd = (const pb_sys *)c->mr_sys.ptr + r0;
         if ((st = pb_force_sets(c, s, ns, r0, n, RP, (double4 *)c->mr_frc.ptr))) return st;
-        pb_group_spread(s, sd, map + off[0], per[0] * n, PB_PREC_MIXED);
+        // spreading: groups of pb_spread_sets sets per workgroup (one staging
+        // and one set of window weights for all of them), then the rest one by one
+        const int n8 = n / pb_spread_sets, rest = n - n8 * pb_spread_sets;
+        pb_group_spread_sets(s, sd, map + off[0], per[0] * n8);
+        if (rest) pb_group_spread(s, sd + n8 * pb_spread_sets, map + off[0], per[0] * rest, PB_PREC_MIXED);
         for (int pass = 0; pass < 5; pass++) {
             const int ax = (pass == 0 || pass == 4) ? 2 : (pass == 2 ? 0 : 1);
             if ((st = pb_group_fft(s, sd, map + off[1 + pass], per[1 + pass] * n, PB_PREC_MIXED, pass, g.n[ax], tw)))

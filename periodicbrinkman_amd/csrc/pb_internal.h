@@ -313,6 +313,10 @@ int pb_group_pair(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, i
 // every axis, or P = 0 for per-axis windows; map over blocks of
 // 4 targets, XCD-renumbered: nblk a multiple of 8, entries (-1, 0) idle).
 void pb_group_spread(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, int precision);
+// FP32 spreading of pb_spread_sets force sets per workgroup that share the
+// particles (consecutive descriptors): map entries (set group, region)
+constexpr int pb_spread_sets = 8;
+void pb_group_spread_sets(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk);
 int pb_group_fft(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, int precision, int pass, int n,
                  const void *twiddles);
 int pb_group_interp(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, int precision, int P);

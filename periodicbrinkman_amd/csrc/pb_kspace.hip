@@ -85,8 +85,10 @@ template <> struct alignas(16) spread_batch_lds<float> {
     alignas(16) float P[kSpreadWaves][16][16]; // wy[j] wz[k] of source h at 4 j + k
 };
 
-template <typename T> struct spread_lds : spread_batch_lds<T> {
-    static constexpr int CAP = spread_cfg<T>::CAP, HB = spread_cfg<T>::HB;
+// NS > 1 (FP32 only): NS force sets spread together from one staging of the
+// sources (the many-right-hand-side k space), with a smaller staging capacity.
+template <typename T, int NS = 1> struct spread_lds : spread_batch_lds<T> {
+    static constexpr int CAP = NS == 1 ? spread_cfg<T>::CAP : 224, HB = spread_cfg<T>::HB;
     int run_j[256];       // first source of each candidate run of the chunk
     int run_pre[257];     // exclusive prefix of the run lengths
     unsigned run_code[256]; // image shifts (l0 + 4) | (l1 + 4) << 4 | (l2 + 4) << 8
@@ -94,7 +96,7 @@ template <typename T> struct spread_lds : spread_batch_lds<T> {
     double sx[3][CAP];    // staged sources: wrapped position, shifted window start,
     int sf0[3][CAP];      // image code and scaled force
     unsigned scode[CAP];
-    T sF[3][CAP];
+    T sF[3 * NS][CAP];
     int ring[kSpreadWaves][128];    // staged indices touching the wavefront's sub-block
                                     // (< HB pending + 64 appended)
 };
@@ -163,8 +165,9 @@ __device__ __forceinline__ void spread_steps16(const float (&P)[16][16], int jk,
 // (zero outside the window or for h >= cnt); it keeps wx F, publishes wz and
 // forms P[h][4 i + k] = wy[i] wz[k].  The lane at point (i, j, k) then adds,
 // for each source, P[h][4 j + k] times wx F of lane h of its own row.
-__device__ __forceinline__ void spread_batch16(spread_lds<float> &S, int wave, int lane, int head, int cnt,
-                                               const int *sb, const pb_kgrid &g, float &a0, float &a1, float &a2)
+template <int NS>
+__device__ __forceinline__ void spread_batch16(spread_lds<float, NS> &S, int wave, int lane, int head, int cnt,
+                                               const int *sb, const pb_kgrid &g, float (&acc)[NS][3])
 {
     const int i = lane >> 4, h = lane & 15;
     __builtin_amdgcn_wave_barrier();
@@ -182,7 +185,6 @@ __device__ __forceinline__ void spread_batch16(spread_lds<float> &S, int wave, i
         }
         w[a] = v;
     }
-    const float f0 = w[0] * S.sF[0][e], f1 = w[0] * S.sF[1][e], f2 = w[0] * S.sF[2][e];
     S.Wz[wave][h][i] = w[2];
     __builtin_amdgcn_wave_barrier();
     const float4 z = *reinterpret_cast<const float4 *>(&S.Wz[wave][h][0]);
@@ -199,29 +201,39 @@ __device__ __forceinline__ void spread_batch16(spread_lds<float> &S, int wave, i
     for (int q = 0; q < 16; q++) p[q] = S.P[wave][q][lane & 15];
 #define PB_FD(H, A, F, Q) "v_fmac_f32_dpp %" #A ", %" #F ", %" #Q " row_newbcast:" #H " row_mask:0xf bank_mask:0xf\n"
 #define PB_F3(H, Q) PB_FD(H, 0, 3, Q) PB_FD(H, 1, 4, Q) PB_FD(H, 2, 5, Q)
-    asm volatile("s_nop 4\n" PB_F3(0, 6) PB_F3(1, 7) PB_F3(2, 8) PB_F3(3, 9) PB_F3(4, 10) PB_F3(5, 11) PB_F3(6, 12)
-                     PB_F3(7, 13) PB_F3(8, 14) PB_F3(9, 15) PB_F3(10, 16) PB_F3(11, 17) PB_F3(12, 18) PB_F3(13, 19)
-                         PB_F3(14, 20) PB_F3(15, 21)
-                 : "+v"(a0), "+v"(a1), "+v"(a2)
-                 : "v"(f0), "v"(f1), "v"(f2), "v"(p[0]), "v"(p[1]), "v"(p[2]), "v"(p[3]), "v"(p[4]), "v"(p[5]),
-                   "v"(p[6]), "v"(p[7]), "v"(p[8]), "v"(p[9]), "v"(p[10]), "v"(p[11]), "v"(p[12]), "v"(p[13]),
-                   "v"(p[14]), "v"(p[15]));
+#pragma unroll
+    for (int s = 0; s < NS; s++) { // one block per force set, the weights shared
+        const float f0 = w[0] * S.sF[3 * s][e], f1 = w[0] * S.sF[3 * s + 1][e], f2 = w[0] * S.sF[3 * s + 2][e];
+        asm volatile("s_nop 4\n" PB_F3(0, 6) PB_F3(1, 7) PB_F3(2, 8) PB_F3(3, 9) PB_F3(4, 10) PB_F3(5, 11)
+                         PB_F3(6, 12) PB_F3(7, 13) PB_F3(8, 14) PB_F3(9, 15) PB_F3(10, 16) PB_F3(11, 17) PB_F3(12, 18)
+                             PB_F3(13, 19) PB_F3(14, 20) PB_F3(15, 21)
+                     : "+v"(acc[s][0]), "+v"(acc[s][1]), "+v"(acc[s][2])
+                     : "v"(f0), "v"(f1), "v"(f2), "v"(p[0]), "v"(p[1]), "v"(p[2]), "v"(p[3]), "v"(p[4]), "v"(p[5]),
+                       "v"(p[6]), "v"(p[7]), "v"(p[8]), "v"(p[9]), "v"(p[10]), "v"(p[11]), "v"(p[12]), "v"(p[13]),
+                       "v"(p[14]), "v"(p[15]));
+    }
 #undef PB_F3
 #undef PB_FD
 #else
-    spread_steps16<0>(S.P[wave], lane & 15, f0, f1, f2, a0, a1, a2);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const float f0 = w[0] * S.sF[3 * s][e], f1 = w[0] * S.sF[3 * s + 1][e], f2 = w[0] * S.sF[3 * s + 2][e];
+        spread_steps16<0>(S.P[wave], lane & 15, f0, f1, f2, acc[s][0], acc[s][1], acc[s][2]);
+    }
 #endif
     __builtin_amdgcn_wave_barrier();
 }
 
-template <typename T>
-__device__ __forceinline__ void spread_batch(spread_lds<T> &S, int wave, int lane, int head, int cnt,
+template <typename T, int NS>
+__device__ __forceinline__ void spread_batch(spread_lds<T, NS> &S, int wave, int lane, int head, int cnt,
                                              const int *sb, const pb_kgrid &g, int li, int lj, int lk,
-                                             T &a0, T &a1, T &a2)
+                                             T (&acc)[NS][3])
 {
     if constexpr (sizeof(T) == 4) {
-        spread_batch16(S, wave, lane, head, cnt, sb, g, a0, a1, a2);
+        spread_batch16<NS>(S, wave, lane, head, cnt, sb, g, acc);
     } else {
+        static_assert(NS == 1, "FP64 spreads one force set");
+        T &a0 = acc[0][0], &a1 = acc[0][1], &a2 = acc[0][2];
         __builtin_amdgcn_wave_barrier();
         if (lane < cnt) spread_weights<T>(S, wave, lane, S.ring[wave][(head + lane) & 127], sb, g);
         __builtin_amdgcn_wave_barrier();
@@ -237,9 +249,9 @@ __device__ __forceinline__ void spread_batch(spread_lds<T> &S, int wave, int lan
 }
 
 // The wavefront's two sub-blocks gather from the nst staged sources.
-template <typename T>
-__device__ __forceinline__ void spread_process(spread_lds<T> &S, int wave, int lane, int nst, const int *sbxy,
-                                               int gz, const pb_kgrid &g, T (&acc)[2][3])
+template <typename T, int NS>
+__device__ __forceinline__ void spread_process(spread_lds<T, NS> &S, int wave, int lane, int nst, const int *sbxy,
+                                               int gz, const pb_kgrid &g, T (&acc)[2][NS][3])
 {
     constexpr int HB = spread_cfg<T>::HB;
     const int li = lane >> 4, lj = (lane >> 2) & 3, lk = lane & 3;
@@ -263,23 +275,23 @@ __device__ __forceinline__ void spread_process(spread_lds<T> &S, int wave, int l
             }
             pend += __popcll(m);
             while (pend >= HB) {
-                spread_batch<T>(S, wave, lane, head, HB, sb, g, li, lj, lk, acc[sz][0], acc[sz][1], acc[sz][2]);
+                spread_batch<T, NS>(S, wave, lane, head, HB, sb, g, li, lj, lk, acc[sz]);
                 head = (head + HB) & 127;
                 pend -= HB;
             }
         }
-        if (pend > 0) spread_batch<T>(S, wave, lane, head, pend, sb, g, li, lj, lk, acc[sz][0], acc[sz][1], acc[sz][2]);
+        if (pend > 0) spread_batch<T, NS>(S, wave, lane, head, pend, sb, g, li, lj, lk, acc[sz]);
     }
 }
 
-template <typename T>
+template <typename T, int NS>
 __device__ __forceinline__ void spread_region(
     const int rid, const int *__restrict__ cstart, const double4 *__restrict__ spos,
-    const double4 *__restrict__ sfrc, const pb_cells &cells, const pb_kgrid &g,
-    typename cplx<T>::type4 *__restrict__ grid4)
+    const double4 *const (&sfrc)[NS], const pb_cells &cells, const pb_kgrid &g,
+    typename cplx<T>::type4 *const (&grid4)[NS])
 {
-    constexpr int CAP = spread_cfg<T>::CAP;
-    __shared__ spread_lds<T> S;
+    constexpr int CAP = spread_lds<T, NS>::CAP;
+    __shared__ spread_lds<T, NS> S;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int nr0 = (g.n[0] + 7) / 8, nr1 = (g.n[1] + 7) / 8, nr2 = (g.n[2] + 7) / 8;
     if (rid >= nr0 * nr1 * nr2) return; // whole workgroup, before any barrier
@@ -298,7 +310,11 @@ __device__ __forceinline__ void spread_region(
     const int ncx = uhi[0] - ulo[0] + 1, ncy = uhi[1] - ulo[1] + 1;
     const int lz0 = floordiv(ulo[2], nc2), nzseg = floordiv(uhi[2], nc2) - lz0 + 1;
     const long long nruns = (long long)ncx * ncy * nzseg;
-    T acc[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    T acc[2][NS][3];
+#pragma unroll
+    for (int sz = 0; sz < 2; sz++)
+#pragma unroll
+        for (int q = 0; q < NS; q++) acc[sz][q][0] = acc[sz][q][1] = acc[sz][q][2] = (T)0;
     int nst = 0;
     for (long long rb = 0; rb < nruns; rb += 256) {
         // one candidate run per thread: column (u0, u1), z segment with one image
@@ -370,14 +386,13 @@ __device__ __forceinline__ void spread_region(
                 tot += S.wcount[w];
             }
             if (nst + tot > CAP) { // no room: gather from what is staged, then restart
-                spread_process<T>(S, wave, lane, nst, sbxy, gA[2], g, acc);
+                spread_process<T, NS>(S, wave, lane, nst, sbxy, gA[2], g, acc);
                 nst = 0;
                 __syncthreads();
             }
             if (hit) {
                 const int e = nst + wbase +
                               (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                const double4 f = sfrc[src];
                 S.sx[0][e] = xw[0];
                 S.sx[1][e] = xw[1];
                 S.sx[2][e] = xw[2];
@@ -385,27 +400,34 @@ __device__ __forceinline__ void spread_region(
                 S.sf0[1][e] = f0[1];
                 S.sf0[2][e] = f0[2];
                 S.scode[e] = cq;
-                S.sF[0][e] = (T)(f.x * g.scale);
-                S.sF[1][e] = (T)(f.y * g.scale);
-                S.sF[2][e] = (T)(f.z * g.scale);
+#pragma unroll
+                for (int q = 0; q < NS; q++) {
+                    const double4 f = sfrc[q][src];
+                    S.sF[3 * q][e] = (T)(f.x * g.scale);
+                    S.sF[3 * q + 1][e] = (T)(f.y * g.scale);
+                    S.sF[3 * q + 2][e] = (T)(f.z * g.scale);
+                }
             }
             nst += tot;
             __syncthreads();
         }
         __syncthreads(); // the run arrays are rewritten by the next chunk
     }
-    spread_process<T>(S, wave, lane, nst, sbxy, gA[2], g, acc);
+    spread_process<T, NS>(S, wave, lane, nst, sbxy, gA[2], g, acc);
     const int li = lane >> 4, lj = (lane >> 2) & 3, lk = lane & 3;
 #pragma unroll
     for (int sz = 0; sz < 2; sz++) {
         const int p0 = sbxy[0] + li, p1 = sbxy[1] + lj, p2 = gA[2] + 4 * sz + lk;
         if (sbxy[0] >= g.n[0] || sbxy[1] >= g.n[1] || gA[2] + 4 * sz >= g.n[2]) continue;
-        typename cplx<T>::type4 o;
-        o.x = acc[sz][0];
-        o.y = acc[sz][1];
-        o.z = acc[sz][2];
-        o.w = (T)0;
-        grid4[((size_t)p0 * g.n[1] + p1) * g.n[2] + p2] = o;
+#pragma unroll
+        for (int q = 0; q < NS; q++) {
+            typename cplx<T>::type4 o;
+            o.x = acc[sz][q][0];
+            o.y = acc[sz][q][1];
+            o.z = acc[sz][q][2];
+            o.w = (T)0;
+            grid4[q][((size_t)p0 * g.n[1] + p1) * g.n[2] + p2] = o;
+        }
     }
 }
 
@@ -415,7 +437,9 @@ __global__ __launch_bounds__(64 * kSpreadWaves) void k_spread_regions(
     const double4 *__restrict__ sfrc, pb_cells cells, pb_kgrid g,
     typename cplx<T>::type4 *__restrict__ grid4)
 {
-    spread_region<T>((int)blockIdx.x, cstart, spos, sfrc, cells, g, grid4);
+    const double4 *const fr[1] = {sfrc};
+    typename cplx<T>::type4 *const gr[1] = {grid4};
+    spread_region<T, 1>((int)blockIdx.x, cstart, spos, fr, cells, g, gr);
 }
 
 template <typename T>
@@ -424,7 +448,30 @@ __global__ __launch_bounds__(64 * kSpreadWaves) void k_gspread(const pb_sys *__r
 {
     const int2 m = map[blockIdx.x];
     const pb_sys &S = sys[m.x];
-    spread_region<T>(m.y, S.sstart, S.spos, S.sfrc, S.cells, S.g, (typename cplx<T>::type4 *)S.grid4);
+    const double4 *const fr[1] = {S.sfrc};
+    typename cplx<T>::type4 *const gr[1] = {(typename cplx<T>::type4 *)S.grid4};
+    spread_region<T, 1>(m.y, S.sstart, S.spos, fr, S.cells, S.g, gr);
+}
+
+// Grouped, NS force sets per workgroup: block b spreads sets NS map[b].x ..
+// NS map[b].x + NS - 1 of region map[b].y (the sets share particles and grid
+// geometry, the first one's descriptor gives them).
+template <int NS>
+__global__ __launch_bounds__(64 * kSpreadWaves) void k_gspread_sets(const pb_sys *__restrict__ sys,
+                                                                    const int2 *__restrict__ map)
+{
+    const int2 m = map[blockIdx.x];
+    const pb_sys *S = sys + NS * m.x;
+    const double4 *fr[NS];
+    float4 *gr[NS];
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+        fr[q] = S[q].sfrc;
+        gr[q] = (float4 *)S[q].grid4;
+    }
+    const double4 *const(&frc)[NS] = fr;
+    float4 *const(&grd)[NS] = gr;
+    spread_region<float, NS>(m.y, S[0].sstart, S[0].spos, frc, S[0].cells, S[0].g, grd);
 }
 
 // ---- interpolation (gather) --------------------------------------------------
@@ -1357,6 +1404,12 @@ void pb_group_spread(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk
         hipLaunchKernelGGL(k_gspread<float>, dim3(nblk), dim3(64 * kSpreadWaves), 0, s, sys, map);
     else
         hipLaunchKernelGGL(k_gspread<double>, dim3(nblk), dim3(64 * kSpreadWaves), 0, s, sys, map);
+}
+
+void pb_group_spread_sets(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk)
+{
+    if (nblk > 0)
+        hipLaunchKernelGGL(k_gspread_sets<pb_spread_sets>, dim3(nblk), dim3(64 * kSpreadWaves), 0, s, sys, map);
 }
 
 int pb_group_fft(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, int precision, int pass, int n,

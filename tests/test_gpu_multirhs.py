@@ -13,7 +13,7 @@ def relL2(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
 
 
-@pytest.mark.parametrize("R,same", [(5, False), (40, True)])
+@pytest.mark.parametrize("R,same", [(5, False), (13, False), (40, True)])
 def test_multirhs_matches_oracle(R, same):
     import torch
     ns, nt = 3000, 2000
