@@ -86,9 +86,11 @@ template <> struct alignas(16) spread_batch_lds<float> {
 };
 
 // NS > 1 (FP32 only): NS force sets spread together from one staging of the
-// sources (the many-right-hand-side k space), with a smaller staging capacity.
+// sources (the many-right-hand-side k space), with a smaller staging capacity
+// (at least 256: one chunk of candidates is staged whole after a flush).
 template <typename T, int NS = 1> struct spread_lds : spread_batch_lds<T> {
-    static constexpr int CAP = NS == 1 ? spread_cfg<T>::CAP : 224, HB = spread_cfg<T>::HB;
+    static constexpr int CAP = NS == 1 ? spread_cfg<T>::CAP : 256, HB = spread_cfg<T>::HB;
+    static_assert(CAP >= 256, "a chunk of 256 candidates must fit the staging");
     int run_j[256];       // first source of each candidate run of the chunk
     int run_pre[257];     // exclusive prefix of the run lengths
     unsigned run_code[256]; // image shifts (l0 + 4) | (l1 + 4) << 4 | (l2 + 4) << 8

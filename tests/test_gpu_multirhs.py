@@ -53,3 +53,27 @@ def test_multirhs_rejects_closed_form_kernel():
         ctx.velocity_multirhs(p, x, F, x)
     assert ei.value.status == pb.PB_ERR_UNSUPPORTED
     ctx.close()
+
+
+def test_multirhs_dense_staging_matches_single_sets():
+    # cells much smaller than a spreading region plus its window (grid 32,
+    # P 12, rc 0.1): nearly every candidate source of a region is staged, so
+    # chunks of 256 candidates stage whole after a flush -- the case the
+    # grouped spread's staging capacity must hold.  Every set against the
+    # single-set call.
+    import torch
+    n, R = 20000, 16
+    L = (1.0, 1.0, 1.0)
+    x, _, _ = pb.uniform_system(pb.config_seed(8), n, 0, L)
+    rng = np.random.default_rng(9)
+    F = rng.standard_normal((R, n, 3)).astype(np.float32)
+    p = pb.Params(L=L, alpha=5.0, eps=1e-3, d=0.07, rc=0.1, grid=(32, 32, 32), P=12, precision="mixed")
+    ctx = pb.Context(0)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+    X = dev(x)
+    u = ctx.velocity_multirhs(p, X, dev(F), X).cpu().numpy()
+    assert np.all(np.isfinite(u))
+    for r in range(R):
+        single = ctx.velocity(p, X, dev(F[r].astype(np.float64)), X).cpu().numpy()
+        assert relL2(u[r], single) < 1e-6, (r, relL2(u[r], single))
+    ctx.close()
