@@ -324,8 +324,9 @@ int pb_group_interp(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk,
 int pb_group_fft_blocks(int precision, int pass, const pb_kgrid &g);
 constexpr int pb_interp_waves = 4; // wavefronts per gather block
 // targets per gather wavefront for window P (P^2 window columns per target:
-// 4 targets fill whole rounds of 64 lanes where P^2 is not a multiple of 64)
-constexpr __host__ __device__ int pb_interp_tw(int P) { return P >= 8 && (P * P) % 64 != 0 ? 4 : 1; }
+// 4 targets fill whole rounds of 64 lanes where P^2 is not a multiple of 64;
+// only up to P = 12, beyond which the unrolled rounds spill registers)
+constexpr __host__ __device__ int pb_interp_tw(int P) { return P >= 8 && P <= 12 && (P * P) % 64 != 0 ? 4 : 1; }
 constexpr int pb_pair_waves = 4;   // wavefronts per pair-kernel workgroup
 // the fused binning (one scan workgroup) applies to this key space
 bool pb_bin_fused_ok(const pb_cells &cells);
