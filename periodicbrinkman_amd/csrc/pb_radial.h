@@ -188,6 +188,17 @@ __device__ __noinline__ double2 vR_H_call(double r, const pb_kparams *K)
 }
 
 // Where the table of diffH(d) lives: z = r/d in [tz0, tz0 + tn/tinv).
+// sqrt(x) for normal x > 0 from the hardware reciprocal square root and one
+// Newton step on the root (r += (x - r^2) / (2 sqrt x)): a few ulp, without the
+// library's scaling and special-value handling (the callers guarantee x is
+// a normal positive number).
+__device__ __forceinline__ double sqrt_pos(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    const double r = x * y;
+    return fma(0.5 * y, fma(-r, r, x), r);
+}
+
 struct tabview {
     double inv_d, tz0, tinv;
     int tn;
@@ -208,7 +219,7 @@ __device__ __forceinline__ void table_H(double r, const tabview &V, const double
     int i = (int)u;
     i = i < V.tn ? i : V.tn - 1;
     const double t = 2.0 * (u - (double)i) - 1.0;
-    const double2 *c = tab + i * (DEG + 1);
+    const double2 *c = tab + __umul24(i, DEG + 1); // i < 2^24: a full-rate 24-bit multiply
     double h1 = c[DEG].x, h2 = c[DEG].y;
 #pragma unroll
     for (int k = DEG - 1; k >= 0; k--) {

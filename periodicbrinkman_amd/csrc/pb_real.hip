@@ -113,7 +113,7 @@ struct batch_consts {
     pbdev::tabview32 tv32;
     pbdev::radial_core core;
     pb_radial32 pf;
-    double rcommon;
+    double rcommon, rcommon2;
 };
 
 template <bool TABLE, int DEG>
@@ -126,7 +126,7 @@ __device__ __forceinline__ void batch64(wave_lds &W, const int lane, const int c
     bool on = lane < cnt;
     uint2 e = make_uint2(0u, 0u);
     int t = 0;
-    double d0 = 0, d1 = 0, d2 = 0, r = 0;
+    double d0 = 0, d1 = 0, d2 = 0, r2 = 0;
     double4 fs = make_double4(0, 0, 0, 0);
     if (on) {
         e = W.q[(head + lane) & (kQ - 1)];
@@ -137,18 +137,18 @@ __device__ __forceinline__ void batch64(wave_lds &W, const int lane, const int c
         d0 = W.y[t][0] - (xs.x + s0);
         d1 = W.y[t][1] - (xs.y + s1);
         d2 = W.y[t][2] - (xs.z + s2);
-        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        r2 = d0 * d0 + d1 * d1 + d2 * d2;
         on = r2 < rc2; // the exact cutoff (the FP32 candidate test lets a few more through)
-        r = sqrt(r2);
     }
     nin += __popcll(__ballot(on));
-    const bool rare = on && r < K.rcommon;
+    const bool rare = on && r2 < K.rcommon2; // (r2 = 0, a target on a source, is rare)
     const unsigned long long mr = __ballot(rare);
     if (mr) {
         if (rare) W.qr[(rh + rn + lane_prefix(mr)) & (kQ - 1)] = e;
         rn += __popcll(mr);
     }
     if (on && !rare) {
+        const double r = pbdev::sqrt_pos(r2); // r2 >= rcommon^2 > 0
         double H1, H2;
         if (TABLE)
             pbdev::table_H<DEG>(r, K.tv, tab, H1, H2);
@@ -232,7 +232,7 @@ __device__ __forceinline__ void pair_body(
 {
     const pb_kparams &kp = *kpp;
     const batch_consts K{pbdev::tabview_of(kp), pbdev::tabview32_of(kp), pbdev::core_of(kp.pd), kp.pf,
-                         kp.rcommon};
+                         kp.rcommon, kp.rcommon * kp.rcommon};
     const double rfar2 = kp.rfar2;
     __shared__ wave_lds lds[kWaves];
     extern __shared__ double dyn_lds[]; // the radial table: FP64 then (FP32 far field) FP32 copy
