@@ -99,3 +99,23 @@ def test_generator_matches_independent_restatement():
     x, f, y = pb.uniform_system(seed, 10, 5)
     assert x.shape == (10, 3) and f.shape == (10, 3) and y.shape == (5, 3)
     assert np.array_equal(x.reshape(-1), np.array(_xoshiro_ref(seed, 30)))
+
+
+def test_batched_workload_generator():
+    # the batched workload: sizes, boxes and alphas from one stream in the
+    # stated ranges, each system's points from its own stream, reproducible
+    seed = pb.config_seed(3)
+    systems, x, f, off = pb.batched_systems(seed, nsys=40)
+    assert len(systems) == 40 and off[0] == 0 and off[-1] == x.shape[0] == f.shape[0]
+    u = np.array(_xoshiro_ref(seed, 3 * 40)).reshape(40, 3)
+    for i, s in enumerate(systems):
+        assert off[i + 1] - off[i] == s["n"]
+        assert s["n"] == int(np.floor(np.exp(np.log(100) + u[i, 0] * (np.log(20000) - np.log(100)))))
+        assert 100 <= s["n"] < 20000 and 0.5 <= s["L"] < 2.0 and 1.0 <= s["alpha"] < 50.0
+        assert s["eps"] == pytest.approx(1e-3 * s["L"], rel=1e-15)
+        xi = x[off[i]:off[i + 1]]
+        assert np.all((xi >= 0) & (xi < s["L"]))
+    x0, _, _ = pb.uniform_system(seed + (1 << 32), systems[0]["n"], 0, (systems[0]["L"],) * 3)
+    assert np.array_equal(x[:off[1]], x0)
+    _, x2, _, _ = pb.batched_systems(seed, nsys=40)
+    assert np.array_equal(x, x2)
