@@ -196,7 +196,8 @@ const pb_kparams &cached_kparams(double a, double d, double e, double rfar)
 // Degree of the radial table for a precision: the FP64 form keeps the table
 // at FP64 rounding (4e-15), the mixed form needs 1e-10 and takes degree 6
 // (112 bytes of coefficients per pair instead of 208, 14 FMAs instead of 26)
-// where that fits the LDS budget (large alpha d needs short intervals), else 12.
+// where that fits the LDS budget (large alpha d needs short intervals), else 8,
+// else 12.
 int table_degree(const pb_params *p)
 {
     return p->precision == PB_PREC_MIXED ? PB_TAB_DEG_MIXED : PB_TAB_DEG;
@@ -213,6 +214,7 @@ void host_kparams(const pb_params *p, double rfar, pb_kparams *kp, std::vector<d
         const bool mixed = p->precision == PB_PREC_MIXED;
         const double z0 = kp->rcommon / p->d, z1 = p->rc / p->d * (1 + 1e-9);
         pb_build_table(kp, z0, z1, coef, table_degree(p), mixed ? 1e-10 : 4e-15);
+        if (kp->tn == 0 && mixed) pb_build_table(kp, z0, z1, coef, PB_TAB_DEG_MID, 1e-10);
         if (kp->tn == 0 && mixed) pb_build_table(kp, z0, z1, coef, PB_TAB_DEG, 1e-10);
         if (kp->tn == 0) coef.clear();
     }

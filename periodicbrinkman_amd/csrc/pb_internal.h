@@ -74,6 +74,7 @@ struct pb_kparams {
 #define PB_TAB_MAXDEG 15
 #define PB_TAB_DEG 12      // degree of the table polynomials (FP64 form, error <= 4e-15)
 #define PB_TAB_DEG_MIXED 6 // degree for the mixed form (error <= 1e-10): fewer bytes and FMAs per pair
+#define PB_TAB_DEG_MID 8   // the mixed form's degree where 6 does not fit the LDS budget (large alpha d)
 #define PB_TAB_LDS_DOUBLES 2048 // LDS budget of the FP64 table (16 KB)
 
 struct pb_box {

@@ -334,11 +334,14 @@ int pb_run_mrhs_real(pb_context *c, hipStream_t s, const pb_cells &cells, const 
     const size_t lds = table_doubles * sizeof(double);
     // 64 right-hand sides per launch (two 32 x 32 accumulator blocks per
     // component, 8 sources per contraction step), a last 32 with one block
-    if (table_deg != PB_TAB_DEG && table_deg != PB_TAB_DEG_MIXED)
+    if (table_deg != PB_TAB_DEG && table_deg != PB_TAB_DEG_MIXED && table_deg != PB_TAB_DEG_MID)
         return pb_fail(c, PB_ERR_INVALID, "radial table of an unsupported degree");
-    const bool hi = table_deg == PB_TAB_DEG;
-    auto k64 = hi ? k_pair_mrhs<MRHS_Q2, 2, PB_TAB_DEG> : k_pair_mrhs<MRHS_Q2, 2, PB_TAB_DEG_MIXED>;
-    auto k32 = hi ? k_pair_mrhs<MRHS_Q1, 1, PB_TAB_DEG> : k_pair_mrhs<MRHS_Q1, 1, PB_TAB_DEG_MIXED>;
+    auto k64 = table_deg == PB_TAB_DEG   ? k_pair_mrhs<MRHS_Q2, 2, PB_TAB_DEG>
+               : table_deg == PB_TAB_DEG_MID ? k_pair_mrhs<MRHS_Q2, 2, PB_TAB_DEG_MID>
+                                             : k_pair_mrhs<MRHS_Q2, 2, PB_TAB_DEG_MIXED>;
+    auto k32 = table_deg == PB_TAB_DEG   ? k_pair_mrhs<MRHS_Q1, 1, PB_TAB_DEG>
+               : table_deg == PB_TAB_DEG_MID ? k_pair_mrhs<MRHS_Q1, 1, PB_TAB_DEG_MID>
+                                             : k_pair_mrhs<MRHS_Q1, 1, PB_TAB_DEG_MIXED>;
     for (int c0 = 0; c0 < RP; c0 += 64)
         hipLaunchKernelGGL(RP - c0 >= 64 ? k64 : k32, dim3(blocks), dim3(64 * kMWaves), lds, s,
                            (const int2 *)c->tiles.ptr, ntiles, (const int *)c->cell_start.ptr,

@@ -513,10 +513,14 @@ int pb_group_pair(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, i
 {
     if (nblk <= 0) return PB_OK;
     dim3 gd(nblk), bd(64 * kWaves);
-    if (tdeg == PB_TAB_DEG_MIXED && far32) hipLaunchKernelGGL((k_gpair_table<true, PB_TAB_DEG_MIXED>), gd, bd, lds, s, sys, map, nsplit);
-    else if (tdeg == PB_TAB_DEG && far32) hipLaunchKernelGGL((k_gpair_table<true, PB_TAB_DEG>), gd, bd, lds, s, sys, map, nsplit);
-    else if (tdeg == PB_TAB_DEG_MIXED) hipLaunchKernelGGL((k_gpair_table<false, PB_TAB_DEG_MIXED>), gd, bd, lds, s, sys, map, nsplit);
-    else if (tdeg == PB_TAB_DEG) hipLaunchKernelGGL((k_gpair_table<false, PB_TAB_DEG>), gd, bd, lds, s, sys, map, nsplit);
+#define PB_GP(M, D) hipLaunchKernelGGL((k_gpair_table<M, D>), gd, bd, lds, s, sys, map, nsplit)
+    if (tdeg == PB_TAB_DEG_MIXED && far32) PB_GP(true, PB_TAB_DEG_MIXED);
+    else if (tdeg == PB_TAB_DEG_MID && far32) PB_GP(true, PB_TAB_DEG_MID);
+    else if (tdeg == PB_TAB_DEG && far32) PB_GP(true, PB_TAB_DEG);
+    else if (tdeg == PB_TAB_DEG_MIXED) PB_GP(false, PB_TAB_DEG_MIXED);
+    else if (tdeg == PB_TAB_DEG_MID) PB_GP(false, PB_TAB_DEG_MID);
+    else if (tdeg == PB_TAB_DEG) PB_GP(false, PB_TAB_DEG);
+#undef PB_GP
     else if (tdeg == 0 && far32) hipLaunchKernelGGL((k_gpair_direct<true>), gd, bd, 0, s, sys, map, nsplit);
     else if (tdeg == 0) hipLaunchKernelGGL((k_gpair_direct<false>), gd, bd, 0, s, sys, map, nsplit);
     else return PB_ERR_INVALID;
@@ -562,13 +566,15 @@ int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kp
         (const double4 *)c->sfrc.ptr, (const double4 *)c->tpos.ptr, (const int *)c->tperm.ptr, nsplit,    \
         cells, box, kp, u, (uint4 *)c->counters.ptr
     dim3 gd(blocks), bd(64 * kWaves);
-    if (ntab && table_deg != PB_TAB_DEG && table_deg != PB_TAB_DEG_MIXED)
+    if (ntab && table_deg != PB_TAB_DEG && table_deg != PB_TAB_DEG_MIXED && table_deg != PB_TAB_DEG_MID)
         return pb_fail(c, PB_ERR_INVALID, "radial table of an unsupported degree");
-    const bool lo = table_deg == PB_TAB_DEG_MIXED;
+    const bool lo = table_deg == PB_TAB_DEG_MIXED, mid = table_deg == PB_TAB_DEG_MID;
     if (mixed && ntab && lo) hipLaunchKernelGGL((k_pair_table<true, PB_TAB_DEG_MIXED>), gd, bd, lds, s, PB_PAIR_ARGS);
+    else if (mixed && ntab && mid) hipLaunchKernelGGL((k_pair_table<true, PB_TAB_DEG_MID>), gd, bd, lds, s, PB_PAIR_ARGS);
     else if (mixed && ntab) hipLaunchKernelGGL((k_pair_table<true, PB_TAB_DEG>), gd, bd, lds, s, PB_PAIR_ARGS);
     else if (mixed) hipLaunchKernelGGL((k_pair_direct<true>), gd, bd, 0, s, PB_PAIR_ARGS);
     else if (ntab && lo) hipLaunchKernelGGL((k_pair_table<false, PB_TAB_DEG_MIXED>), gd, bd, lds, s, PB_PAIR_ARGS);
+    else if (ntab && mid) hipLaunchKernelGGL((k_pair_table<false, PB_TAB_DEG_MID>), gd, bd, lds, s, PB_PAIR_ARGS);
     else if (ntab) hipLaunchKernelGGL((k_pair_table<false, PB_TAB_DEG>), gd, bd, lds, s, PB_PAIR_ARGS);
     else hipLaunchKernelGGL((k_pair_direct<false>), gd, bd, 0, s, PB_PAIR_ARGS);
     pb_mark(c, s, PB_NPHASE); // the pair interval ends with the launch, on its own stream
