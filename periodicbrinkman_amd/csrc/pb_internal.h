@@ -253,10 +253,9 @@ int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_
                   const double *pos, const double *frc, bool is_source);
 // Many right-hand sides (pb_mrhs.hip): Fs[j][b][rp] from the caller's R force
 // sets (float, set r at r * 3 ns) padded to RP columns; the FP64 source forces
-// of set r for the k-space pass; the real-space contraction added into u
-// (double, set r at r * 3 nt).
+// of a chunk of sets for the k-space pass; the real-space contraction added
+// into u (double, set r at r * 3 nt).
 int pb_sort_forces(pb_context *c, hipStream_t s, int ns, int R, int RP, const float *f);
-int pb_force_set(pb_context *c, hipStream_t s, int ns, int r, int RP);
 // frc[q * ns + j] = force set r0 + q of sorted source j, q < nr
 int pb_force_sets(pb_context *c, hipStream_t s, int ns, int r0, int nr, int RP, double4 *frc);
 int pb_run_mrhs_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp,

@@ -274,15 +274,6 @@ __global__ void k_sort_forces(int ns, int R, int RP, const int *__restrict__ per
     Fs[q] = rp < R ? f[(size_t)rp * 3 * ns + 3 * (size_t)perm[j] + b] : 0.0f;
 }
 
-// sfrc[j] = force set r of sorted source j (for the k-space pass).
-__global__ void k_force_set(int ns, int r, int RP, const float *__restrict__ Fs, double4 *__restrict__ sfrc)
-{
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ns) return;
-    const float *p = Fs + (size_t)j * 3 * RP + r;
-    sfrc[j] = make_double4((double)p[0], (double)p[RP], (double)p[2 * RP], 0.0);
-}
-
 __global__ void k_force_sets(int ns, int r0, int nr, int RP, const float *__restrict__ Fs, double4 *__restrict__ frc)
 {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -312,14 +303,6 @@ int pb_sort_forces(pb_context *c, hipStream_t s, int ns, int R, int RP, const fl
         hipLaunchKernelGGL(k_sort_forces, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ns, R, RP,
                            (const int *)c->sperm.ptr, f, (float *)c->mrhs_f.ptr);
     return pb_hip_check(c, hipGetLastError(), "force sort launch");
-}
-
-int pb_force_set(pb_context *c, hipStream_t s, int ns, int r, int RP)
-{
-    if (ns > 0)
-        hipLaunchKernelGGL(k_force_set, dim3((ns + 255) / 256), dim3(256), 0, s, ns, r, RP,
-                           (const float *)c->mrhs_f.ptr, (double4 *)c->sfrc.ptr);
-    return pb_hip_check(c, hipGetLastError(), "force set launch");
 }
 
 int pb_run_mrhs_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp,
