@@ -319,7 +319,6 @@ __device__ void scan_counts(const int *__restrict__ count, int n, int *__restric
                             int2 *__restrict__ tiles, int *__restrict__ ntiles)
 {
     int carry = 0, tcarry = 0;
-    // the counts of the next round are loaded while this round is scanned
     auto load16 = [&](int i0, int (&v)[16]) {
         if (i0 + 16 <= n) {
             const int4 *c4 = reinterpret_cast<const int4 *>(count + i0);
@@ -336,14 +335,10 @@ __device__ void scan_counts(const int *__restrict__ count, int n, int *__restric
             for (int q = 0; q < 16; q++) v[q] = i0 + q < n ? count[i0 + q] : 0;
         }
     };
-    int vn[16];
-    load16(16 * (int)threadIdx.x, vn);
     for (int base = 0; base < n; base += 16 * 1024) {
         const int i0 = base + 16 * (int)threadIdx.x;
         int v[16];
-#pragma unroll
-        for (int q = 0; q < 16; q++) v[q] = vn[q];
-        if (base + 16 * 1024 < n) load16(i0 + 16 * 1024, vn);
+        load16(i0, v);
         int sum = 0;
 #pragma unroll
         for (int q = 0; q < 16; q++) sum += v[q];
