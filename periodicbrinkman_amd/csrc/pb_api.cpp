@@ -457,6 +457,8 @@ int pb_destroy(pb_context *c)
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
     if (c->pair_stream) (void)hipStreamDestroy(c->pair_stream);
+    if (c->pair_stream2) (void)hipStreamDestroy(c->pair_stream2);
+    if (c->pair_event2) (void)hipEventDestroy(c->pair_event2);
     for (int k = 0; k < 2; k++)
         if (c->pair_event[k]) (void)hipEventDestroy(c->pair_event[k]);
     pb_buffer *bufs[] = {&c->cell_count, &c->cell_start, &c->cell_of_s, &c->rank_s, &c->cell_of_t,
@@ -874,8 +876,8 @@ int group_prepare(pb_context *c, hipStream_t s, const std::vector<int> &idx, con
     return PB_OK;
 }
 
-// The group's launches on stream g (pair kernels forked to c->pair_stream and
-// joined back); u of the group's targets must be zero.
+// The group's launches on stream g (pair kernels forked to two pair streams
+// and joined back); u of the group's targets must be zero.
 int group_enqueue(pb_context *c, hipStream_t g)
 {
     int st;
@@ -883,6 +885,7 @@ int group_enqueue(pb_context *c, hipStream_t g)
     const int2 *map = (const int2 *)c->gb_map.ptr;
     pb_fill_zero(g, (long long)c->group_counts, (int *)c->gb_int.ptr);
     bool forked = false;
+    int npair = 0;
     for (const auto &L : c->group_launches) {
         const int2 *m = map + L.map;
         switch (L.kind) {
@@ -900,12 +903,24 @@ int group_enqueue(pb_context *c, hipStream_t g)
                         (st = pb_hip_check(c, hipEventCreateWithFlags(&c->pair_event[k], hipEventDisableTiming),
                                            "event")))
                         return st;
+                if (!c->pair_stream2 &&
+                    (st = pb_hip_check(c, hipStreamCreateWithFlags(&c->pair_stream2, hipStreamNonBlocking),
+                                       "stream")))
+                    return st;
+                if (!c->pair_event2 &&
+                    (st = pb_hip_check(c, hipEventCreateWithFlags(&c->pair_event2, hipEventDisableTiming), "event")))
+                    return st;
                 if ((st = pb_hip_check(c, hipEventRecord(c->pair_event[0], g), "fork"))) return st;
                 if ((st = pb_hip_check(c, hipStreamWaitEvent(c->pair_stream, c->pair_event[0], 0), "fork")))
                     return st;
+                if ((st = pb_hip_check(c, hipStreamWaitEvent(c->pair_stream2, c->pair_event[0], 0), "fork")))
+                    return st;
                 forked = true;
             }
-            if ((st = pb_group_pair(c->pair_stream, sys, m, L.nblk, L.c, L.a != 0, L.b, L.lds)))
+            // the classes alternate between the two pair streams, so they run
+            // side by side (and beside the k space on g)
+            if ((st = pb_group_pair(npair++ % 2 ? c->pair_stream2 : c->pair_stream, sys, m, L.nblk, L.c, L.a != 0, L.b,
+                                    L.lds)))
                 return pb_fail(c, st, "grouped pair kernel launch");
             break;
         case 4: pb_group_spread(g, sys, m, L.nblk, L.a); break;
@@ -922,6 +937,8 @@ int group_enqueue(pb_context *c, hipStream_t g)
     if (forked) {
         if ((st = pb_hip_check(c, hipEventRecord(c->pair_event[1], c->pair_stream), "join"))) return st;
         if ((st = pb_hip_check(c, hipStreamWaitEvent(g, c->pair_event[1], 0), "join"))) return st;
+        if ((st = pb_hip_check(c, hipEventRecord(c->pair_event2, c->pair_stream2), "join"))) return st;
+        if ((st = pb_hip_check(c, hipStreamWaitEvent(g, c->pair_event2, 0), "join"))) return st;
     }
     return pb_hip_check(c, hipGetLastError(), "grouped launch");
 }

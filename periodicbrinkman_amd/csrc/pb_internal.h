@@ -192,6 +192,9 @@ struct pb_context {
     // k-space pipeline on the caller's stream (fork and join by events)
     hipStream_t pair_stream = nullptr;
     hipEvent_t pair_event[2] = {};
+    // a grouped call's second pair-kernel class runs on a second stream
+    hipStream_t pair_stream2 = nullptr;
+    hipEvent_t pair_event2 = {};
     // grouped batched systems: scratch of every system (ints, double4 points,
     // tiles, grids, spectra), device descriptors and block maps, and the
     // launch list built for the group layout in group_key
