@@ -401,24 +401,27 @@ __global__ __launch_bounds__(1024) void k_bin_scan2(const int *__restrict__ scou
 // ---- grouped binning: every system of a batched call in one launch each ----
 __global__ __launch_bounds__(256) void k_gbin_count(const pb_sys *__restrict__ sys, const int2 *__restrict__ map)
 {
-    const int2 m = map[blockIdx.x];
-    const pb_sys &S = sys[m.x];
-    bin_count2(m.y * 256 + (int)threadIdx.x, S.ns, S.xn, S.nt, S.y, S.box, S.cells, S.scount, S.skey, S.srank,
+    const int2 m = pb_map_at(map, blockIdx.x);
+    const auto &S = ((pb_const_sys)sys)[m.x];
+    const pb_box box = S.box;
+    const pb_cells cells = S.cells;
+    bin_count2(m.y * 256 + (int)threadIdx.x, S.ns, S.xn, S.nt, S.y, box, cells, S.scount, S.skey, S.srank,
                S.tcount, S.tkey, S.trank);
 }
 
 __global__ __launch_bounds__(256) void k_gbin_scatter(const pb_sys *__restrict__ sys, const int2 *__restrict__ map)
 {
-    const int2 m = map[blockIdx.x];
-    const pb_sys &S = sys[m.x];
-    bin_scatter2(m.y * 256 + (int)threadIdx.x, S.ns, S.xn, S.f, S.nt, S.y, S.box, S.sstart, S.skey, S.srank, S.spos,
+    const int2 m = pb_map_at(map, blockIdx.x);
+    const auto &S = ((pb_const_sys)sys)[m.x];
+    const pb_box box = S.box;
+    bin_scatter2(m.y * 256 + (int)threadIdx.x, S.ns, S.xn, S.f, S.nt, S.y, box, S.sstart, S.skey, S.srank, S.spos,
                  S.f ? S.sfrc : nullptr, S.sperm, S.tstart, S.tkey, S.trank, S.tpos, S.tperm);
 }
 
 __global__ __launch_bounds__(1024) void k_gbin_scan(const pb_sys *__restrict__ sys, const int2 *__restrict__ map)
 {
     __shared__ int wsum[16];
-    const pb_sys &S = sys[map[blockIdx.x].x];
+    const auto &S = ((pb_const_sys)sys)[pb_map_at(map, blockIdx.x).x];
     scan_counts(S.scount, S.cells.ncell, S.sstart, wsum, nullptr, nullptr);
     scan_counts(S.tcount, S.cells.nkey_t, S.tstart, wsum, S.tiles, S.ntiles);
 }

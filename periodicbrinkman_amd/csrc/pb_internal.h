@@ -293,6 +293,22 @@ __device__ __forceinline__ int pb_xcd_block(int b, int nblk8)
 {
     return (b & 7) * (nblk8 >> 3) + (b >> 3);
 }
+
+// Grouped kernels read their descriptors and maps through the constant
+// address space: the data pointers loaded from there are known to be global
+// (scalar-base global loads and stores instead of flat ones), and the
+// compiler may reload a descriptor field instead of keeping it live.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PB_CONSTANT_AS __attribute__((address_space(4)))
+#else
+#define PB_CONSTANT_AS // (the host pass only parses the kernel bodies)
+#endif
+typedef const PB_CONSTANT_AS pb_sys *pb_const_sys;
+typedef const PB_CONSTANT_AS int *pb_const_map; // (system, block) pairs
+__device__ __forceinline__ int2 pb_map_at(const int2 *map, long long i)
+{
+    return make_int2(((pb_const_map)map)[2 * i], ((pb_const_map)map)[2 * i + 1]);
+}
 // ---- grouped launches (one per phase and class, for many systems) ----------
 // sys: device descriptors; map: (system, block) per launched block; nblk:
 // launched blocks.  Binning: count and scatter take one block of 256 points

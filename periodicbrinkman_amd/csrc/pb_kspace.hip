@@ -448,11 +448,13 @@ template <typename T>
 __global__ __launch_bounds__(64 * kSpreadWaves) void k_gspread(const pb_sys *__restrict__ sys,
                                                                const int2 *__restrict__ map)
 {
-    const int2 m = map[blockIdx.x];
-    const pb_sys &S = sys[m.x];
+    const int2 m = pb_map_at(map, blockIdx.x);
+    const auto &S = ((pb_const_sys)sys)[m.x];
     const double4 *const fr[1] = {S.sfrc};
     typename cplx<T>::type4 *const gr[1] = {(typename cplx<T>::type4 *)S.grid4};
-    spread_region<T, 1>(m.y, S.sstart, S.spos, fr, S.cells, S.g, gr);
+    const pb_cells cells = S.cells;
+    const pb_kgrid g = S.g;
+    spread_region<T, 1>(m.y, S.sstart, S.spos, fr, cells, g, gr);
 }
 
 // Grouped, NS force sets per workgroup: block b spreads sets NS map[b].x ..
@@ -462,8 +464,8 @@ template <int NS>
 __global__ __launch_bounds__(64 * kSpreadWaves) void k_gspread_sets(const pb_sys *__restrict__ sys,
                                                                     const int2 *__restrict__ map)
 {
-    const int2 m = map[blockIdx.x];
-    const pb_sys *S = sys + NS * m.x;
+    const int2 m = pb_map_at(map, blockIdx.x);
+    const pb_const_sys S = (pb_const_sys)sys + NS * m.x;
     const double4 *fr[NS];
     float4 *gr[NS];
 #pragma unroll
@@ -473,7 +475,9 @@ __global__ __launch_bounds__(64 * kSpreadWaves) void k_gspread_sets(const pb_sys
     }
     const double4 *const(&frc)[NS] = fr;
     float4 *const(&grd)[NS] = gr;
-    spread_region<float, NS>(m.y, S[0].sstart, S[0].spos, frc, S[0].cells, S[0].g, grd);
+    const pb_cells cells = S[0].cells;
+    const pb_kgrid g = S[0].g;
+    spread_region<float, NS>(m.y, S[0].sstart, S[0].spos, frc, cells, g, grd);
 }
 
 // ---- interpolation (gather) --------------------------------------------------
@@ -500,9 +504,10 @@ template <typename T> __device__ __forceinline__ T wave_sum(T v)
 // lanes (P = 12: 9 rounds instead of 4 x 3 with a quarter of the lanes idle).
 // For P >= 8 a round spans at most two targets, so a lane's partial sums
 // go to one of two accumulators fixed at compile time.
-template <typename T, int P>
+// GR: pb_kgrid, or its constant-address-space form in the grouped kernels.
+template <typename T, int P, typename GR>
 __device__ __forceinline__ void interp_wave(const int i0, int nt, const double4 *__restrict__ tpos,
-                                            const int *__restrict__ tperm, const pb_kgrid &g,
+                                            const int *__restrict__ tperm, const GR &g,
                                             const trip<T> *__restrict__ U3, double *__restrict__ u, int accumulate)
 {
     constexpr int TW = pb_interp_tw(P), NQ = P * P, NR = (TW * NQ + 63) / 64;
@@ -604,17 +609,17 @@ __global__ __launch_bounds__(64 * kInterpWaves) void k_ginterp(const pb_sys *__r
                                                              const int2 *__restrict__ map, int nblk8)
 {
     constexpr int TW = pb_interp_tw(P);
-    const int2 m = map[pb_xcd_block(blockIdx.x, nblk8)];
+    const int2 m = pb_map_at(map, pb_xcd_block(blockIdx.x, nblk8));
     if (m.x < 0) return;
-    const pb_sys &S = sys[m.x];
+    const auto &S = ((pb_const_sys)sys)[m.x];
     interp_wave<T, P>((m.y * kInterpWaves + (int)(threadIdx.x >> 6)) * TW, S.nt, S.tpos, S.tperm, S.g,
                       (const trip<T> *)S.grid4, S.u, S.acc_mode);
 }
 
 // Per-axis window sizes that differ (fftBrinkman's P[3]) take this path.
-template <typename T>
+template <typename T, typename GR>
 __device__ __forceinline__ void interp_any_target(const int i, int nt, const double4 *__restrict__ tpos,
-                                                  const int *__restrict__ tperm, const pb_kgrid &g,
+                                                  const int *__restrict__ tperm, const GR &g,
                                                   const trip<T> *__restrict__ U3, double *__restrict__ u,
                                                   int accumulate)
 {
@@ -672,9 +677,9 @@ __global__ __launch_bounds__(256) void k_interp_any(int nt, const double4 *__res
 template <typename T>
 __global__ __launch_bounds__(256) void k_ginterp_any(const pb_sys *__restrict__ sys, const int2 *__restrict__ map)
 {
-    const int2 m = map[blockIdx.x];
+    const int2 m = pb_map_at(map, blockIdx.x);
     if (m.x < 0) return;
-    const pb_sys &S = sys[m.x];
+    const auto &S = ((pb_const_sys)sys)[m.x];
     interp_any_target<T>(m.y * 256 + (int)threadIdx.x, S.nt, S.tpos, S.tperm, S.g, (const trip<T> *)S.grid4, S.u,
                          S.acc_mode);
 }
@@ -1154,16 +1159,17 @@ __global__ __launch_bounds__(256) void k_gfft(const pb_sys *__restrict__ sys, co
                                               const typename cplx<T>::type *__restrict__ tw)
 {
     using C = typename cplx<T>::type;
-    const int2 m = map[blockIdx.x];
-    const pb_sys &S = sys[m.x];
+    const int2 m = pb_map_at(map, blockIdx.x);
+    const auto &S = ((pb_const_sys)sys)[m.x];
     const int h2 = S.g.n[2] / 2 + 1;
     if constexpr (PASS == 0)
         fft_z_r2c<T, N>(m.y, (const typename cplx<T>::type4 *)S.grid4, (C *)S.spec, S.g.n[0], S.g.n[1], tw);
     else if constexpr (PASS == 1 || PASS == 3)
         fft_y<T, N>(m.y, (C *)S.spec, S.g.n[0], h2, PASS == 3, tw);
-    else if constexpr (PASS == 2)
-        fft_x_mult<T, N>(m.y, (C *)S.spec, S.g.n[1], h2, S.km, tw);
-    else
+    else if constexpr (PASS == 2) {
+        const kmult km = S.km;
+        fft_x_mult<T, N>(m.y, (C *)S.spec, S.g.n[1], h2, km, tw);
+    } else
         fft_z_c2r<T, N>(m.y, (const C *)S.spec, (trip<T> *)S.grid4, S.g.n[0], S.g.n[1], tw);
 }
 

@@ -487,22 +487,12 @@ __global__ __launch_bounds__(64 * kWaves) void k_pair_direct(PB_PAIR_PARAMS)
 // Grouped: every system of a class, block b on system map[b].x.
 #define PB_GPAIR_FWD                                                                                    \
     m.y, S.tiles, S.ntiles, S.sstart, S.spos, S.sfrc, S.tpos, S.tperm, nsplit, S.cells, S.box, S.kp, S.u, S.stats
-// The descriptors and maps are read through the constant address space: the
-// compiler may then reload a descriptor field (scalar load) instead of
-// keeping it live across the kernel.
-#if defined(__HIP_DEVICE_COMPILE__)
-#define PB_CONSTANT_AS __attribute__((address_space(4))) // the device's constant address space
-#else
-#define PB_CONSTANT_AS // (the host pass only parses the kernel bodies)
-#endif
-typedef const PB_CONSTANT_AS pb_sys *pb_const_sys;
-typedef const PB_CONSTANT_AS int *pb_const_map; // (system, block) pairs
 
 template <bool MIXED, int DEG>
 __global__ __launch_bounds__(64 * kWaves, PB_PAIR_OCC) void k_gpair_table(const pb_sys *__restrict__ sys_,
                                                                           const int2 *__restrict__ map_, int nsplit)
 {
-    const int2 m = make_int2(((pb_const_map)map_)[2 * blockIdx.x], ((pb_const_map)map_)[2 * blockIdx.x + 1]);
+    const int2 m = pb_map_at(map_, blockIdx.x);
     const auto &S = ((pb_const_sys)sys_)[m.x];
     pair_body<MIXED, true, DEG>(PB_GPAIR_FWD);
 }
@@ -511,7 +501,7 @@ template <bool MIXED>
 __global__ __launch_bounds__(64 * kWaves) void k_gpair_direct(const pb_sys *__restrict__ sys_,
                                                               const int2 *__restrict__ map_, int nsplit)
 {
-    const int2 m = make_int2(((pb_const_map)map_)[2 * blockIdx.x], ((pb_const_map)map_)[2 * blockIdx.x + 1]);
+    const int2 m = pb_map_at(map_, blockIdx.x);
     const auto &S = ((pb_const_sys)sys_)[m.x];
     pair_body<MIXED, false, PB_TAB_DEG>(PB_GPAIR_FWD);
 }

@@ -77,3 +77,26 @@ def test_multirhs_dense_staging_matches_single_sets():
         single = ctx.velocity(p, X, dev(F[r].astype(np.float64)), X).cpu().numpy()
         assert relL2(u[r], single) < 1e-6, (r, relL2(u[r], single))
     ctx.close()
+
+
+@pytest.mark.parametrize("P,grid", [(6, 16), (8, 16), (10, 32), (16, 32)])
+def test_multirhs_windows_match_single_sets(P, grid):
+    # window sizes other than the 12 of the cases above (the gather's one and
+    # four targets per wavefront, grids of 16 and 32): every set against the
+    # single-set call
+    import torch
+    ns, nt, R = 4000, 3000, 6
+    L = (1.0, 1.0, 1.0)
+    x, _, y = pb.uniform_system(pb.config_seed(10), ns, nt, L)
+    rng = np.random.default_rng(11)
+    F = rng.standard_normal((R, ns, 3)).astype(np.float32)
+    p = pb.Params(L=L, alpha=8.0, eps=1e-3, d=0.05, rc=0.12, grid=(grid,) * 3, P=P, precision="mixed")
+    ctx = pb.Context(0)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+    X, Y = dev(x), dev(y)
+    u = ctx.velocity_multirhs(p, X, dev(F), Y).cpu().numpy()
+    assert np.all(np.isfinite(u))
+    for r in range(R):
+        single = ctx.velocity(p, X, dev(F[r].astype(np.float64)), Y).cpu().numpy()
+        assert relL2(u[r], single) < 1e-6, (r, relL2(u[r], single))
+    ctx.close()
