@@ -574,11 +574,18 @@ int mrhs_kspace(pb_context *c, hipStream_t s, const pb_params *p, const pb_cells
         sys[r].spec = (char *)c->mr_spec.ptr + sbytes * q;
         sys[r].u = u + (size_t)r * 3 * nt;
     }
-    if ((st = pb_reserve(c, &c->mr_sys, sizeof(pb_sys) * R))) return st;
-    if ((st = pb_hip_check(c, hipMemcpyAsync(c->mr_sys.ptr, sys.data(), sizeof(pb_sys) * R, hipMemcpyHostToDevice, s),
-                           "descriptor upload")))
-        return st;
-    if ((st = pb_hip_check(c, hipStreamSynchronize(s), "descriptor upload"))) return st;
+    // uploaded (with a wait for the copy) only when they differ from the last
+    // call's: a repeated call enqueues its work without blocking the host
+    if (sys.size() != c->mr_sys_host.size() ||
+        std::memcmp(sys.data(), c->mr_sys_host.data(), sizeof(pb_sys) * sys.size()) != 0) {
+        c->mr_sys_host.clear();
+        if ((st = pb_reserve(c, &c->mr_sys, sizeof(pb_sys) * R))) return st;
+        if ((st = pb_hip_check(c, hipMemcpyAsync(c->mr_sys.ptr, sys.data(), sizeof(pb_sys) * R,
+                                                 hipMemcpyHostToDevice, s), "descriptor upload")))
+            return st;
+        if ((st = pb_hip_check(c, hipStreamSynchronize(s), "descriptor upload"))) return st;
+        c->mr_sys_host = sys;
+    }
     // maps: spread, the five FFT passes, the gather (XCD-renumbered: a multiple of 8)
     const int nreg = ((g.n[0] + 7) / 8) * ((g.n[1] + 7) / 8) * ((g.n[2] + 7) / 8);
     int per[7];

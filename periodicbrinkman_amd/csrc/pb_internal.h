@@ -212,6 +212,7 @@ struct pb_context {
     size_t group_counts = 0;    // ints to zero at the start of gb_int (the count arrays)
     // many right-hand sides: k space of up to 64 force sets per grouped launch
     pb_buffer mr_sys, mr_map, mr_frc, mr_grid, mr_spec;
+    std::vector<pb_sys> mr_sys_host; // the descriptors last uploaded to mr_sys
     long long group_items = 0;  // pair statistics items
     // the batched call's captured graph and the calls it replays
     hipGraphExec_t graph_exec = nullptr;
