@@ -88,3 +88,17 @@ def test_clustered_full_size_split_invariance(ctx):
     pm = pb.Params(alpha=25.0, eps=5e-4, grid=(256, 256, 256), P=16, tol=1e-10, precision="mixed").tune(n, n)
     um = ctx.velocity(pm, X, F, X).cpu().numpy()
     assert relL2(um, us[1]) < 1e-6
+
+
+def test_one_context_through_changing_cell_grids(ctx):
+    # one context, cell grids that grow, shrink and grow again (the binning's
+    # count arrays are cleared by each call's scan and refilled only when
+    # they are reallocated): every call against the oracle
+    L = (1.0, 1.0, 1.0)
+    x, f, y = pb.uniform_system(pb.config_seed(44), 1500, 1000, L)
+    for rc in (0.45, 0.2, 0.45, 0.12):
+        p = pb.Params(L=L, alpha=10.0, eps=1e-3, d=rc / 4.5, rc=rc, grid=(32, 32, 32), P=12,
+                      m=float(np.sqrt(np.pi * 12)))
+        want = po.velocity_ewald(y, x, f, p.L, p.alpha, p.d, p.eps, p.rc, p.grid, (p.P,) * 3, p.m)
+        got = ctx.velocity(p, dev(x), dev(f), dev(y)).cpu().numpy()
+        assert relL2(got, want) < 1e-10, (rc, relL2(got, want))
