@@ -131,6 +131,11 @@ int make_cells(pb_context *c, const pb_params *p, pb_cells *cells)
         int n = (int)std::floor(3.0 * p->L[i] / p->rc);
         if (n < 1) n = 1;
         if (n > 160) n = 160;
+        // an even count (cells only grow, so 3 cells still span rc): target
+        // tiles are 2 x 2 cells across, and an odd count leaves a last column
+        // one cell wide whose tiles pay a full candidate walk for half the
+        // targets (1.1e5 targets, rc = 0.089: 33 -> 32 cells, 2% faster)
+        if (n >= 8) n -= n % 2;
         cells->nc[i] = n;
         cells->nb[i] = (n + 3) / 4;
         cells->h[i] = p->L[i] / n;
