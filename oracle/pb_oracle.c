@@ -799,28 +799,47 @@ void pbo_kspace_spectral(double *u, const double *y, int nt, const double *x, co
     double *H[3];
     for (int c = 0; c < 3; c++) H[c] = calloc(2 * Ng, sizeof(double));
     int Pmax = P[0] > P[1] ? (P[0] > P[2] ? P[0] : P[2]) : (P[1] > P[2] ? P[1] : P[2]);
-    double *wx = malloc(sizeof(double) * 3 * Pmax), *wy = wx + Pmax, *wz = wx + 2 * Pmax;
-    /* spread (sequential: deterministic) */
-    for (int n = 0; n < ns; n++) {
-        double xc[3];
-        int f0[3];
-        for (int c = 0; c < 3; c++) xc[c] = wrap_coord(x[3 * n + c], L[c]);
-        window_weights(xc[0], h[0], ih[0], P[0], eta, &f0[0], wx);
-        window_weights(xc[1], h[1], ih[1], P[1], eta, &f0[1], wy);
-        window_weights(xc[2], h[2], ih[2], P[2], eta, &f0[2], wz);
-        for (int px = 0; px < P[0]; px++) {
-            int jx = ((f0[0] + px) % M0 + M0) % M0;
-            for (int py = 0; py < P[1]; py++) {
-                int jy = ((f0[1] + py) % M1 + M1) % M1;
-                double wxy = scale * wx[px] * wy[py];
-                for (int pz = 0; pz < P[2]; pz++) {
-                    int jz = ((f0[2] + pz) % M2 + M2) % M2;
-                    double wt = wxy * wz[pz];
-                    size_t idx = ((size_t)jx * M1 + jy) * M2 + jz;
-                    for (int c = 0; c < 3; c++) H[c][2 * idx] += f[3 * n + c] * wt;
+    /* spread (fftBrinkman.h:120-155).  Each thread owns a slab of x planes and
+     * adds, source by source in order, the window points in its slab: every
+     * grid point receives its sources in source order, so the sums are the
+     * same as those of one sequential pass, whatever the thread count. */
+#pragma omp parallel
+    {
+        const int nth = omp_get_num_threads(), th = omp_get_thread_num();
+        const int x0 = (int)((long)M0 * th / nth), x1 = (int)((long)M0 * (th + 1) / nth);
+        double *wx = malloc(sizeof(double) * 3 * Pmax), *wy = wx + Pmax, *wz = wx + 2 * Pmax;
+        for (int n = 0; n < ns && x0 < x1; n++) {
+            double xc[3];
+            int f0[3];
+            xc[0] = wrap_coord(x[3 * n], L[0]);
+            /* the window's planes f0 .. f0 + P - 1 (mod M0): skip sources whose
+             * window misses the slab before evaluating any weight */
+            int lo = (int)floor(xc[0] * ih[0] - 0.5 * P[0]) + 1, hit = 0;
+            for (int px = 0; px < P[0] && !hit; px++) {
+                int jx = ((lo + px) % M0 + M0) % M0;
+                hit = jx >= x0 && jx < x1;
+            }
+            if (!hit) continue;
+            for (int c = 1; c < 3; c++) xc[c] = wrap_coord(x[3 * n + c], L[c]);
+            window_weights(xc[0], h[0], ih[0], P[0], eta, &f0[0], wx);
+            window_weights(xc[1], h[1], ih[1], P[1], eta, &f0[1], wy);
+            window_weights(xc[2], h[2], ih[2], P[2], eta, &f0[2], wz);
+            for (int px = 0; px < P[0]; px++) {
+                int jx = ((f0[0] + px) % M0 + M0) % M0;
+                if (jx < x0 || jx >= x1) continue;
+                for (int py = 0; py < P[1]; py++) {
+                    int jy = ((f0[1] + py) % M1 + M1) % M1;
+                    double wxy = scale * wx[px] * wy[py];
+                    for (int pz = 0; pz < P[2]; pz++) {
+                        int jz = ((f0[2] + pz) % M2 + M2) % M2;
+                        double wt = wxy * wz[pz];
+                        size_t idx = ((size_t)jx * M1 + jy) * M2 + jz;
+                        for (int c = 0; c < 3; c++) H[c][2 * idx] += f[3 * n + c] * wt;
+                    }
                 }
             }
         }
+        free(wx);
     }
     for (int c = 0; c < 3; c++) pbo_fft3d(H[c], grid, -1);
     /* multiply by the Brinkman projection with the deconvolution e^{eta k^2/2}
@@ -885,7 +904,6 @@ void pbo_kspace_spectral(double *u, const double *y, int nt, const double *x, co
         }
         free(gx);
     }
-    free(wx);
     for (int c = 0; c < 3; c++) free(H[c]);
 }
 

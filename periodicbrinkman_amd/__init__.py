@@ -153,6 +153,13 @@ def _dptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _shape(t, shape, name):
+    """Raise PBError(PB_ERR_INVALID) unless tensor t has the given shape (a
+    mismatch would make the kernels read or write outside the arrays)."""
+    if tuple(t.shape) != tuple(int(v) for v in shape):
+        raise PBError(PB_ERR_INVALID, f"{name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+
+
 def _stream_ptr(stream):
     if stream is None:
         import torch
@@ -245,9 +252,13 @@ class Context:
 
     def _vel(self, fn, params, xn, f, y, u=None, stream=None):
         import torch
-        nt = y.shape[0]
+        ns, nt = xn.shape[0], y.shape[0]
+        _shape(xn, (ns, 3), "xn")
+        _shape(f, (ns, 3), "f")
+        _shape(y, (nt, 3), "y")
         if u is None:
             u = torch.empty((nt, 3), dtype=torch.float64, device=y.device)
+        _shape(u, (nt, 3), "u")
         p = params._struct()
         st = fn(self.handle, ctypes.byref(p), int(xn.shape[0]), _dptr(xn), _dptr(f), int(nt),
                 _dptr(y), _dptr(u), _stream_ptr(stream))
@@ -271,11 +282,17 @@ class Context:
         tensors; src_off, tgt_off: host sequences of nsys + 1 offsets."""
         import torch
         n = len(params)
+        if len(src_off) != n + 1 or len(tgt_off) != n + 1:
+            raise PBError(PB_ERR_INVALID, f"src_off and tgt_off need {n + 1} entries")
         structs = (_Params * n)(*[q._struct() for q in params])
         so = (ctypes.c_int * (n + 1))(*[int(v) for v in src_off])
         to = (ctypes.c_int * (n + 1))(*[int(v) for v in tgt_off])
+        _shape(xn, (int(src_off[-1]), 3), "xn")
+        _shape(f, (int(src_off[-1]), 3), "f")
+        _shape(y, (int(tgt_off[-1]), 3), "y")
         if u is None:
             u = torch.empty((y.shape[0], 3), dtype=torch.float64, device=y.device)
+        _shape(u, (int(tgt_off[-1]), 3), "u")
         st = lib().pb_velocity_batched(self.handle, n, structs, so, _dptr(xn), _dptr(f), to, _dptr(y), _dptr(u),
                                        _stream_ptr(stream))
         _check(st, self)
@@ -291,9 +308,13 @@ class Context:
         import torch
         if not (F.is_cuda and F.dtype == torch.float32 and F.is_contiguous() and F.dim() == 3):
             raise PBError(PB_ERR_INVALID, "expected a contiguous float32 CUDA tensor (R, ns, 3)")
-        R, nt = F.shape[0], y.shape[0]
+        R, ns, nt = F.shape[0], xn.shape[0], y.shape[0]
+        _shape(xn, (ns, 3), "xn")
+        _shape(F, (R, ns, 3), "F")
+        _shape(y, (nt, 3), "y")
         if u is None:
             u = torch.empty((R, nt, 3), dtype=torch.float64, device=y.device)
+        _shape(u, (R, nt, 3), "u")
         p = params._struct()
         st = lib().pb_velocity_multirhs(self.handle, ctypes.byref(p), int(xn.shape[0]), _dptr(xn), int(R),
                                         ctypes.c_void_p(F.data_ptr()), int(nt), _dptr(y), _dptr(u),
@@ -334,6 +355,10 @@ def periodicBrinkman(dA, dx, dx0, a, d, e, M, N, L, maxBlockSize=256):
     """dA += Ewald matrix (device tensors; dA is (3nt, 3ns) column-major, i.e. a
     Fortran-ordered view: pass dA.T of a contiguous (3ns, 3nt) tensor)."""
     nt, ns = dx.shape[0], dx0.shape[0]
+    _shape(dx, (nt, 3), "dx")
+    _shape(dx0, (ns, 3), "dx0")
+    if dA.numel() != 9 * nt * ns or not dA.is_contiguous():
+        raise PBError(PB_ERR_INVALID, f"dA needs 9 nt ns = {9 * nt * ns} contiguous elements")
     Lh = np.ascontiguousarray(np.asarray(L, dtype=np.float64).reshape(3))
     lda = 3 * nt
     st = lib().periodicBrinkman(ctypes.c_void_p(dA.data_ptr()), lda, _dptr(dx), nt, _dptr(dx0), ns,
@@ -394,6 +419,7 @@ def vR_batch(xh, a, d, e, stream=None):
     """9 values per point (column-major 3x3) for device tensor xh (n, 3)."""
     import torch
     n = xh.shape[0]
+    _shape(xh, (n, 3), "xh")
     A = torch.empty((n, 9), dtype=torch.float64, device=xh.device)
     _check(lib().pb_vR_batch(_dptr(A), _dptr(xh), n, float(a), float(d), float(e),
                              _stream_ptr(stream)))
@@ -403,6 +429,8 @@ def vR_batch(xh, a, d, e, stream=None):
 def vF_batch(xh, k, a, d, stream=None):
     import torch
     n = xh.shape[0]
+    _shape(xh, (n, 3), "xh")
+    _shape(k, (n, 3), "k")
     A = torch.empty((n, 9), dtype=torch.float64, device=xh.device)
     _check(lib().pb_vF_batch(_dptr(A), _dptr(xh), _dptr(k), n, float(a), float(d),
                              _stream_ptr(stream)))

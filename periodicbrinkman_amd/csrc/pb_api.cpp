@@ -153,18 +153,36 @@ int make_cells(pb_context *c, const pb_params *p, pb_cells *cells)
     return PB_OK;
 }
 
-int make_kgrid(pb_context *c, const pb_params *p, pb_kgrid *g)
+// The FFT supports powers of two in [4, 512] per axis, the window 1 <= P <=
+// min(32, n) points: checked before anything is enqueued (the gather wraps a
+// window index once, so a window wider than the grid would read outside it).
+int check_kgrid(pb_context *c, const pb_kgrid &g)
+{
+    for (int ax = 0; ax < 3; ax++) {
+        const int m = g.n[ax];
+        if (m < 4 || (m & (m - 1)) || m > 512)
+            return pb_fail(c, PB_ERR_UNSUPPORTED, "FFT grid sizes must be powers of two in [4, 512]");
+        if (g.P[ax] < 1 || g.P[ax] > 32 || g.P[ax] > m)
+            return pb_fail(c, PB_ERR_INVALID, "window points P must be in [1, min(32, grid)]");
+    }
+    return PB_OK;
+}
+
+// Window points per axis: P on every axis, or Paxis[i] (fftBrinkman's P[3]).
+int make_kgrid(pb_context *c, const pb_params *p, pb_kgrid *g, const int *Paxis = nullptr)
 {
     std::memset(g, 0, sizeof(*g));
     g->ng = 1;
     for (int i = 0; i < 3; i++) {
         if (p->grid[i] < 2) return pb_fail(c, PB_ERR_INVALID, "grid sizes must be >= 2 (run pb_tune)");
         g->n[i] = p->grid[i];
-        g->P[i] = p->P;
+        g->P[i] = Paxis ? Paxis[i] : p->P;
         g->h[i] = p->L[i] / p->grid[i];
         g->inv_h[i] = (double)p->grid[i] / p->L[i];
         g->ng *= p->grid[i];
     }
+    int st;
+    if ((st = check_kgrid(c, *g))) return st;
     double m = p->m > 0 ? p->m : std::sqrt(M_PI * p->P);
     double eta = p->P * g->h[0] / m; // fftBrinkman.h:113-114
     g->eta = eta * eta / 2;
@@ -286,6 +304,8 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
     pb_fill_box(&box, p->L);
     pb_cells cells;
     if ((st = make_cells(c, p, &cells))) return st;
+    pb_kgrid g;
+    if (do_k && (st = make_kgrid(c, p, &g))) return st; // (validated before anything is enqueued)
     if ((st = pb_bin_all(c, s, cells, box, ns, xn, f, nt, y))) return st;
     pb_mark(c, s, 1);
     // Both halves: the pair kernel on a stream of its own, concurrently with
@@ -305,19 +325,16 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
         pb_zero_u(s, nt, u);
         if ((st = pb_hip_check(c, hipEventRecord(c->pair_event[0], s), "fork"))) return st;
         if ((st = pb_hip_check(c, hipStreamWaitEvent(c->pair_stream, c->pair_event[0], 0), "fork"))) return st;
-        if ((st = pb_run_real(c, c->pair_stream, cells, kp, ntab, tdeg, box, ns, nt, u, true,
-                              far32_of(p, ntab))))
-            return st;
-        if ((st = pb_hip_check(c, hipEventRecord(c->pair_event[1], c->pair_stream), "join"))) return st;
-        pb_kgrid g;
-        if ((st = make_kgrid(c, p, &g))) return st;
-        if ((st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u, 2, p->precision))) return st;
-        return pb_hip_check(c, hipStreamWaitEvent(s, c->pair_event[1], 0), "join");
+        // from here every return joins the pair stream back into s, so no
+        // work of this call is left running beside the caller's next call
+        st = pb_run_real(c, c->pair_stream, cells, kp, ntab, tdeg, box, ns, nt, u, true, far32_of(p, ntab));
+        const int jst = pb_hip_check(c, hipEventRecord(c->pair_event[1], c->pair_stream), "join");
+        if (!st && !jst) st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u, 2, p->precision);
+        const int wst = jst ? jst : pb_hip_check(c, hipStreamWaitEvent(s, c->pair_event[1], 0), "join");
+        return st ? st : wst;
     }
     bool acc = false;
     if (do_k) {
-        pb_kgrid g;
-        if ((st = make_kgrid(c, p, &g))) return st;
         if ((st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u, 0, p->precision))) return st;
         acc = true;
     }
@@ -348,15 +365,26 @@ int velocity_impl(pb_context *c, const pb_params *p, int ns, const double *xn, c
     return PB_OK;
 }
 
-// A default context per host thread for the reference-signature entry points.
+// A default context per host thread and device for the reference-signature
+// entry points (created on the thread's current device at first use,
+// destroyed when the thread ends).
+struct default_ctx_holder {
+    std::vector<pb_context *> ctx; // by device ordinal
+    ~default_ctx_holder()
+    {
+        for (pb_context *c : ctx)
+            if (c) pb_destroy(c);
+    }
+};
+
 pb_context *default_ctx()
 {
-    static thread_local pb_context *ctx = nullptr;
-    if (!ctx) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess || pb_create(&ctx, dev) != PB_OK) ctx = nullptr;
-    }
-    return ctx;
+    static thread_local default_ctx_holder h;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return nullptr;
+    if ((int)h.ctx.size() <= dev) h.ctx.resize(dev + 1, nullptr);
+    if (!h.ctx[dev] && pb_create(&h.ctx[dev], dev) != PB_OK) h.ctx[dev] = nullptr;
+    return h.ctx[dev];
 }
 
 // hipMemcpy that folds failures into a status
@@ -405,10 +433,12 @@ int shells_impl(double *A, double *absA, int lda, const double *x, int nt, const
     pb_box box;
     pb_fill_box(&box, L);
     const pb_kparams &kp = cached_kparams(a, d, e, 0.0);
+    pb_context *c = default_ctx();
+    if (!c) return PB_ERR_HIP;
     if (fourier)
-        st = pb_run_dense(nullptr, dA, lda, dx, nt, dx0, ns, kp, box, nullptr, nullptr, 0, oldM, newM, 1, dB);
+        st = pb_run_dense(c, nullptr, dA, lda, dx, nt, dx0, ns, kp, box, nullptr, nullptr, 0, oldM, newM, 1, dB);
     else
-        st = pb_run_dense(nullptr, dA, lda, dx, nt, dx0, ns, kp, box, oldM, newM, home ? 1 : 2, nullptr,
+        st = pb_run_dense(c, nullptr, dA, lda, dx, nt, dx0, ns, kp, box, oldM, newM, home ? 1 : 2, nullptr,
                           nullptr, 0, dB);
     if (st) return st;
     if (hipDeviceSynchronize() != hipSuccess) return PB_ERR_HIP;
@@ -534,6 +564,7 @@ int pb_velocity_kspace(pb_context *ctx, const pb_params *p, int ns, const double
     return velocity_impl(ctx, p, ns, xn, f, nt, y, u, stream, false, true);
 }
 
+
 namespace {
 
 size_t pad16(size_t n) { return (n + 15) & ~(size_t)15; }
@@ -658,19 +689,24 @@ int pb_velocity_multirhs(pb_context *c, const pb_params *p, int ns, const double
     pb_fill_box(&box, p->L);
     pb_cells cells;
     if ((st = make_cells(c, p, &cells))) return st;
+    pb_kgrid g;
+    if ((st = make_kgrid(c, p, &g))) return st;
+    // always the mixed form: the table at the mixed form's degree and
+    // tolerance, built (and checked to exist) before anything is enqueued
+    const double rfar = std::fmax(3 * p->d, 8 * p->eps);
+    pb_params pt = *p;
+    pt.kernel = PB_KERNEL_TABLE;
+    pt.precision = PB_PREC_MIXED;
+    const pb_kparams *kp = nullptr;
+    size_t ntab = 0;
+    if ((st = device_kparams(c, s, &pt, rfar, &kp, &ntab))) return st;
+    if (nt > 0 && !ntab)
+        return pb_fail(c, PB_ERR_UNSUPPORTED, "the multi-RHS path needs the radial table (cutoff rc too small)");
     const int RP = (R + 31) / 32 * 32;
     if ((st = pb_bin_all(c, s, cells, box, ns, xn, nullptr, nt, y))) return st;
     if ((st = pb_sort_forces(c, s, ns, R, RP, f))) return st;
     if (nt == 0) return PB_OK;
-    pb_kgrid g;
-    if ((st = make_kgrid(c, p, &g))) return st;
     if ((st = mrhs_kspace(c, s, p, cells, box, g, ns, nt, R, RP, u))) return st;
-    double rfar = std::fmax(3 * p->d, 8 * p->eps);
-    pb_params pt = *p;
-    pt.kernel = PB_KERNEL_TABLE;
-    const pb_kparams *kp = nullptr;
-    size_t ntab = 0;
-    if ((st = device_kparams(c, s, &pt, rfar, &kp, &ntab))) return st;
     return pb_run_mrhs_real(c, s, cells, kp, ntab, c->table_deg, box, ns, nt, R, RP, u);
 }
 
@@ -704,14 +740,7 @@ int group_prepare(pb_context *c, hipStream_t s, const std::vector<int> &idx, con
         std::memset(&S, 0, sizeof S);
         pb_fill_box(&S.box, p.L);
         if ((st = make_cells(c, &p, &S.cells))) return st;
-        if ((st = make_kgrid(c, &p, &S.g))) return st;
-        for (int ax = 0; ax < 3; ax++) {
-            const int m = S.g.n[ax];
-            if (m < 4 || (m & (m - 1)) || m > 512)
-                return pb_fail(c, PB_ERR_UNSUPPORTED, "FFT grid sizes must be powers of two in [4, 512]");
-            if (S.g.P[ax] < 1 || S.g.P[ax] > 32 || S.g.P[ax] > m)
-                return pb_fail(c, PB_ERR_INVALID, "window points P must be in [1, min(32, grid)]");
-        }
+        if ((st = make_kgrid(c, &p, &S.g))) return st; // (checks the grid and window)
         S.km = pb_make_kmult(S.g, S.box, p.alpha, p.d);
         S.ns = src_off[i + 1] - src_off[i];
         S.nt = tgt_off[i + 1] - tgt_off[i];
@@ -975,6 +1004,8 @@ int pb_velocity_batched(pb_context *c, int nsys, pb_params *params, const int *s
             if ((st = pb_autotune(&p, ns, nt, nullptr))) return pb_fail(c, st, "autotune failed for a system");
         }
         if ((st = check_params(c, &p))) return st;
+        pb_kgrid g;
+        if ((st = make_kgrid(c, &p, &g))) return st; // every system's grid and window, before any launch
     }
     if ((src_off[nsys] > 0 && (!xn || !f)) || (tgt_off[nsys] > 0 && (!y || !u)))
         return pb_fail(c, PB_ERR_INVALID, "null point array");
@@ -1233,9 +1264,14 @@ int periodicBrinkman(double *dA, int lda, const double *dx, int nt, const double
     const pb_kparams &kp = cached_kparams(a, d, e, 0.0);
     int zero[3] = {0, 0, 0};
     int st = PB_OK;
-    if (N > 0) st = pb_run_dense(nullptr, dA, lda, dx, nt, dx0, ns, kp, box, nullptr, nullptr, 0, zero, Nx, 1, nullptr);
+    if (nt == 0 || ns == 0) return PB_OK;
+    pb_context *c = default_ctx();
+    if (!c) return PB_ERR_HIP;
+    // both launches on the null stream with no host synchronization, as the
+    // reference's (periodicBrinkman.h:500,511)
+    if (N > 0) st = pb_run_dense(c, nullptr, dA, lda, dx, nt, dx0, ns, kp, box, nullptr, nullptr, 0, zero, Nx, 1, nullptr);
     if (st) return st;
-    if (M >= 0) st = pb_run_dense(nullptr, dA, lda, dx, nt, dx0, ns, kp, box, zero, Mx, 3, nullptr, nullptr, 0, nullptr);
+    if (M >= 0) st = pb_run_dense(c, nullptr, dA, lda, dx, nt, dx0, ns, kp, box, zero, Mx, 3, nullptr, nullptr, 0, nullptr);
     return st;
 }
 
@@ -1345,11 +1381,14 @@ int fftBrinkman(double *U, const double *y, int m, const double *f, const double
     p.m = msd;
     p.P = P[0]; // eta follows axis 0 (fftBrinkman.h:113-114)
     p.rc = std::fmin(L[0], std::fmin(L[1], L[2])); // binning only
+    pb_kgrid g;
+    int st;
+    if ((st = make_kgrid(c, &p, &g, P))) return st; // grid and windows checked before any copy
     dev_tmp t;
     double *dy = t.alloc(3 * (size_t)m), *df = t.alloc(3 * (size_t)N), *dx = t.alloc(3 * (size_t)N),
            *du = t.alloc(3 * (size_t)m);
     if (!dy || !df || !dx || !du) return PB_ERR_NOMEM;
-    int st = copy(dy, y, 3 * (size_t)m * sizeof(double), hipMemcpyHostToDevice);
+    st = copy(dy, y, 3 * (size_t)m * sizeof(double), hipMemcpyHostToDevice);
     if (!st) st = copy(df, f, 3 * (size_t)N * sizeof(double), hipMemcpyHostToDevice);
     if (!st) st = copy(dx, xn, 3 * (size_t)N * sizeof(double), hipMemcpyHostToDevice);
     if (st) return st;
@@ -1368,9 +1407,6 @@ int fftBrinkman(double *U, const double *y, int m, const double *f, const double
     cells.nkey_t = 64;
     if ((st = pb_bin_points(c, nullptr, cells, box, N, dx, df, true))) return st;
     if ((st = pb_bin_points(c, nullptr, cells, box, m, dy, nullptr, false))) return st;
-    pb_kgrid g;
-    if ((st = make_kgrid(c, &p, &g))) return st;
-    for (int i = 0; i < 3; i++) g.P[i] = P[i];
     if ((st = pb_run_kspace(c, nullptr, g, cells, box, alpha, d, N, m, du, 0, PB_PREC_FP64))) return st;
     if (hipDeviceSynchronize() != hipSuccess) return PB_ERR_HIP;
     return copy(U, du, 3 * (size_t)m * sizeof(double), hipMemcpyDeviceToHost);

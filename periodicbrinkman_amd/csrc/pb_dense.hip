@@ -6,11 +6,11 @@
 // One thread owns one (target, source) 3x3 block and loops over the image
 // set and the k set, so no reduction is needed; consecutive threads take
 // consecutive targets (coalesced column segments of A).  The k-space factor
-// b(k) = phi_d(k)/(k^2 (k^2 + a^2)) / tau is tabulated once per call.
+// b(k) = phi_d(k)/(k^2 (k^2 + a^2)) / tau is tabulated once per call, by a
+// kernel on the same stream (the call enqueues work and returns).
 #include "pb_radial.h"
 
 #include <cmath>
-#include <vector>
 
 namespace {
 
@@ -64,7 +64,8 @@ __global__ __launch_bounds__(256) void k_dense(double *__restrict__ A, int lda,
                 }
     }
     for (int q = 0; q < nk; q++) {
-        double4 kk = ktab[q]; // (k0, k1, k2, b(k)/tau)
+        const double4 kk = ktab[q]; // (k0, k1, k2, b(k)/tau); 0 outside the layer and at k = 0
+        if (kk.w == 0.0) continue;  // (the same entry for every thread: a uniform branch)
         double bh = kk.w * cos(kk.x * r0 + kk.y * r1 + kk.z * r2);
         double kv[3] = {kk.x, kk.y, kk.z};
 #pragma unroll
@@ -91,6 +92,30 @@ __global__ __launch_bounds__(256) void k_dense(double *__restrict__ A, int lda,
                 if (absA) absA[o] += aab[b * 3 + a];
             }
         }
+}
+
+// k table of the cube |l_i| <= hi_i: entry q = (k, b(k)/tau) with
+// b = (1 + kappa |k|^2) e^{-d^2 |k|^2/4} / (|k|^2 (|k|^2 + a^2)), weight 0 for
+// the inner cube |l_i| <= lo_i and for k = 0 (periodicBrinkman.h:189-195, with
+// the 1/tau of fourierShells :718,745).
+struct kcube {
+    int lo[3], hi[3];
+    double k[3];                    // 2 pi / L per axis
+    double kappa, d2q, a2, inv_tau; // (1 - e^{-a^2 d^2/4})/a^2, d^2/4, a^2, 1/(Lx Ly Lz)
+};
+
+__global__ void k_ktable(kcube kc, long long n, double4 *__restrict__ out)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int n0 = 2 * kc.hi[0] + 1, n1 = 2 * kc.hi[1] + 1;
+    const int l0 = (int)(q % n0) - kc.hi[0], l1 = (int)((q / n0) % n1) - kc.hi[1], l2 = (int)(q / n0 / n1) - kc.hi[2];
+    const double k0 = kc.k[0] * l0, k1 = kc.k[1] * l1, k2 = kc.k[2] * l2;
+    const double kn = k0 * k0 + k1 * k1 + k2 * k2;
+    const bool inner = abs(l0) <= kc.lo[0] && abs(l1) <= kc.lo[1] && abs(l2) <= kc.lo[2];
+    double b = 0.0;
+    if (!inner && kn > 0.0) b = (1.0 + kc.kappa * kn) * exp(-kc.d2q * kn) / (kn * (kn + kc.a2)) * kc.inv_tau;
+    out[q] = make_double4(k0, k1, k2, b);
 }
 
 __global__ void k_vR_batch(double *__restrict__ A, const double *__restrict__ xh, int n,
@@ -128,54 +153,46 @@ __global__ void k_vF_batch(double *__restrict__ A, const double *__restrict__ xh
 
 } // namespace
 
-int pb_run_dense(hipStream_t s, double *dA, int lda, const double *y, int nt, const double *x,
-                 int ns, const pb_kparams &kp, const pb_box &box, const int *lo_real,
-                 const int *hi_real, int real_mode, const int *lo_k, const int *hi_k, int k_mode,
-                 double *absA)
+int pb_run_dense(pb_context *c, hipStream_t s, double *dA, int lda, const double *y, int nt, const double *x,
+                 int ns, const pb_kparams &kp, const pb_box &box, const int *lo_real, const int *hi_real,
+                 int real_mode, const int *lo_k, const int *hi_k, int k_mode, double *absA)
 {
-    // k table on the host: the k layer hi_k \ lo_k, k = 0 excluded
-    std::vector<double4> kt;
+    // k table of the whole cube hi_k, built on the device into the context's
+    // buffer (no host upload, no synchronization: the call stays
+    // asynchronous); entries of the inner cube lo_k and k = 0 get weight 0
+    int nk = 0;
+    const double4 *dk = nullptr;
     if (k_mode) {
-        double tau = box.L[0] * box.L[1] * box.L[2];
-        double a = kp.pd.a, d = kp.pd.d;
-        double X = a * d * a * d, a1 = -std::expm1(-X / 4);
-        for (int l2 = -hi_k[2]; l2 <= hi_k[2]; l2++)
-            for (int l1 = -hi_k[1]; l1 <= hi_k[1]; l1++)
-                for (int l0 = -hi_k[0]; l0 <= hi_k[0]; l0++) {
-                    if (std::abs(l0) <= lo_k[0] && std::abs(l1) <= lo_k[1] && std::abs(l2) <= lo_k[2])
-                        continue;
-                    if (l0 == 0 && l1 == 0 && l2 == 0) continue;
-                    double k0 = 2 * M_PI * l0 / box.L[0], k1 = 2 * M_PI * l1 / box.L[1],
-                           k2 = 2 * M_PI * l2 / box.L[2];
-                    double kn = k0 * k0 + k1 * k1 + k2 * k2;
-                    double b = (1.0 + a1 * d * d * kn / X) * std::exp(-d * d * kn / 4) / (kn * (kn + a * a)) / tau;
-                    kt.push_back(make_double4(k0, k1, k2, b));
-                }
-    }
-    double4 *dk = nullptr;
-    if (!kt.empty()) {
-        if (hipMalloc(&dk, kt.size() * sizeof(double4)) != hipSuccess) return PB_ERR_NOMEM;
-        if (hipMemcpyAsync(dk, kt.data(), kt.size() * sizeof(double4), hipMemcpyHostToDevice, s) !=
-            hipSuccess) {
-            (void)hipFree(dk);
-            return PB_ERR_HIP;
+        const long long n = (2LL * hi_k[0] + 1) * (2LL * hi_k[1] + 1) * (2LL * hi_k[2] + 1);
+        if (n > (1LL << 28)) return pb_fail(c, PB_ERR_UNSUPPORTED, "too many Fourier shells");
+        int st;
+        if ((st = pb_reserve(c, &c->dense_tmp, sizeof(double4) * (size_t)n))) return st;
+        kcube kc;
+        for (int q = 0; q < 3; q++) {
+            kc.lo[q] = lo_k[q];
+            kc.hi[q] = hi_k[q];
         }
+        const double a = kp.pd.a, d = kp.pd.d, X = a * d * a * d;
+        kc.kappa = -std::expm1(-X / 4) * d * d / X; // (1 - e^{-a^2 d^2/4}) / a^2
+        kc.d2q = d * d / 4;
+        kc.a2 = a * a;
+        kc.inv_tau = 1.0 / (box.L[0] * box.L[1] * box.L[2]);
+        for (int q = 0; q < 3; q++) kc.k[q] = 2 * M_PI / box.L[q];
+        hipLaunchKernelGGL(k_ktable, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, kc, n,
+                           (double4 *)c->dense_tmp.ptr);
+        nk = (int)n;
+        dk = (const double4 *)c->dense_tmp.ptr;
     }
     box3i rb;
-    for (int c = 0; c < 3; c++) {
-        rb.lo[c] = lo_real ? lo_real[c] : 0;
-        rb.hi[c] = hi_real ? hi_real[c] : 0;
+    for (int q = 0; q < 3; q++) {
+        rb.lo[q] = lo_real ? lo_real[q] : 0;
+        rb.hi[q] = hi_real ? hi_real[q] : 0;
     }
     long long npair = (long long)nt * ns;
     if (npair > 0)
         hipLaunchKernelGGL(k_dense, dim3((unsigned)((npair + 255) / 256)), dim3(256), 0, s, dA, lda, y,
-                           nt, x, ns, kp, box, rb, real_mode, dk, (int)kt.size(), absA);
-    hipError_t e = hipGetLastError();
-    if (dk) {
-        if (hipStreamSynchronize(s) != hipSuccess) e = hipErrorUnknown;
-        (void)hipFree(dk);
-    }
-    return e == hipSuccess ? PB_OK : PB_ERR_HIP;
+                           nt, x, ns, kp, box, rb, real_mode, dk, nk, absA);
+    return pb_hip_check(c, hipGetLastError(), "dense launch");
 }
 
 int pb_run_vR_batch(hipStream_t s, double *A, const double *xh, int n, const pb_kparams &kp)

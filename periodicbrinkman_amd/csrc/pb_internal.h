@@ -363,10 +363,10 @@ int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kp
 int pb_run_kspace(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells,
                   const pb_box &box, double a, double d, int ns, int nt, double *u, int accumulate,
                   int precision);
-int pb_run_dense(hipStream_t s, double *dA, int lda, const double *y, int nt, const double *x,
-                 int ns, const pb_kparams &kp, const pb_box &box, const int *lo_real,
-                 const int *hi_real, int real_mode, const int *lo_k, const int *hi_k,
-                 int k_mode, double *absA);
+// The k table goes to c->dense_tmp (built on the device on stream s).
+int pb_run_dense(pb_context *c, hipStream_t s, double *dA, int lda, const double *y, int nt, const double *x,
+                 int ns, const pb_kparams &kp, const pb_box &box, const int *lo_real, const int *hi_real,
+                 int real_mode, const int *lo_k, const int *hi_k, int k_mode, double *absA);
 int pb_run_vR_batch(hipStream_t s, double *A, const double *xh, int n, const pb_kparams &kp);
 int pb_run_vF_batch(hipStream_t s, double *A, const double *xh, const double *k, int n,
                     double a, double d);

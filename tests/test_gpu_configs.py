@@ -1,0 +1,201 @@
+"""The benchmark workloads at their full sizes, through the C ABI, against the
+FP64 oracle on the same seeded inputs.
+
+Each evaluation is compared with the oracle at the same parameters (d, rc,
+grid, P, m): the FP64 forms to rounding, the mixed form within 1e-6 relative
+L2.  Where the oracle's cost grows with the targets, it runs on a fixed
+subsample of targets against all sources.  The mixed form's error against a
+converged FP64 Ewald sum (tighter split, cutoff and window) is checked too:
+that is the accuracy the benchmark reports.
+
+  config 1: 1000 sources, targets = sources, alpha 10, eps 0.01, 32^3, P 8,
+            split for 1e-8 -- pb_velocity, and the reference's own sequence
+            (realShells home + layer to M = 1, fftBrinkman) in full;
+  config 2: 1e5 sources -> 1e5 targets, alpha 10, eps 1e-3, 128^3, P 12;
+  config 4: 512 systems of uneven size, autotuned, one batched call;
+  config 5: 2e5 particles x 64 FP32 force sets, 128^3, P 12.
+"""
+import numpy as np
+import pytest
+
+import periodicbrinkman_amd as pb
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def relL2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def dev(a, dtype=np.float64):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=dtype), device="cuda")
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = pb.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def oracle_threads():
+    po.set_threads(0)  # every host core for the oracle
+    yield
+
+
+def oracle(y, x, f, p):
+    return po.velocity_ewald(y, x, f, p.L, p.alpha, p.d, p.eps, p.rc, p.grid, (p.P,) * 3, p.m)
+
+
+# ---- configuration 1 ------------------------------------------------------------
+def config1():
+    n = 1000
+    x, f, _ = pb.uniform_system(pb.config_seed(0), n, 0)
+    p = pb.Params(alpha=10.0, eps=0.01, grid=(32, 32, 32), P=8, tol=1e-8).tune(n, n)
+    return x, f, p
+
+
+def test_config1_velocity_full(ctx):
+    import torch
+    x, f, p = config1()
+    want = oracle(x, x, f, p)
+    for prec, kernel, bar in (("fp64", "table", 1e-11), ("fp64", "direct", 1e-11), ("mixed", "table", 1e-6)):
+        say(f"config 1 pb_velocity {prec} {kernel}")
+        u = ctx.velocity(p.copy(precision=prec, kernel=kernel), dev(x), dev(f), dev(x)).cpu().numpy()
+        torch.cuda.synchronize()
+        assert relL2(u, want) < bar, (prec, kernel, relL2(u, want))
+
+
+def test_config1_reference_sequence_full():
+    # the reference's main(): realShells for the home box, then the layers to
+    # M = 1 (27 images), as a 3n x 3n matrix applied to the forces
+    # (periodicBrinkman.cu:176-183, periodicBrinkman.h:564-693); the k-space part
+    # by fftBrinkman (fftBrinkman.h:34-287) with an integer window half-width
+    x, f, p = config1()
+    n = x.shape[0]
+    msd = 5
+    A = np.zeros((3 * n, 3 * n), order="F")
+    B = np.zeros_like(A)
+    A2, B2 = A.copy(order="F"), B.copy(order="F")
+    say("config 1 realShells home + layer")
+    pb.realShells(A, B, x, x, [0, 0, 0], [0, 0, 0], p.L, p.alpha, p.d, p.eps)
+    pb.realShells(A, B, x, x, [1, 1, 1], [0, 0, 0], p.L, p.alpha, p.d, p.eps)
+    po.realShells(A2, B2, x, x, [0, 0, 0], [0, 0, 0], p.L, p.alpha, p.d, p.eps)
+    po.realShells(A2, B2, x, x, [1, 1, 1], [0, 0, 0], p.L, p.alpha, p.d, p.eps)
+    assert np.max(np.abs(A - A2)) <= 1e-12 * np.max(np.abs(A2))
+    assert np.max(np.abs(B - B2)) <= 1e-12 * np.max(np.abs(B2))
+    say("config 1 fftBrinkman")
+    Uk = pb.fftBrinkman(x, f, x, p.L, p.grid, (p.P,) * 3, msd, 0, p.eps, 1e-8, p.d, p.alpha)
+    Uk2 = po.fftBrinkman(x, f, x, p.L, p.grid, (p.P,) * 3, msd, 0, p.eps, 1e-8, p.d, p.alpha)
+    assert relL2(Uk, Uk2) <= 1e-12
+    u = (A @ f.reshape(-1)).reshape(n, 3) + Uk
+    want = po.real_velocity_cube(x, x, f, p.L, p.alpha, p.d, p.eps, (1, 1, 1)) + \
+        po.kspace_spectral(x, x, f, p.L, p.alpha, p.d, p.grid, (p.P,) * 3, float(msd))
+    assert relL2(u, want) <= 1e-12
+    # the 27 images hold every pair within rc (rc < L/2 ... L): the cutoff
+    # evaluator gives the same sum up to the pairs beyond rc (< e^{-(rc/d)^2})
+    assert p.rc < 1.0
+    ucut = oracle(x, x, f, p.copy(m=float(msd)))
+    assert relL2(u, ucut) <= 1e-7
+
+
+# ---- configuration 2 ------------------------------------------------------------
+def test_config2_headline_full_size(ctx):
+    import torch
+    ns = nt = 100_000
+    x, f, y = pb.uniform_system(pb.config_seed(1), ns, nt)
+    p = pb.Params(alpha=10.0, eps=1e-3, grid=(128, 128, 128), P=12, tol=4e-6).tune(ns, nt)
+    idx = np.linspace(0, nt - 1, 256).astype(np.int64)
+    say("config 2 oracle, same parameters")
+    want = oracle(y[idx], x, f, p)
+    X, F, Y = dev(x), dev(f), dev(y)
+    got = {}
+    for prec, kernel, bar in (("fp64", "table", 1e-10), ("fp64", "direct", 1e-10), ("mixed", "table", 1e-6),
+                              ("mixed", "table_far32", 1e-6)):
+        say(f"config 2 pb_velocity {prec} {kernel}")
+        u = ctx.velocity(p.copy(precision=prec, kernel=kernel), X, F, Y).cpu().numpy()
+        torch.cuda.synchronize()
+        assert np.all(np.isfinite(u))
+        assert relL2(u[idx], want) < bar, (prec, kernel, relL2(u[idx], want))
+        got[(prec, kernel)] = u
+    # the accuracy of the benchmark's parameters: against a converged sum
+    # (another split, cutoff 5.7 d, window P 20)
+    say("config 2 oracle, converged")
+    pc = pb.Params(alpha=10.0, eps=1e-3, grid=(128, 128, 128), P=20, tol=1e-11).tune(ns, nt)
+    conv = oracle(y[idx], x, f, pc)
+    assert relL2(got[("mixed", "table")][idx], conv) <= 1e-6
+    assert relL2(got[("fp64", "table")][idx], conv) <= 1e-6
+
+
+# ---- configuration 4 ------------------------------------------------------------
+@pytest.mark.parametrize("prec,tol,bar", [("mixed", 3e-6, 1e-6), ("fp64", 1e-8, 1e-10)])
+def test_config4_batched_512_systems(ctx, prec, tol, bar):
+    import torch
+    systems, x, f, off = pb.batched_systems(pb.config_seed(3))
+    assert len(systems) == 512 and off[-1] > 1e6
+    X, F = dev(x), dev(f)
+    U = torch.empty_like(X)
+    ps = [pb.Params(L=(s["L"],) * 3, alpha=s["alpha"], eps=s["eps"], precision=prec, tol=tol) for s in systems]
+    say(f"config 4 batched {prec}: eager, capture, replay")
+    ctx.velocity_batched(ps, X, F, off, X, off, U)  # eager (sizes every buffer)
+    u1 = U.cpu().numpy().copy()
+    ctx.velocity_batched(ps, X, F, off, X, off, U)  # captures the graph
+    ctx.velocity_batched(ps, X, F, off, X, off, U)  # replays it
+    torch.cuda.synchronize()
+    u = U.cpu().numpy()
+    assert relL2(u, u1) < (1e-13 if prec == "fp64" else 1e-6)
+    say(f"config 4 oracle per system ({prec})")
+    worst = 0.0
+    for i, (s, q) in enumerate(zip(systems, ps)):
+        lo, hi = int(off[i]), int(off[i + 1])
+        xi, fi = x[lo:hi], f[lo:hi]
+        k = np.linspace(0, hi - lo - 1, min(hi - lo, 48)).astype(np.int64)
+        e = relL2(u[lo:hi][k], oracle(xi[k], xi, fi, q))
+        worst = max(worst, e)
+        assert e < bar, (prec, i, s, q, e)
+    say(f"config 4 {prec}: worst system rel L2 vs oracle {worst:.2e}")
+    if prec == "mixed":
+        # against converged sums, every 16th system
+        for i in range(0, 512, 16):
+            s = systems[i]
+            lo, hi = int(off[i]), int(off[i + 1])
+            xi, fi = x[lo:hi], f[lo:hi]
+            k = np.linspace(0, hi - lo - 1, min(hi - lo, 48)).astype(np.int64)
+            qc = pb.Params(L=(s["L"],) * 3, alpha=s["alpha"], eps=s["eps"], precision="fp64", tol=1e-11)
+            qc.autotune(hi - lo, hi - lo)
+            assert relL2(u[lo:hi][k], oracle(xi[k], xi, fi, qc)) <= 1e-6, i
+
+
+# ---- configuration 5 ------------------------------------------------------------
+def test_config5_multirhs_2e5_by_64(ctx):
+    import torch
+    n, R = 200_000, 64
+    x, _, _ = pb.uniform_system(pb.config_seed(4), n, 0)
+    Fh = pb.Rng(pb.config_seed(4) + (1 << 32)).normal(3 * n * R).reshape(R, n, 3).astype(np.float32)
+    p = pb.Params(alpha=10.0, eps=1e-3, grid=(128, 128, 128), P=12, tol=4e-6, precision="mixed").tune(n, n)
+    X = dev(x)
+    say("config 5 multi-RHS call")
+    U = ctx.velocity_multirhs(p, X, dev(Fh, np.float32), X)
+    torch.cuda.synchronize()
+    u = U.cpu().numpy()
+    assert u.shape == (R, n, 3) and np.all(np.isfinite(u))
+    del U
+    say("config 5 single-set calls")
+    V = torch.empty_like(X)
+    for r in range(R):
+        ctx.velocity(p, X, dev(Fh[r], np.float64), X, V)
+        assert relL2(u[r], V.cpu().numpy()) < 1e-6, r
+    idx = np.linspace(0, n - 1, 256).astype(np.int64)
+    pc = pb.Params(alpha=10.0, eps=1e-3, grid=(128, 128, 128), P=20, tol=1e-11).tune(n, n)
+    for r in (0, R // 2, R - 1):
+        say(f"config 5 oracle, set {r}")
+        fr = Fh[r].astype(np.float64)
+        assert relL2(u[r][idx], oracle(x[idx], x, fr, p)) < 1e-6, r
+        assert relL2(u[r][idx], oracle(x[idx], x, fr, pc)) <= 1e-6, r

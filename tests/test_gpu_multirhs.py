@@ -127,3 +127,70 @@ def test_multirhs_repeated_calls_follow_the_forces():
             single = ctx.velocity(p, X, dev(Fh[r].astype(np.float64)), Y).cpu().numpy()
             assert relL2(u[r], single) < 1e-6, (k, r, relL2(u[r], single))
     ctx.close()
+
+
+def test_multirhs_more_than_64_sets():
+    # R = 80: two k-space chunks are not needed (the grids fit), but the
+    # real space runs a 64-set and a 32-set contraction (RP = 96) and the
+    # spreading takes 10 groups of 8 sets; the last set (79) against the
+    # single-set call and the oracle
+    import torch
+    ns, nt, R = 3000, 2000, 80
+    L = (1.0, 1.0, 1.0)
+    x, _, y = pb.uniform_system(pb.config_seed(14), ns, nt, L)
+    rng = np.random.default_rng(15)
+    F = rng.standard_normal((R, ns, 3)).astype(np.float32)
+    p = pb.Params(L=L, alpha=10.0, eps=1e-3, grid=(32, 32, 32), P=12, tol=1e-7, precision="mixed").tune(ns, nt)
+    ctx = pb.Context(0)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+    X, Y = dev(x), dev(y)
+    u = ctx.velocity_multirhs(p, X, dev(F), Y).cpu().numpy()
+    assert u.shape == (R, nt, 3) and np.all(np.isfinite(u))
+    for r in (0, 31, 63, 64, 79):
+        single = ctx.velocity(p, X, dev(F[r].astype(np.float64)), Y).cpu().numpy()
+        assert relL2(u[r], single) < 1e-6, (r, relL2(u[r], single))
+    f79 = F[79].astype(np.float64)
+    want = po.velocity_ewald(y, x, f79, L, p.alpha, p.d, p.eps, p.rc, p.grid, (p.P,) * 3, p.m)
+    assert relL2(u[79], want) < 1e-6
+    ctx.close()
+
+
+@pytest.mark.parametrize("grid,P", [((4, 4, 4), 12), ((8, 8, 8), 20), ((16, 12, 16), 8), ((16, 16, 16), 0)])
+def test_invalid_grid_or_window_rejected_before_any_work(grid, P):
+    # checked before binning or any launch, on every velocity entry point: a
+    # window wider than the grid would make the gather read outside the grid
+    import torch
+    ctx = pb.Context(0)
+    n = 50
+    x = torch.rand((n, 3), dtype=torch.float64, device="cuda")
+    F = torch.rand((2, n, 3), dtype=torch.float32, device="cuda")
+    p = pb.Params(alpha=10.0, eps=1e-3, d=0.05, rc=0.2, grid=grid, P=P, precision="mixed")
+    u = torch.full((n, 3), 3.0, dtype=torch.float64, device="cuda")
+    U = torch.full((2, n, 3), 3.0, dtype=torch.float64, device="cuda")
+    calls = [lambda: ctx.velocity(p, x, x, x, u), lambda: ctx.velocity_kspace(p, x, x, x, u),
+             lambda: ctx.velocity_multirhs(p, x, F, x, U),
+             lambda: ctx.velocity_batched([p], x, x, [0, n], x, [0, n], u)]
+    for call in calls:
+        with pytest.raises(pb.PBError) as ei:
+            call()
+        assert ei.value.status in (pb.PB_ERR_INVALID, pb.PB_ERR_UNSUPPORTED)
+    torch.cuda.synchronize()
+    assert bool(torch.all(u == 3.0)) and bool(torch.all(U == 3.0))
+    ctx.close()
+
+
+def test_python_wrappers_reject_mismatched_shapes():
+    import torch
+    ctx = pb.Context(0)
+    p = pb.Params(alpha=10.0, eps=1e-3, d=0.05, rc=0.2, grid=(16, 16, 16), P=8, precision="mixed")
+    x = torch.rand((40, 3), dtype=torch.float64, device="cuda")
+    for args in [(x, x[:39], x), (x, x, x, torch.empty((39, 3), dtype=torch.float64, device="cuda"))]:
+        with pytest.raises(pb.PBError) as ei:
+            ctx.velocity(p, *args)
+        assert ei.value.status == pb.PB_ERR_INVALID
+    F = torch.rand((2, 39, 3), dtype=torch.float32, device="cuda")
+    with pytest.raises(pb.PBError):
+        ctx.velocity_multirhs(p, x, F, x)
+    with pytest.raises(pb.PBError):
+        ctx.velocity_batched([p], x, x, [0, 41], x, [0, 40])
+    ctx.close()

@@ -155,3 +155,36 @@ def test_shells_semantics():
     B = A.copy(order="F")
     po.fourierShells(B, absA, y, x, [3, 3, 2], [3, 3, 1], L, a, d, e)  # newM == oldM in 0,1: no-op
     assert np.array_equal(A, B)
+
+
+def test_spectral_kspace_matches_direct_anisotropic():
+    # box and grid both uneven: per-axis spacing h_i = L_i / n_i in the window
+    # offsets and wave numbers (fftBrinkman.h:101-103), the Gaussian width from
+    # axis 0 (:113-114); the direct k-sum over (2K_i + 1) modes per axis is the
+    # reference's fourierShells with 1/tau
+    L = (1.0, 1.3, 0.8)
+    x, f, y = rand_system(9, 40, 30, L)
+    y = np.vstack([y, x[:2]])
+    a, d = 3.0, 0.15
+    K = [int(np.ceil(17 / d * Li / (2 * np.pi))) for Li in L]
+    want = po.kspace_direct(y, x, f, L, a, d, K)
+    got = po.kspace_spectral(y, x, f, L, a, d, (48, 64, 40), (24, 24, 24), np.sqrt(np.pi * 24))
+    assert np.linalg.norm(got - want) < 1e-12 * np.linalg.norm(want)
+    # a transposed box and grid give the transposed field (axis bookkeeping)
+    Lt, perm = (0.8, 1.0, 1.3), [2, 0, 1]
+    got_t = po.kspace_spectral(y[:, perm], x[:, perm], f[:, perm], Lt, a, d, (40, 48, 64), (24, 24, 24),
+                               np.sqrt(np.pi * 24) * (0.8 / 40) / (1.0 / 48))
+    assert np.linalg.norm(got_t - want[:, perm]) < 1e-11 * np.linalg.norm(want)
+
+
+@pytest.mark.parametrize("shape", [(6, 10, 15), (7, 12, 9), (8, 8, 8), (5, 1, 3)])
+def test_oracle_fft3d_matches_numpy(shape):
+    # the oracle's own mixed-radix FFT (it replaces FFTW's fftw_plan_dft_3d,
+    # fftBrinkman.h:72-76) against numpy, forward and unnormalised inverse,
+    # on sizes with factors 2, 3, 5 and 7
+    r = np.random.default_rng(sum(shape))
+    a = r.standard_normal(shape) + 1j * r.standard_normal(shape)
+    fwd = po.fft3d(a.copy(), -1)
+    assert np.max(np.abs(fwd - np.fft.fftn(a))) < 1e-13 * np.max(np.abs(fwd))
+    inv = po.fft3d(a.copy(), +1)
+    assert np.max(np.abs(inv - np.fft.ifftn(a) * a.size)) < 1e-13 * np.max(np.abs(inv))
