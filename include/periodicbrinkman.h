@@ -157,6 +157,25 @@ const char *pb_status_string(int status);
 int pb_velocity(pb_context *ctx, const pb_params *p, int ns, const double *xn, const double *f,
                 int nt, const double *y, double *u, void *stream);
 
+/* pb_velocity without allocating: no reference counterpart.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, result and
+ * overwrite semantics as pb_velocity; it calls no hipMalloc, hipFree or other
+ * allocating routine.  Every device buffer beyond xn, f, y and u comes from
+ * one static device array of PB_NOALLOC_MAX_BYTES per device, held until the
+ * process ends and shared by every call on that device (the last call's
+ * contents stay in it): calls that overlap on one device -- on different
+ * streams, or from several host threads -- race; calls on one stream are
+ * ordered.  The pair kernel runs after the k space on `stream` (no internal
+ * stream).  Rejected with PB_ERR_UNSUPPORTED, before any work is enqueued, when
+ * the workload needs more scratch than the array holds (about 128 MiB: e.g.
+ * 1e5 sources and targets on a 128^3 grid fit in both precisions, a 256^3
+ * grid does not) or more than 2^18 cell and target keys.  The first call with
+ * new parameters uploads the kernel constants and waits for that copy. */
+#define PB_NOALLOC_MAX_BYTES (128u << 20)
+int pb_velocity_noalloc(pb_context *ctx, const pb_params *p, int ns, const double *xn, const double *f,
+                        int nt, const double *y, double *u, void *stream);
+
 /* The two halves separately (same conventions; used for diagnostics). */
 int pb_velocity_real(pb_context *ctx, const pb_params *p, int ns, const double *xn,
                      const double *f, int nt, const double *y, double *u, void *stream);

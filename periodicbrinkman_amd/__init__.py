@@ -106,6 +106,7 @@ def _declare(L):
     L.pb_velocity.argtypes = vel
     L.pb_velocity_real.argtypes = vel
     L.pb_velocity_kspace.argtypes = vel
+    L.pb_velocity_noalloc.argtypes = vel
     L.pb_velocity_batched.argtypes = [_VP, ii, ctypes.POINTER(_Params), ctypes.POINTER(ctypes.c_int), _VP, _VP,
                                       ctypes.POINTER(ctypes.c_int), _VP, _VP, _VP]
     L.pb_velocity_multirhs.argtypes = [_VP, ctypes.POINTER(_Params), ii, _VP, ii, _VP, ii, _VP, _VP, _VP]
@@ -268,6 +269,12 @@ class Context:
     def velocity(self, params, xn, f, y, u=None, stream=None):
         """u = real + k-space velocity at y from forces f at xn (device tensors (n, 3))."""
         return self._vel(lib().pb_velocity, params, xn, f, y, u, stream)
+
+    def velocity_noalloc(self, params, xn, f, y, u, stream=None):
+        """velocity() without allocating device memory (pb_velocity_noalloc):
+        u must be given; the library's fixed device scratch (shared by every
+        call on the device) holds everything else."""
+        return self._vel(lib().pb_velocity_noalloc, params, xn, f, y, u, stream)
 
     def velocity_real(self, params, xn, f, y, u=None, stream=None):
         return self._vel(lib().pb_velocity_real, params, xn, f, y, u, stream)

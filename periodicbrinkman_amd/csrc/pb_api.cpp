@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 #include <thread>
 
 // ---- context helpers --------------------------------------------------------
@@ -29,6 +30,15 @@ int pb_reserve(pb_context *c, pb_buffer *b, size_t bytes)
 {
     if (bytes == 0) bytes = 16;
     if (b->bytes >= bytes) return PB_OK;
+    if (c->arena) { // a slice of the device scratch; nothing is allocated
+        const size_t need = (bytes + 255) & ~(size_t)255;
+        if (need > c->arena_end - c->arena_used)
+            return pb_fail(c, PB_ERR_UNSUPPORTED, "the workload exceeds the no-allocation scratch");
+        b->ptr = c->arena + c->arena_used;
+        b->bytes = need;
+        c->arena_used += need;
+        return PB_OK;
+    }
     if (b->ptr) {
         hipError_t e = hipFree(b->ptr);
         b->ptr = nullptr;
@@ -447,6 +457,79 @@ int shells_impl(double *A, double *absA, int lda, const double *x, int nt, const
     return st;
 }
 
+// ---- the allocation-free entry point ----------------------------------------
+// One context per device works in arena mode on the device's scratch array
+// (pb_noalloc.hip): the constants at fixed offsets of its first
+// PB_NOALLOC_FIXED bytes (kept across calls, re-uploaded when the parameters
+// change), every other buffer carved anew by each call.  A mutex per device
+// serializes the calls' host-side set-up; the device work of calls on one
+// stream is ordered by that stream, calls on different streams race.
+constexpr int kNaDevices = 64;
+constexpr size_t kNaTable = 0, kNaKparams = 32u << 10, kNaTwid64 = 36u << 10, kNaTwid32 = 52u << 10;
+static_assert(PB_TAB_LDS_DOUBLES * (sizeof(double) + sizeof(float)) <= kNaKparams - kNaTable, "table slot");
+static_assert(sizeof(pb_kparams) <= kNaTwid64 - kNaKparams, "kernel-constant slot");
+static_assert(1020 * sizeof(double2) <= kNaTwid32 - kNaTwid64, "FP64 twiddle slot");
+static_assert(kNaTwid32 + 1020 * sizeof(float2) <= PB_NOALLOC_FIXED, "FP32 twiddle slot");
+pb_context *na_ctx[kNaDevices] = {};
+std::mutex na_mutex[kNaDevices];
+std::mutex na_setup_mutex;
+
+void noalloc_setup(int device, int cu_count)
+{
+    if (device < 0 || device >= kNaDevices) return;
+    std::lock_guard<std::mutex> lock(na_setup_mutex);
+    if (na_ctx[device] || pb_noalloc_scratch_init(device) != PB_OK) return;
+    char *base = (char *)pb_noalloc_scratch(device);
+    if (!base) return;
+    pb_context *na = new pb_context();
+    na->device = device;
+    na->cu_count = cu_count;
+    na->arena = base;
+    na->arena_end = PB_NOALLOC_BYTES;
+    na->arena_used = PB_NOALLOC_FIXED;
+    na->table = {base + kNaTable, kNaKparams - kNaTable};
+    na->kparams = {base + kNaKparams, kNaTwid64 - kNaKparams};
+    na->twid[0] = {base + kNaTwid64, kNaTwid32 - kNaTwid64};
+    na->twid[1] = {base + kNaTwid32, PB_NOALLOC_FIXED - kNaTwid32};
+    na_ctx[device] = na;
+}
+
+// Device bytes the arena hands out for one evaluation (the pb_reserve calls of
+// the fused binning, the k space and the pair kernel, 256-byte slices).
+size_t noalloc_bytes(const pb_context *na, const pb_cells &cells, const pb_kgrid &g, int precision, int ns,
+                     int nt)
+{
+    auto sl = [](size_t b) { return ((b ? b : 16) + 255) & ~(size_t)255; };
+    const size_t nsn = (size_t)(ns > 0 ? ns : 1), ntn = (size_t)(nt > 0 ? nt : 1);
+    const size_t ncell = (size_t)cells.ncell, nkt = (size_t)cells.nkey_t, nq = nkt / 16;
+    const size_t mt = std::min<size_t>(nq, (size_t)nt) + (size_t)nt / 64 + 1;
+    size_t b = 2 * sl(4 * (ncell + 1)) + 2 * sl(4 * nsn) + 2 * sl(32 * nsn) + sl(4 * nsn);  // sources
+    b += 2 * sl(4 * (nkt + 1)) + 2 * sl(4 * ntn) + sl(32 * ntn) + sl(4 * ntn);            // targets
+    b += sl(4 * (nq + 1)) + sl(8 * mt);                                                    // tiles
+    const size_t cb = precision == PB_PREC_MIXED ? 8 : 16;                                 // complex bytes
+    b += sl((size_t)g.ng * 2 * cb) + sl(3 * (size_t)g.n[0] * g.n[1] * (g.n[2] / 2 + 1) * cb); // grid, spectrum
+    if (nt > 0) { // pair-kernel statistics (pb_run_real's launch size)
+        const double tiles_est = std::max(1.0, nt / 48.0);
+        const int nsplit = (int)std::min(16.0, std::max(1.0, std::round(2.25 * 16 * na->cu_count / tiles_est)));
+        const size_t blocks = ((size_t)mt * nsplit + pb_pair_waves - 1) / pb_pair_waves;
+        b += sl(sizeof(uint4) * blocks * pb_pair_waves);
+    }
+    return b;
+}
+
+// Every buffer of a context (the constants' slots of the arena context are
+// kept, the others carved anew per call).
+template <typename F> void for_each_buffer(pb_context *c, F fn)
+{
+    pb_buffer *bufs[] = {&c->cell_count, &c->cell_start, &c->cell_of_s, &c->rank_s, &c->cell_of_t, &c->rank_t,
+                         &c->tcell_count, &c->tcell_start, &c->spos, &c->sfrc, &c->tpos, &c->tperm, &c->counters,
+                         &c->scan_tmp, &c->qcount, &c->qoff, &c->tiles, &c->grid, &c->spec, &c->sperm, &c->mrhs_f,
+                         &c->twid[0], &c->twid[1], &c->batch_kp, &c->batch_tab, &c->dense_tmp, &c->kparams,
+                         &c->table, &c->gb_int, &c->gb_d4, &c->gb_tiles, &c->gb_grid, &c->gb_spec, &c->gb_sys,
+                         &c->gb_map, &c->gb_stats, &c->mr_sys, &c->mr_map, &c->mr_frc, &c->mr_grid, &c->mr_spec};
+    for (pb_buffer *b : bufs) fn(c, b);
+}
+
 } // namespace
 
 extern "C" {
@@ -475,6 +558,9 @@ int pb_create(pb_context **ctx, int device)
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
         c->cu_count = cus;
+    // set-up of the allocation-free entry point on this device (once per
+    // device; a failure only disables that entry point)
+    noalloc_setup(device, c->cu_count);
     *ctx = c;
     return PB_OK;
 }
@@ -496,15 +582,10 @@ int pb_destroy(pb_context *c)
     if (c->pair_event2) (void)hipEventDestroy(c->pair_event2);
     for (int k = 0; k < 2; k++)
         if (c->pair_event[k]) (void)hipEventDestroy(c->pair_event[k]);
-    pb_buffer *bufs[] = {&c->cell_count, &c->cell_start, &c->cell_of_s, &c->rank_s, &c->cell_of_t,
-                         &c->rank_t, &c->tcell_count, &c->tcell_start, &c->spos, &c->sfrc, &c->tpos,
-                         &c->tperm, &c->counters, &c->scan_tmp, &c->qcount, &c->qoff, &c->tiles, &c->grid, &c->spec,
-                         &c->sperm, &c->mrhs_f,
-                         &c->twid[0], &c->twid[1], &c->batch_kp, &c->batch_tab, &c->dense_tmp, &c->kparams, &c->table,
-                         &c->gb_int, &c->gb_d4, &c->gb_tiles, &c->gb_grid, &c->gb_spec, &c->gb_sys, &c->gb_map,
-                         &c->gb_stats, &c->mr_sys, &c->mr_map, &c->mr_frc, &c->mr_grid, &c->mr_spec};
-    for (pb_buffer *b : bufs)
-        if (b->ptr) (void)hipFree(b->ptr);
+    if (!c->arena) // (an arena context's buffers are slices of the device scratch)
+        for_each_buffer(c, [](pb_context *, pb_buffer *b) {
+            if (b->ptr) (void)hipFree(b->ptr);
+        });
     if (c->ev_ready)
         for (int k = 0; k < PB_TSLOTS; k++)
             for (int i = 0; i <= PB_NPHASE; i++) (void)hipEventDestroy(c->ev[k][i]);
@@ -564,6 +645,51 @@ int pb_velocity_kspace(pb_context *ctx, const pb_params *p, int ns, const double
     return velocity_impl(ctx, p, ns, xn, f, nt, y, u, stream, false, true);
 }
 
+int pb_velocity_noalloc(pb_context *c, const pb_params *p, int ns, const double *xn, const double *f, int nt,
+                        const double *y, double *u, void *stream)
+{
+    if (!c) return PB_ERR_INVALID;
+    int st;
+    if ((st = check_params(c, p))) return st;
+    if (ns < 0 || nt < 0) return pb_fail(c, PB_ERR_INVALID, "negative point count");
+    if ((ns > 0 && (!xn || !f)) || (nt > 0 && (!y || !u))) return pb_fail(c, PB_ERR_INVALID, "null point array");
+    pb_context *na = c->device >= 0 && c->device < kNaDevices ? na_ctx[c->device] : nullptr;
+    if (!na) return pb_fail(c, PB_ERR_HIP, "the device scratch of the no-allocation entry point is unavailable");
+    pb_cells cells;
+    pb_kgrid g;
+    if ((st = make_cells(c, p, &cells))) return st;
+    if ((st = make_kgrid(c, p, &g))) return st;
+    if (!pb_bin_fused_ok(cells))
+        return pb_fail(c, PB_ERR_UNSUPPORTED, "the no-allocation entry point needs cell keys + target keys <= 2^18 "
+                                              "(a larger cutoff rc)");
+    const size_t need = noalloc_bytes(na, cells, g, p->precision, ns, nt);
+    if (need > PB_NOALLOC_BYTES - PB_NOALLOC_FIXED) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "the workload needs %.1f MiB of scratch; the no-allocation entry point "
+                      "holds at most %.1f MiB", need / 1048576.0, (PB_NOALLOC_BYTES - PB_NOALLOC_FIXED) / 1048576.0);
+        return pb_fail(c, PB_ERR_UNSUPPORTED, msg);
+    }
+    if ((st = pb_hip_check(c, hipSetDevice(c->device), "hipSetDevice"))) return st;
+    std::lock_guard<std::mutex> lock(na_mutex[c->device]);
+    // this call's buffers are carved from the scratch anew (the constants'
+    // slots at its start are kept)
+    for_each_buffer(na, [](pb_context *a, pb_buffer *b) {
+        if (b != &a->table && b != &a->kparams && b != &a->twid[0] && b != &a->twid[1]) *b = pb_buffer{};
+    });
+    na->arena_used = PB_NOALLOC_FIXED;
+    na->err.clear();
+    hipStream_t s = (hipStream_t)stream;
+    const pb_kparams *kp = nullptr;
+    size_t ntab = 0;
+    // the binning's count arrays are cleared by a fill kernel, the grid is
+    // written point by point, u overwritten by the gather: nothing in the
+    // scratch needs to start at zero
+    st = device_kparams(na, s, p, std::fmax(3 * p->d, 8 * p->eps), &kp, &ntab);
+    if (!st)
+        st = enqueue_velocity(na, s, p, kp, ntab, na->table_deg, ns, xn, f, nt, y, u, true, true, false);
+    if (st) return pb_fail(c, st, na->err);
+    return PB_OK;
+}
 
 namespace {
 

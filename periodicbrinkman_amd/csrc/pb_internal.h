@@ -160,10 +160,26 @@ struct pb_buffer {
     size_t bytes = 0;
 };
 
+// The allocation-free entry point's device scratch (pb_noalloc.hip): one
+// array of PB_NOALLOC_BYTES per device.  Its first PB_NOALLOC_FIXED bytes hold
+// the constants (radial table, kernel constants, FFT twiddles) at fixed
+// offsets; each call carves its other buffers from the rest.
+#define PB_NOALLOC_BYTES PB_NOALLOC_MAX_BYTES
+#define PB_NOALLOC_FIXED (64u << 10)
+// Look up the array's device address for a device (with that device current;
+// once per device, at context creation) / the address found (null: none).
+int pb_noalloc_scratch_init(int device);
+void *pb_noalloc_scratch(int device);
+
 struct pb_context {
     int device = 0;
     int cu_count = 256; // compute units of the device
     std::string err;
+    // arena mode (the allocation-free context of a device): pb_reserve hands
+    // out slices of [arena, arena + arena_end) from arena_used on and never
+    // allocates or frees
+    char *arena = nullptr;
+    size_t arena_end = 0, arena_used = 0;
     // real space
     pb_buffer cell_count, cell_start, cell_of_s, rank_s, cell_of_t, rank_t, tcell_count,
         tcell_start;
