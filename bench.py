@@ -238,33 +238,114 @@ def multirhs_leg(ctx, steps):
         ctx.velocity(p, X, F64[k], X, V)
     torch.cuda.synchronize()
     ms_single = (time.perf_counter() - t0) * 1e3
+    pairs = sum(ctx.last_pair_stats()[:2])  # (pair-images inside rc: the same for every force set)
     pr = pb.Params(alpha=10.0, eps=1e-3, grid=(128, 128, 128), P=20, tol=1e-11, precision="fp64").tune(n, n)
     errs = [relL2(u[k], ctx.velocity(pr, X, F64[k], X).cpu().numpy()) for k in (0, R // 2, R - 1)]
     return {
         "workload": "2e5 uniform particles, 64 force sets (FP32, N(0,1)), targets = sources, alpha 10, "
                     "eps 1e-3, 128^3 grid, P 12, tuner tol %g" % TOL,
         "value": n * R / (ms / 1e3), "unit": "velocity evals/s (targets x force sets)", "ms_per_call": ms,
-        "calls": steps, "rel_l2_sets_0_32_63": errs, "meets_accuracy": bool(max(errs) <= ACCURACY),
+        "calls": steps, "pairs": pairs, "rel_l2_sets_0_32_63": errs, "meets_accuracy": bool(max(errs) <= ACCURACY),
         "single_set_calls_ms": ms_single, "speedup_vs_single_set_calls": ms_single / ms,
         "note": "real space as the MFMA contraction of FP32 kernel tiles with the 64 sets; errors of three "
                 "sets against FP64 (P 20, tuner tol 1e-11)",
     }
 
 
-def cpu_baseline(x, f, y, p, n_real_targets, threads=1):
-    """The reference algorithm (FP64 oracle) on the host: real space over all
-    source pairs x 27 images (realShells) on a target subsample, extrapolated
-    linearly in targets; k space by the spectral fftBrinkman algorithm in full."""
+def host_threads():
+    """Host threads this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS when that is set (the share of the host given to one GPU)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_reference_algorithm(x, f, y, p, n_real, threads):
+    """The reference algorithm (FP64 oracle) on the host with `threads`
+    threads: real space over all source pairs x 27 images (realShells,
+    periodicBrinkman.h:564-693) on the first n_real targets, extrapolated
+    linearly to all targets; k space by the spectral fftBrinkman algorithm
+    (fftBrinkman.h:34-287) in full.  Returns (evals/s, real s, k s, real s
+    per target)."""
     from oracle import pyoracle as po
     po.set_threads(threads)
-    sub = y[:n_real_targets]
     t0 = time.perf_counter()
-    po.real_velocity_cube(sub, x, f, p.L, p.alpha, p.d, p.eps, (1, 1, 1))
-    t_real = (time.perf_counter() - t0) / n_real_targets * y.shape[0]
+    po.real_velocity_cube(y[:n_real], x, f, p.L, p.alpha, p.d, p.eps, (1, 1, 1))
+    per_target = (time.perf_counter() - t0) / n_real
+    t_real = per_target * y.shape[0]
     t0 = time.perf_counter()
     po.kspace_spectral(y, x, f, p.L, p.alpha, p.d, p.grid, (p.P,) * 3, p.m)
     t_k = time.perf_counter() - t0
-    return y.shape[0] / (t_real + t_k), t_real, t_k
+    return y.shape[0] / (t_real + t_k), t_real, t_k, per_target
+
+
+def cpu_baseline(x, f, y, p, n1, nN):
+    """The reference algorithm on the host at 1 thread (the reference is
+    single-threaded) and at every thread this process may use: real space
+    sampled on n1 and nN targets, k space in full."""
+    threads = host_threads()
+    v1, r1, k1, pt1 = cpu_reference_algorithm(x, f, y, p, n1, 1)
+    log(f"cpu 1 thread: {v1:.3f} evals/s (real {r1:.1f} s extrapolated, k {k1:.1f} s)")
+    vN, rN, kN, ptN = cpu_reference_algorithm(x, f, y, p, nN, threads)
+    log(f"cpu {threads} threads: {vN:.3f} evals/s (real {rN:.1f} s extrapolated, k {kN:.1f} s)")
+    return {
+        "value": vN, "unit": "velocity evals/s", "cores": threads, "kind": "port",
+        "sample": f"FP64 oracle of the reference algorithm, {threads} threads (OpenMP over targets, spreading over "
+                  f"grid slabs): real space {nN} targets x {NS} sources x 27 images (realShells), extrapolated "
+                  f"to {NT} targets ({rN:.1f} s); k space in full, 128^3 spectral fftBrinkman ({kN:.1f} s)",
+        "single_thread": {
+            "value": v1, "cores": 1,
+            "sample": f"real space {n1} targets x {NS} sources x 27 images, extrapolated to {NT} targets "
+                      f"({r1:.1f} s); k space in full ({k1:.1f} s)"},
+        "real_space_s_per_target": {"1_thread_%d_targets" % n1: pt1,
+                                    "%d_threads_%d_targets_x_threads" % (threads, nN): ptN * threads},
+    }
+
+
+def config1_leg(ctx, steps, cpu):
+    """The 1000-source workload (targets = sources, alpha 10, eps 0.01, 32^3,
+    P 8, split for 1e-8): GPU mixed and FP64 forms; with cpu, the reference
+    algorithm in full on the host (27 images, all pairs; spectral k space) at
+    1 thread and at every host thread, and the errors against the oracle."""
+    import torch
+
+    import periodicbrinkman_amd as pb
+
+    n = 1000
+    x, f, _ = pb.uniform_system(pb.config_seed(0), n, 0)
+    X = torch.as_tensor(x, device="cuda:0")
+    F = torch.as_tensor(f, device="cuda:0")
+    U = torch.empty_like(X)
+    p = pb.Params(alpha=10.0, eps=0.01, grid=(32, 32, 32), P=8, tol=1e-8).tune(n, n)
+    out = {"workload": "1000 sources uniform in the unit box, targets = sources, alpha 10, eps 0.01, 32^3 grid, "
+                       "P 8, split for 1e-8 (d %.4f, rc %.4f)" % (p.d, p.rc)}
+    res = {}
+    for form, q in (("mixed", p.copy(precision="mixed")), ("fp64", p.copy(precision="fp64"))):
+        for _ in range(5):
+            ctx.velocity(q, X, F, X, U)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.velocity(q, X, F, X, U)
+        torch.cuda.synchronize()
+        out["gpu_%s_ms" % form] = (time.perf_counter() - t0) * 1e3 / steps
+        res[form] = U.cpu().numpy().copy()
+    if cpu:
+        from oracle import pyoracle as po
+        want = po.velocity_ewald(x, x, f, p.L, p.alpha, p.d, p.eps, p.rc, p.grid, (p.P,) * 3, p.m)
+        for form in res:
+            out["rel_l2_%s_vs_oracle" % form] = relL2(res[form], want)
+        threads = host_threads()
+        for th in (1, threads):
+            po.set_threads(th)
+            t0 = time.perf_counter()
+            po.real_velocity_cube(x, x, f, p.L, p.alpha, p.d, p.eps, (1, 1, 1))
+            po.kspace_spectral(x, x, f, p.L, p.alpha, p.d, p.grid, (p.P,) * 3, p.m)
+            out["cpu_%d_threads_ms" % th] = (time.perf_counter() - t0) * 1e3
+        out["gpu_mixed_vs_cpu_%d_threads" % threads] = out["cpu_%d_threads_ms" % threads] / out["gpu_mixed_ms"]
+    return out
 
 
 def oracle_subsample_check(x, f, y, pref, uref, nsub):
@@ -277,6 +358,88 @@ def oracle_subsample_check(x, f, y, pref, uref, nsub):
     return relL2(uref[idx], want)
 
 
+def profile_workload(name, steps, warmup):
+    """Run one workload's forms for `steps` calls after `warmup`, with no
+    reference evaluation and no CPU work, so a profiler run covers only the
+    measured kernels: "headline" (mixed, FP64-table and plain FP64 forms of
+    the 1e5 -> 1e5 system), "batched" (the 512 systems, mixed, graph replays),
+    "clustered" (mixed), "multirhs" (64 force sets)."""
+    import torch
+
+    import periodicbrinkman_amd as pb
+
+    ctx = pb.Context(0)
+    out = {"profile": name, "steps": steps, "warmup": warmup}
+    if name == "headline":
+        x, f, y = pb.uniform_system(pb.config_seed(1), NS, NT)
+        dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda:0")
+        xs, fs, ys = dev(x), dev(f), dev(y)
+        u = torch.empty((NT, 3), dtype=torch.float64, device="cuda:0")
+        p = pb.Params(alpha=ALPHA, eps=EPS, grid=GRID, P=P, tol=TOL).tune(NS, NT)
+        for form, q in (("mixed", p.copy(precision="mixed")), ("fp64", p.copy(precision="fp64")),
+                        ("plain_fp64", p.copy(precision="fp64", kernel="direct"))):
+            for _ in range(warmup):
+                ctx.velocity(q, xs, fs, ys, u)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                ctx.velocity(q, xs, fs, ys, u)
+            torch.cuda.synchronize()
+            out[form + "_ms"] = (time.perf_counter() - t0) * 1e3 / steps
+            out[form + "_pairs"] = sum(ctx.last_pair_stats()[:2])
+    elif name == "batched":
+        systems, x, f, off = pb.batched_systems(pb.config_seed(3))
+        X = torch.as_tensor(x, device="cuda:0")
+        F = torch.as_tensor(f, device="cuda:0")
+        U = torch.empty_like(X)
+        ps = [pb.Params(L=(s["L"],) * 3, alpha=s["alpha"], eps=s["eps"], precision="mixed", tol=BATCH_TOL)
+              for s in systems]
+        for _ in range(max(2, warmup)):
+            ctx.velocity_batched(ps, X, F, off, X, off, U)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.velocity_batched(ps, X, F, off, X, off, U)
+        torch.cuda.synchronize()
+        out["mixed_ms"] = (time.perf_counter() - t0) * 1e3 / steps
+    elif name == "clustered":
+        n = 1_000_000
+        x, f = pb.clustered_system(pb.config_seed(2), n, 64, 0.02)
+        X = torch.as_tensor(x, device="cuda:0")
+        F = torch.as_tensor(f, device="cuda:0")
+        U = torch.empty_like(X)
+        q = pb.Params(alpha=25.0, eps=5e-4, grid=(256, 256, 256), P=16, tol=CLUSTER_TOL,
+                      precision="mixed").tune(n, n)
+        for _ in range(warmup):
+            ctx.velocity(q, X, F, X, U)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.velocity(q, X, F, X, U)
+        torch.cuda.synchronize()
+        out["mixed_ms"] = (time.perf_counter() - t0) * 1e3 / steps
+        out["mixed_pairs"] = sum(ctx.last_pair_stats()[:2])
+    else:
+        n, R = 200_000, 64
+        x, _, _ = pb.uniform_system(pb.config_seed(4), n, 0)
+        Fh = pb.Rng(pb.config_seed(4) + (1 << 32)).normal(3 * n * R).reshape(R, n, 3).astype(np.float32)
+        X = torch.as_tensor(x, device="cuda:0")
+        F = torch.as_tensor(Fh, device="cuda:0")
+        q = pb.Params(alpha=10.0, eps=1e-3, grid=(128, 128, 128), P=12, tol=TOL, precision="mixed").tune(n, n)
+        U = ctx.velocity_multirhs(q, X, F, X)
+        for _ in range(warmup):
+            ctx.velocity_multirhs(q, X, F, X, U)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.velocity_multirhs(q, X, F, X, U)
+        torch.cuda.synchronize()
+        out["mixed_ms"] = (time.perf_counter() - t0) * 1e3 / steps
+        out["pairs"] = sum(ctx.last_pair_stats()[:2])
+    ctx.close()
+    print(json.dumps(out))
+
+
 def copy_calibration():
     """A plain device-to-device copy of COPY_BYTES, three times: the HBM
     counters (FETCH_SIZE, WRITE_SIZE) of its kernel scale the pair kernel's."""
@@ -287,6 +450,113 @@ def copy_calibration():
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes": COPY_BYTES, "copies": 3}))
+
+
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "kernel_counters.json")
+PEAK_HBM = 8000.0  # GB/s, MI355X HBM3E
+
+
+def kernel_counters(workload):
+    """Per-dispatch counter averages of one workload's kernels from the
+    committed rocprofv3 summary (bench.py --profile <workload> under
+    rocprofv3), or {}."""
+    try:
+        with open(COUNTERS_FILE) as fh:
+            return json.load(fh)["workloads"].get(workload, {})
+    except (OSError, ValueError, KeyError):
+        return {}
+
+
+def _entry(name, kernels, ms, counters, bound, algorithmic=None, note=None):
+    """One per-kernel roofline entry: the counted work per launch (sum over
+    `kernels`) over the launch duration `ms`, against the MI355X peaks."""
+    c = [counters[k] for k in kernels if k in counters]
+    if not c or not ms:
+        return None
+    t = ms / 1e3
+    f64 = sum(e["valu_fp64_flop"] for e in c)
+    f32 = sum(e["valu_fp32_flop"] for e in c)
+    mf = sum(e["mfma_fp32_flop"] for e in c)
+    hbm = sum(e["hbm_bytes"] for e in c)
+    e = {"kernel": name, "kernels": kernels, "ms": ms, "bound": bound,
+         "counted": {"valu_fp64_tflops": f64 / t / 1e12, "valu_fp32_tflops": f32 / t / 1e12,
+                     "mfma_fp32_tflops": mf / t / 1e12, "hbm_gbps": hbm / t / 1e9, "hbm_bytes": hbm}}
+    if bound == "valu-fp64":
+        e["achieved"], e["peak"], e["unit"] = f64 / t / 1e12, PEAK_FP64, "TFLOP/s"
+    elif bound == "valu-fp32":
+        e["achieved"], e["peak"], e["unit"] = f32 / t / 1e12, PEAK_FP32, "TFLOP/s"
+    elif bound == "mfma":
+        e["achieved"], e["peak"], e["unit"] = mf / t / 1e12, PEAK_FP32, "TFLOP/s"
+    else:
+        e["achieved"], e["peak"], e["unit"] = hbm / t / 1e9, PEAK_HBM, "GB/s"
+    e["frac"] = e["achieved"] / e["peak"]
+    busy = [x["mfma_busy_frac"] for x in c if x.get("mfma_busy_frac")]
+    if bound == "mfma" and busy:
+        e["mfma_busy_frac"] = busy[0]
+    if algorithmic:
+        work, unit, peak = algorithmic
+        e["algorithmic"] = {"per_launch": work, "rate": work / t / (1e12 if unit == "TFLOP/s" else 1e9),
+                            "unit": unit, "frac": work / t / (1e12 if unit == "TFLOP/s" else 1e9) / peak}
+    if note:
+        e["note"] = note
+    return e
+
+
+def headline_kernel_rooflines(ph, pairs, prec="float", pair_kernel="k_pair_table<false, 6>"):
+    """Counter-based roofline of every hot kernel of the headline step: work
+    counted by rocprofv3 (committed summary) over the kernel's live duration
+    in this run (HIP events: the pair kernel beside the k space and alone, the
+    other phases one after another)."""
+    cnt = kernel_counters("headline")
+    if not cnt:
+        return []
+    n3 = GRID[0] * GRID[1] * GRID[2]
+    ngc = GRID[0] * GRID[1] * (GRID[2] // 2 + 1)
+    spread_flop = 7.0 * P ** 3 * NS
+    gather_flop = 7.0 * P ** 3 * NT
+    fft_bytes = 2 * 3 * ngc * 8  # one transform of the three fields, read + write (FP32 complex, half spectrum)
+    out = [
+        _entry("pair kernel (beside the k space)", [pair_kernel], ph["pair"], cnt, "valu-fp64",
+               (FLOP_PER_PAIR * pairs, "TFLOP/s", PEAK_FP64)),
+        _entry("pair kernel (alone)", [pair_kernel], ph["pair_alone"], cnt, "valu-fp64",
+               (FLOP_PER_PAIR * pairs, "TFLOP/s", PEAK_FP64)),
+        _entry("spread", ["k_spread_regions<%s>" % prec], ph["spread"], cnt, "valu-fp32",
+               (spread_flop, "TFLOP/s", PEAK_FP32),
+               "algorithmic: 7 P^3 FLOP per source; its bytes (24 B per source + 12 B per grid point) are in "
+               "counted.hbm_gbps's terms %.1f MB" % ((24 * NS + 12 * n3) / 1e6)),
+        _entry("FFT forward (z r2c + y)", ["k_fft_z_r2c<%s, 128>" % prec, "k_fft_y<%s, 128>" % prec],
+               ph["fft_forward"], cnt, "hbm"),
+        _entry("FFT x pass with the Brinkman multiply (forward + inverse)", ["k_fft_x_mult<%s, 128>" % prec],
+               ph["multiply"], cnt, "hbm"),
+        _entry("FFT inverse (y + z c2r)", ["k_fft_y<%s, 128>" % prec, "k_fft_z_c2r<%s, 128>" % prec],
+               ph["fft_inverse"], cnt, "hbm"),
+        _entry("FFT and multiply, all five passes (forward and inverse transforms)",
+               ["k_fft_z_r2c<%s, 128>" % prec, "k_fft_y<%s, 128>" % prec, "k_fft_x_mult<%s, 128>" % prec,
+                "k_fft_y<%s, 128>" % prec, "k_fft_z_c2r<%s, 128>" % prec],
+               ph["fft_forward"] + ph["multiply"] + ph["fft_inverse"], cnt, "hbm",
+               (2 * fft_bytes, "GB/s", PEAK_HBM),
+               "algorithmic: 2 transforms x 2 (read + write) x 3 fields x n^2 (n/2 + 1) x 8 B"),
+        _entry("gather", ["k_interp<%s, 12>" % prec], ph["gather"], cnt, "valu-fp32",
+               (gather_flop, "TFLOP/s", PEAK_FP32),
+               "bound in practice by the L1 return of 12-byte window points (P^3 per target)"),
+        _entry("binning", ["k_fill_int2", "k_bin_count2", "k_bin_scan2", "k_bin_scatter2"], ph["binning"], cnt,
+               "hbm"),
+    ]
+    return [e for e in out if e]
+
+
+def leg_kernel_rooflines(workload, spec):
+    """Per-kernel roofline of a leg from the committed counters and kernel
+    trace (durations: the trace average of the profiled run).  spec: (label,
+    kernels, bound, algorithmic or None) per entry."""
+    cnt = kernel_counters(workload)
+    out = []
+    for label, kernels, bound, alg in spec:
+        ms = sum(cnt[k]["trace_avg_us"] for k in kernels if k in cnt) / 1e3
+        e = _entry(label, kernels, ms, cnt, bound, alg, "duration: rocprofv3 trace average")
+        if e:
+            out.append(e)
+    return out
 
 
 def pair_traffic(workload):
@@ -310,15 +580,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle check")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-systems leg")
     ap.add_argument("--no-more", action="store_true", help="skip the clustered and multi-RHS legs")
-    ap.add_argument("--cpu-targets", type=int, default=8,
-                    help="targets in the CPU real-space sample (all 1e5 sources x 27 images each)")
+    ap.add_argument("--cpu-targets", type=int, default=32,
+                    help="targets in the single-thread CPU real-space sample (all 1e5 sources x 27 images each)")
+    ap.add_argument("--cpu-targets-threads", type=int, default=1024,
+                    help="targets in the CPU real-space sample at every host thread")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of known size (for HBM counter calibration)")
     ap.add_argument("--only", choices=("batched", "clustered", "multirhs"),
-                    help="run one extra leg alone and print its JSON (for profiling it)")
+                    help="run one extra leg alone and print its JSON")
+    ap.add_argument("--profile", choices=("headline", "batched", "clustered", "multirhs"),
+                    help="only run one workload's forms for --steps calls (no references, no CPU work: "
+                         "for rocprofv3)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.profile:
+        profile_workload(args.profile, args.steps, args.warmup)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
@@ -426,12 +704,13 @@ def main():
         "speedup_mixed_vs_plain_fp64": ms_plain / ms_mixed,
         "phases_ms": ph_mixed,
         "pairs": {"fp64": n64, "fp32": n32},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu-fp64", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": pair_traffic(workload),
                      "kernel": "k_pair_table (real-space pair kernel)",
-                     "note": "compute roof of the FP64 pair kernel, which runs on the vector ALUs "
-                             "(MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s); "
-                             "%d FLOP per pair-image inside rc, %d pair-images per launch, "
+                     "per_kernel": headline_kernel_rooflines(ph_mixed, pairs),
+                     "note": "the FP64 vector-ALU roof (78.6 TFLOP/s on MI355X) of the pair kernel, which "
+                             "computes on the vector ALUs (no matrix cores); achieved = the frozen "
+                             "%d FLOP per pair-image inside rc x %d pair-images per launch / "
                              "%.4f ms per launch (HIP events on the launch stream around each "
                              "launch, in as many steps as the timed ones, where it runs "
                              "concurrently with the k-space pipeline on another stream; alone "
@@ -441,20 +720,36 @@ def main():
     }
     if not args.no_batched:
         result["batched_form"] = batched_leg(ctx, args.steps, args.warmup)
+        result["batched_form"]["per_kernel_roofline"] = leg_kernel_rooflines("batched", [
+            ("pair kernel, degree-6 class", ["k_gpair_table<false, 6>"], "valu-fp64", None),
+            ("pair kernel, degree-8 class", ["k_gpair_table<false, 8>"], "valu-fp64", None),
+            ("spread", ["k_gspread<float>"], "valu-fp32", None),
+            ("gather", ["k_ginterp<float, 10>"], "valu-fp32", None),
+            ("FFT passes, 64^3 systems", ["k_gfft<float, 64, %d>" % q for q in range(5)], "hbm", None),
+            ("FFT passes, 32^3 systems", ["k_gfft<float, 32, %d>" % q for q in range(5)], "hbm", None)])
     if not args.no_more:
         result["clustered_form"] = clustered_leg(ctx, 3)
+        result["clustered_form"]["per_kernel_roofline"] = leg_kernel_rooflines("clustered", [
+            ("pair kernel", ["k_pair_table<false, 6>"], "valu-fp64", None),
+            ("spread", ["k_spread_regions<float>"], "valu-fp32", None),
+            ("gather", ["k_interp<float, 16>"], "valu-fp32", None)])
         result["multirhs_form"] = multirhs_leg(ctx, 3)
+        mr = result["multirhs_form"]
+        result["multirhs_form"]["per_kernel_roofline"] = leg_kernel_rooflines("multirhs", [
+            ("contraction (MFMA)", ["k_pair_mrhs<4, 2, 6>"], "mfma",
+             (18.0 * 64 * mr.get("pairs", 0), "TFLOP/s", PEAK_FP32)),
+            ("spread, 8 sets per workgroup", ["k_gspread_sets<8>"], "valu-fp32", None),
+            ("gather", ["k_ginterp<float, 12>"], "valu-fp32", None),
+            ("FFT x pass with the multiply", ["k_gfft<float, 128, 2>"], "hbm", None)])
     if not args.no_cpu:
         log("cpu baseline")
-        cps, t_real, t_k = cpu_baseline(x, f, y, p64, args.cpu_targets)
+        result["cpu_baseline"] = cpu_baseline(x, f, y, p64, args.cpu_targets, args.cpu_targets_threads)
         log("oracle subsample check")
-        result["cpu_baseline"] = {
-            "value": cps, "unit": "velocity evals/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 oracle of the reference algorithm, 1 thread: real space "
-                      f"{args.cpu_targets} targets x {NS} sources x 27 images (realShells), "
-                      f"extrapolated to {NT} targets ({t_real:.1f} s); k space full 128^3 "
-                      f"spectral fftBrinkman ({t_k:.1f} s)"}
         result["reference_vs_oracle_subsample"] = oracle_subsample_check(x, f, y, pref, uref, 256)
+    result["config1_form"] = config1_leg(ctx, 100, not args.no_cpu)
+    if "cpu_baseline" in result:
+        result["cpu_baseline"]["config1_full"] = {
+            k: v for k, v in result["config1_form"].items() if k.startswith("cpu_") or k == "workload"}
     ctx.close()
     print(json.dumps(result))
 
