@@ -523,7 +523,7 @@ template <typename F> void for_each_buffer(pb_context *c, F fn)
 {
     pb_buffer *bufs[] = {&c->cell_count, &c->cell_start, &c->cell_of_s, &c->rank_s, &c->cell_of_t, &c->rank_t,
                          &c->tcell_count, &c->tcell_start, &c->spos, &c->sfrc, &c->tpos, &c->tperm, &c->counters,
-                         &c->scan_tmp, &c->qcount, &c->qoff, &c->tiles, &c->grid, &c->spec, &c->sperm, &c->mrhs_f,
+                         &c->scan_tmp, &c->qcount, &c->qoff, &c->tiles, &c->grid, &c->spec, &c->sp_int, &c->sp_items, &c->sp_part, &c->sperm, &c->mrhs_f, &c->mrhs_u,
                          &c->twid[0], &c->twid[1], &c->batch_kp, &c->batch_tab, &c->dense_tmp, &c->kparams,
                          &c->table, &c->gb_int, &c->gb_d4, &c->gb_tiles, &c->gb_grid, &c->gb_spec, &c->gb_sys,
                          &c->gb_map, &c->gb_stats, &c->mr_sys, &c->mr_map, &c->mr_frc, &c->mr_grid, &c->mr_spec};
@@ -829,11 +829,33 @@ int pb_velocity_multirhs(pb_context *c, const pb_params *p, int ns, const double
     if (nt > 0 && !ntab)
         return pb_fail(c, PB_ERR_UNSUPPORTED, "the multi-RHS path needs the radial table (cutoff rc too small)");
     const int RP = (R + 31) / 32 * 32;
+    const long long nu = (long long)R * 3 * nt;
+    if ((st = pb_reserve(c, &c->mrhs_u, sizeof(double) * (size_t)std::max(nu, 1LL)))) return st;
     if ((st = pb_bin_all(c, s, cells, box, ns, xn, nullptr, nt, y))) return st;
     if ((st = pb_sort_forces(c, s, ns, R, RP, f))) return st;
     if (nt == 0) return PB_OK;
-    if ((st = mrhs_kspace(c, s, p, cells, box, g, ns, nt, R, RP, u))) return st;
-    return pb_run_mrhs_real(c, s, cells, kp, ntab, c->table_deg, box, ns, nt, R, RP, u);
+    // The real-space contraction (matrix cores and FP64 radial functions) runs
+    // on the pair stream beside the k space (memory- and VALU-bound spreading,
+    // FFT and gather) on s, into a buffer of its own that it overwrites; the k
+    // space overwrites u; after the join u += the real-space part.
+    if (!c->pair_stream &&
+        (st = pb_hip_check(c, hipStreamCreateWithFlags(&c->pair_stream, hipStreamNonBlocking), "stream")))
+        return st;
+    for (int k = 0; k < 2; k++)
+        if (!c->pair_event[k] &&
+            (st = pb_hip_check(c, hipEventCreateWithFlags(&c->pair_event[k], hipEventDisableTiming), "event")))
+            return st;
+    if ((st = pb_hip_check(c, hipEventRecord(c->pair_event[0], s), "fork"))) return st;
+    if ((st = pb_hip_check(c, hipStreamWaitEvent(c->pair_stream, c->pair_event[0], 0), "fork"))) return st;
+    // from here every return joins the pair stream back into s
+    st = pb_run_mrhs_real(c, c->pair_stream, cells, kp, ntab, c->table_deg, box, ns, nt, R, RP,
+                          (double *)c->mrhs_u.ptr);
+    const int jst = pb_hip_check(c, hipEventRecord(c->pair_event[1], c->pair_stream), "join");
+    if (!st && !jst) st = mrhs_kspace(c, s, p, cells, box, g, ns, nt, R, RP, u);
+    const int wst = jst ? jst : pb_hip_check(c, hipStreamWaitEvent(s, c->pair_event[1], 0), "join");
+    if (st || wst) return st ? st : wst;
+    pb_add(s, nu, u, (const double *)c->mrhs_u.ptr);
+    return pb_hip_check(c, hipGetLastError(), "multi-RHS add");
 }
 
 namespace {

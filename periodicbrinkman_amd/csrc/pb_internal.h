@@ -186,10 +186,14 @@ struct pb_context {
     pb_buffer spos, sfrc, tpos, tperm, counters, scan_tmp;
     pb_buffer sperm;  // sorted source -> caller's index
     pb_buffer mrhs_f; // forces of the sorted sources for many right-hand sides
+    pb_buffer mrhs_u; // the real-space part of their velocities (R x 3 nt doubles)
     pb_buffer qcount, qoff, tiles; // target tiles: int2 (first, count); count at qoff[nkey_t/16]
     int max_tiles = 0;             // bound on the tile count (launch size)
     // k space
     pb_buffer grid, spec; // real grid (T4 per point), half spectrum
+    // balanced spreading: candidate counts, split-region list, slot bases and
+    // totals (ints); the work items (int2); the partial sums of split regions
+    pb_buffer sp_int, sp_items, sp_part;
     pb_buffer twid[2];              // FFT twiddles for n = 4 .. 512: [0] FP64, [1] FP32
     int twid_ok[2] = {0, 0};
     // batched systems: kernel constants and radial tables of every system,
@@ -278,6 +282,9 @@ int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_
 int pb_sort_forces(pb_context *c, hipStream_t s, int ns, int R, int RP, const float *f);
 // frc[q * ns + j] = force set r0 + q of sorted source j, q < nr
 int pb_force_sets(pb_context *c, hipStream_t s, int ns, int r0, int nr, int RP, double4 *frc);
+// The real-space contraction OVERWRITES u (set r at r * 3 nt) for every
+// target and set r < R; pb_add adds n doubles of ur to u on stream s.
+void pb_add(hipStream_t s, long long n, double *u, const double *ur);
 int pb_run_mrhs_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp,
                      size_t table_doubles, int table_deg, const pb_box &box, int ns, int nt, int R, int RP,
                      double *u);

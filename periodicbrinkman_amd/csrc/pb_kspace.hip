@@ -11,6 +11,7 @@
 // wrapped positions are always computed in FP64 with the oracle's formula.
 #include "pb_internal.h"
 
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -47,65 +48,139 @@ __device__ __forceinline__ int wrapi(int i, int n)
 }
 
 // ---- spreading ----------------------------------------------------------------
-// One workgroup owns a region of 8 x 8 x 8 grid points (four wavefronts, each
-// two 4 x 4 x 4 sub-blocks, one point per lane) and gathers every source whose
-// centred window (first index f0 = floor(x/h - P/2) + 1, the oracle's
-// formula) touches the region.  Candidates come from the real-space cell list:
-// the cells covering [(gA - P/2) h, (gB + P/2) h) per axis, unwrapped so that a
-// source near the opposite face enters with its periodic image.  The
-// workgroup sets up the candidate runs (one per thread), scans their lengths,
-// tests the candidates 256 at a time and stages the hits in LDS in a fixed
-// order.  Each wavefront then takes the staged sources that touch its
-// sub-block in batches.  FP64, 32 at a time: one lane per source computes the
-// sub-block's 4+4+4 window weights and stores P[j][k] = wy[j] wz[k] and
-// wx[i] F in LDS; every lane then accumulates its point, P[j][k] * (wx[i] F),
-// three FMAs per source.  FP32, 16 at a time: the lanes of row i (16 lanes)
-// evaluate the weights at offset i of the 16 sources, one source per lane, and
-// keep wx[i] F in registers; each lane of the row then takes it by a DPP row
-// broadcast folded into its multiply-adds, so only P[j][k] comes from LDS (a
-// b96 broadcast read of wx F per source and lane would make the loop LDS
-// bound).  Every grid point is written exactly once: no atomics, no clearing,
-// and a fixed summation order.
+// The grid is cut into regions of 8 x 8 x 8 points.  A workgroup (four
+// wavefronts, each two 4 x 4 x 4 sub-blocks, one point per lane) spreads onto
+// one region the sources whose centred window (first index
+// f0 = floor(x/h - P/2) + 1, the oracle's formula) touches it.  Candidates
+// come from the real-space cell list: the cells covering
+// [(gA - P/2) h, (gB + P/2) h) per axis, unwrapped so that a source near the
+// opposite face enters with its periodic image; these cell runs, in a fixed
+// order, number the region's candidates 0 .. cand - 1.  The workgroup sets up
+// the runs (one per thread), scans their lengths, tests candidates 256 at a
+// time and stages the hits in LDS in a fixed order; each wavefront then takes
+// the staged sources that touch its sub-block in batches.
+//
+// FP32, 16 sources per batch: at staging, the thread that found a source
+// evaluates its window weights at the region's 8 points per axis (24 values,
+// zero outside the window) and records which 4-point halves of the region
+// the window touches; in a batch, lane (i, h) reads source h's weight at
+// offset i of the sub-block on each axis, keeps wx[i] F in registers and forms
+// P[h][4 i + k] = wy[i] wz[k] in LDS; the lane at point (i, j, k) then adds,
+// per source, P[h][4 j + k] times wx F of lane h of its own row, taken by a
+// DPP row broadcast folded into the multiply-add (three per source).  FP64, 32
+// per batch: one lane per source evaluates the sub-block's 4+4+4 weights and
+// stores P[j][k] = wy[j] wz[k] and wx[i] F in LDS.
+//
+// Work: a region whose candidates are few is one work item.  Dense regions
+// (clustered sources) are split into chunks of C candidates, each a work item,
+// so that no workgroup serializes a cluster: a count pass numbers each
+// region's candidates and gives each split region consecutive partial slots,
+// the spreading workgroups take items (the chunks first, then the whole
+// regions) from an atomic counter, and a reduction pass adds a split region's
+// chunk partials in chunk order.  Every grid point is written exactly once and
+// every sum has a fixed order: the result does not depend on scheduling.
 constexpr int kSpreadWaves = 4;
 
 template <typename T> struct spread_cfg;
-template <> struct spread_cfg<float> { static constexpr int CAP = 448, HB = 16; };
+template <> struct spread_cfg<float> { static constexpr int CAP = 256, HB = 16; };
 template <> struct spread_cfg<double> { static constexpr int CAP = 256, HB = 32; };
 
-// Per-batch weight arrays.  FP64: one lane per source writes P[h][4 j + k] =
-// wy[j] wz[k] and wx[i] F.  FP32: wx F stays in the registers of the lane that
-// computed it (broadcast along its row of 16 lanes with DPP), wz goes through
-// Wz, and P[h][4 j + k] is formed by the lanes.
+// Per-batch arrays.  FP64: one lane per source writes P[h][4 j + k] = wy[j]
+// wz[k] and wx[i] F.  FP32: P[h][4 i + k] = wy[i] wz[k] (wx F stays in the
+// registers of the lane that formed it).
 template <typename T> struct spread_batch_lds {
-    T P[kSpreadWaves][spread_cfg<T>::HB][17];                      // wy[j] wz[k] at 4 j + k (odd stride)
+    T P[kSpreadWaves][spread_cfg<T>::HB][17];                        // wy[j] wz[k] at 4 j + k (odd stride)
     typename cplx<T>::type4 wxF[kSpreadWaves][spread_cfg<T>::HB][4]; // wx[i] (fx, fy, fz)
 };
 template <> struct alignas(16) spread_batch_lds<float> {
-    alignas(16) float Wz[kSpreadWaves][16][4]; // wz[k] of source h
+    alignas(16) float Wz[kSpreadWaves][16][4]; // wz[k] of source h (weights evaluated per batch)
     alignas(16) float P[kSpreadWaves][16][16]; // wy[j] wz[k] of source h at 4 j + k
 };
 
-// NS > 1 (FP32 only): NS force sets spread together from one staging of the
-// sources (the many-right-hand-side k space), with a smaller staging capacity
-// (at least 256: one chunk of candidates is staged whole after a flush).
-template <typename T, int NS = 1> struct spread_lds : spread_batch_lds<T> {
-    static constexpr int CAP = NS == 1 ? spread_cfg<T>::CAP : 256, HB = spread_cfg<T>::HB;
-    static_assert(CAP >= 256, "a chunk of 256 candidates must fit the staging");
-    int run_j[256];       // first source of each candidate run of the chunk
-    int run_pre[257];     // exclusive prefix of the run lengths
-    unsigned run_code[256]; // image shifts (l0 + 4) | (l1 + 4) << 4 | (l2 + 4) << 8
-    int wcount[kSpreadWaves];
-    double sx[3][CAP];    // staged sources: wrapped position, shifted window start,
-    int sf0[3][CAP];      // image code and scaled force
+// Staged sources.  FP32 with one force set: one record per source of WS
+// floats: the weights at the region's 8 points on each axis (24), the scaled
+// force, and the mask of the 4-point halves its window touches (bit 2 a + h);
+// the weights are evaluated once per source.  Otherwise (FP64; FP32 with
+// several force sets, whose records would not fit at this occupancy) the
+// wrapped position, shifted window start, image code and scaled forces, and
+// the batches evaluate the weights.
+template <typename T, int NS> struct spread_stage {
+    static constexpr bool PREW = false;
+    static constexpr int CAP = spread_cfg<T>::CAP;
+    double sx[3][CAP];
+    int sf0[3][CAP];
     unsigned scode[CAP];
     T sF[3 * NS][CAP];
-    int ring[kSpreadWaves][128];    // staged indices touching the wavefront's sub-block
-                                    // (< HB pending + 64 appended)
+};
+template <> struct spread_stage<float, 1> {
+    static constexpr bool PREW = true;
+    static constexpr int CAP = spread_cfg<float>::CAP;
+    static constexpr int WS = 28, MASK = 27;
+    alignas(16) float st[CAP][WS];
 };
 
-// Weights of hit slot `lane` (staged index e) on the sub-block at sb.
-template <typename T>
-__device__ __forceinline__ void spread_weights(spread_lds<T> &S, int wave, int lane, int e, const int *sb,
+// NS > 1 (FP32 only): NS force sets spread together from one staging of the
+// sources (the many-right-hand-side k space).
+template <typename T, int NS = 1> struct spread_lds : spread_batch_lds<T>, spread_stage<T, NS> {
+    static constexpr int CAP = spread_stage<T, NS>::CAP, HB = spread_cfg<T>::HB;
+    static_assert(CAP >= 256, "a chunk of 256 candidates must fit the staging");
+    int run_j[256];         // first source of each candidate run of the chunk
+    int run_pre[257];       // exclusive prefix of the run lengths
+    unsigned run_code[256]; // image shifts (l0 + 4) | (l1 + 4) << 4 | (l2 + 4) << 8
+    int wcount[kSpreadWaves];
+    int ring[kSpreadWaves][128]; // staged indices touching the wavefront's sub-block
+                                 // (< HB pending + 64 appended)
+};
+
+// Region rid: its first and last grid points per axis and its candidate runs
+// (cell columns (u0, u1) x z segments of one periodic image each).
+struct region_geo {
+    int gA[3], gB[3], ulo[3], uhi[3];
+    int ncy, lz0, nzseg;
+    long long nruns;
+};
+
+__device__ __forceinline__ bool region_of(int rid, const pb_cells &cells, const pb_kgrid &g, region_geo &R)
+{
+    const int nr0 = (g.n[0] + 7) / 8, nr1 = (g.n[1] + 7) / 8, nr2 = (g.n[2] + 7) / 8;
+    if (rid < 0 || rid >= nr0 * nr1 * nr2) return false;
+    const int rr[3] = {rid / (nr1 * nr2), (rid / nr2) % nr1, rid % nr2};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        R.gA[a] = 8 * rr[a];
+        R.gB[a] = min(R.gA[a] + 7, g.n[a] - 1);
+        const double lo = (R.gA[a] - 0.5 * g.P[a]) * g.h[a], hi = (R.gB[a] + 0.5 * g.P[a]) * g.h[a];
+        R.ulo[a] = (int)floor(lo * cells.inv_h[a] - 1e-9);
+        R.uhi[a] = (int)floor(hi * cells.inv_h[a] + 1e-9);
+    }
+    const int ncx = R.uhi[0] - R.ulo[0] + 1;
+    R.ncy = R.uhi[1] - R.ulo[1] + 1;
+    R.lz0 = floordiv(R.ulo[2], cells.nc[2]);
+    R.nzseg = floordiv(R.uhi[2], cells.nc[2]) - R.lz0 + 1;
+    R.nruns = (long long)ncx * R.ncy * R.nzseg;
+    return true;
+}
+
+// Run r of the region: sources j .. j + len - 1 of one cell column segment
+// and its image code.
+__device__ __forceinline__ void region_run(const region_geo &R, long long r, const pb_cells &cells,
+                                           const int *__restrict__ cstart, int &j, int &len, unsigned &code)
+{
+    const int nc0 = cells.nc[0], nc1 = cells.nc[1], nc2 = cells.nc[2];
+    const int seg = (int)(r % R.nzseg);
+    const long long col = r / R.nzseg;
+    const int u0 = R.ulo[0] + (int)(col / R.ncy), u1 = R.ulo[1] + (int)(col % R.ncy);
+    const int l0 = floordiv(u0, nc0), l1 = floordiv(u1, nc1), l2 = R.lz0 + seg;
+    const int zs = max(R.ulo[2], l2 * nc2) - l2 * nc2, ze = min(R.uhi[2], (l2 + 1) * nc2 - 1) - l2 * nc2;
+    const int colbase = ((u0 - l0 * nc0) * nc1 + (u1 - l1 * nc1)) * nc2;
+    j = cstart[colbase + zs];
+    len = cstart[colbase + ze + 1] - j;
+    code = (unsigned)(l0 + 4) | ((unsigned)(l1 + 4) << 4) | ((unsigned)(l2 + 4) << 8);
+}
+
+// FP64 weights of staged source e on the sub-block at sb (lane = batch slot).
+template <typename T, int NS>
+__device__ __forceinline__ void spread_weights(spread_lds<T, NS> &S, int wave, int lane, int e, const int *sb,
                                                const pb_kgrid &g)
 {
     T w[3][4];
@@ -120,10 +195,7 @@ __device__ __forceinline__ void spread_weights(spread_lds<T> &S, int wave, int l
             T v = (T)0;
             if (m >= 0 && m < g.P[a]) {
                 const double dx = (double)(gq - l * g.n[a]) * g.h[a] - x;
-                if (sizeof(T) == 8)
-                    v = (T)exp(-dx * dx / g.eta);
-                else
-                    v = (T)__expf(-(float)(dx * dx) * (float)(1.0 / g.eta));
+                v = (T)exp(-dx * dx / g.eta);
             }
             w[a][q] = v;
         }
@@ -162,14 +234,73 @@ __device__ __forceinline__ void spread_steps16(const float (&P)[16][16], int jk,
     if constexpr (H < 15) spread_steps16<H + 1>(P, jk, f0, f1, f2, a0, a1, a2);
 }
 
-// FP32 batch of up to 16 staged sources.  Lane (i, h) = (lane / 16, lane % 16)
-// evaluates source h's window weight at offset i of the sub-block on each axis
-// (zero outside the window or for h >= cnt); it keeps wx F, publishes wz and
-// forms P[h][4 i + k] = wy[i] wz[k].  The lane at point (i, j, k) then adds,
-// for each source, P[h][4 j + k] times wx F of lane h of its own row.
+// The 48 multiply-adds of a batch of 16 sources per force set, with the row
+// broadcast folded into each (v_fmac_f32 with a DPP source): the lane at point
+// (i, j, k) adds, for each source h, P[h][4 j + k] times wx F of lane h of its
+// own row.  The leading s_nop 4 covers the wait states a DPP read needs after
+// a VALU write of its source (two) or of the execution mask (five).  Build
+// with PB_SPREAD_DPP_MOV for the compiler's form (a DPP move before each
+// multiply-add).
+// p[h] = P[h][4 j + k] of this lane's point, read once per batch.
+__device__ __forceinline__ void spread_p16(const float (&P)[16][16], int lane, float (&p)[16])
+{
+#pragma unroll
+    for (int q = 0; q < 16; q++) p[q] = P[q][lane & 15];
+}
+
+__device__ __forceinline__ void spread_fma48(const float (&P)[16][16], const float (&p)[16], int lane, float f0,
+                                             float f1, float f2, float (&acc)[3])
+{
+#ifndef PB_SPREAD_DPP_MOV
+    (void)P;
+    (void)lane;
+#define PB_FD(H, A, F, Q) "v_fmac_f32_dpp %" #A ", %" #F ", %" #Q " row_newbcast:" #H " row_mask:0xf bank_mask:0xf\n"
+#define PB_F3(H, Q) PB_FD(H, 0, 3, Q) PB_FD(H, 1, 4, Q) PB_FD(H, 2, 5, Q)
+    asm volatile("s_nop 4\n" PB_F3(0, 6) PB_F3(1, 7) PB_F3(2, 8) PB_F3(3, 9) PB_F3(4, 10) PB_F3(5, 11) PB_F3(6, 12)
+                     PB_F3(7, 13) PB_F3(8, 14) PB_F3(9, 15) PB_F3(10, 16) PB_F3(11, 17) PB_F3(12, 18) PB_F3(13, 19)
+                         PB_F3(14, 20) PB_F3(15, 21)
+                 : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2])
+                 : "v"(f0), "v"(f1), "v"(f2), "v"(p[0]), "v"(p[1]), "v"(p[2]), "v"(p[3]), "v"(p[4]), "v"(p[5]),
+                   "v"(p[6]), "v"(p[7]), "v"(p[8]), "v"(p[9]), "v"(p[10]), "v"(p[11]), "v"(p[12]), "v"(p[13]),
+                   "v"(p[14]), "v"(p[15]));
+#undef PB_F3
+#undef PB_FD
+#else
+    (void)p;
+    spread_steps16<0>(P, lane & 15, f0, f1, f2, acc[0], acc[1], acc[2]);
+#endif
+}
+
+// FP32 batch of up to 16 staged records (one force set) on the sub-block at
+// offsets off[a] (0 or 4) of the region.  Lane (i, h) = (lane / 16, lane % 16)
+// reads record h's weights at offset i (wx zero for h >= cnt), keeps wx F and
+// forms P[h][4 i + k] = wy[i] wz[k].
+__device__ __forceinline__ void spread_batch16w(spread_lds<float, 1> &S, int wave, int lane, int head, int cnt,
+                                                const int *off, float (&acc)[1][3])
+{
+    const int i = lane >> 4, h = lane & 15;
+    __builtin_amdgcn_wave_barrier();
+    const int e = S.ring[wave][(head + (h < cnt ? h : 0)) & 127];
+    const float *rec = S.st[e];
+    const float wx = h < cnt ? rec[off[0] + i] : 0.0f;
+    const float wy = rec[8 + off[1] + i];
+    const float4 z = *reinterpret_cast<const float4 *>(rec + 16 + off[2]);
+    *reinterpret_cast<float4 *>(&S.P[wave][h][4 * i]) = make_float4(wy * z.x, wy * z.y, wy * z.z, wy * z.w);
+    __builtin_amdgcn_wave_barrier();
+    float p[16];
+    spread_p16(S.P[wave], lane, p);
+    spread_fma48(S.P[wave], p, lane, wx * rec[24], wx * rec[25], wx * rec[26], acc[0]);
+    __builtin_amdgcn_wave_barrier();
+}
+
+// FP32 batch of up to 16 staged sources, NS force sets, weights evaluated
+// here: lane (i, h) evaluates source h's weight at offset i of the sub-block
+// on each axis (zero outside the window or for h >= cnt), keeps wx[i] F,
+// publishes wz and forms P[h][4 i + k] = wy[i] wz[k]; the 48 multiply-adds
+// then run once per force set with the same weights.
 template <int NS>
-__device__ __forceinline__ void spread_batch16(spread_lds<float, NS> &S, int wave, int lane, int head, int cnt,
-                                               const int *sb, const pb_kgrid &g, float (&acc)[NS][3])
+__device__ __forceinline__ void spread_batch16r(spread_lds<float, NS> &S, int wave, int lane, int head, int cnt,
+                                                const int *sb, const pb_kgrid &g, float (&acc)[NS][3])
 {
     const int i = lane >> 4, h = lane & 15;
     __builtin_amdgcn_wave_barrier();
@@ -192,52 +323,29 @@ __device__ __forceinline__ void spread_batch16(spread_lds<float, NS> &S, int wav
     const float4 z = *reinterpret_cast<const float4 *>(&S.Wz[wave][h][0]);
     *reinterpret_cast<float4 *>(&S.P[wave][h][4 * i]) = make_float4(w[1] * z.x, w[1] * z.y, w[1] * z.z, w[1] * z.w);
     __builtin_amdgcn_wave_barrier();
-#ifndef PB_SPREAD_DPP_MOV
-    // the 48 multiply-adds with the row broadcast folded into each
-    // (v_fmac_f32 with a DPP source).  The leading s_nop 4 covers the wait
-    // states a DPP read needs after a VALU write of its source (two) or of
-    // the execution mask (five).  Build with PB_SPREAD_DPP_MOV for the
-    // compiler's form (a DPP move before each multiply-add).
     float p[16];
+    spread_p16(S.P[wave], lane, p);
 #pragma unroll
-    for (int q = 0; q < 16; q++) p[q] = S.P[wave][q][lane & 15];
-#define PB_FD(H, A, F, Q) "v_fmac_f32_dpp %" #A ", %" #F ", %" #Q " row_newbcast:" #H " row_mask:0xf bank_mask:0xf\n"
-#define PB_F3(H, Q) PB_FD(H, 0, 3, Q) PB_FD(H, 1, 4, Q) PB_FD(H, 2, 5, Q)
-#pragma unroll
-    for (int s = 0; s < NS; s++) { // one block per force set, the weights shared
-        const float f0 = w[0] * S.sF[3 * s][e], f1 = w[0] * S.sF[3 * s + 1][e], f2 = w[0] * S.sF[3 * s + 2][e];
-        asm volatile("s_nop 4\n" PB_F3(0, 6) PB_F3(1, 7) PB_F3(2, 8) PB_F3(3, 9) PB_F3(4, 10) PB_F3(5, 11)
-                         PB_F3(6, 12) PB_F3(7, 13) PB_F3(8, 14) PB_F3(9, 15) PB_F3(10, 16) PB_F3(11, 17) PB_F3(12, 18)
-                             PB_F3(13, 19) PB_F3(14, 20) PB_F3(15, 21)
-                     : "+v"(acc[s][0]), "+v"(acc[s][1]), "+v"(acc[s][2])
-                     : "v"(f0), "v"(f1), "v"(f2), "v"(p[0]), "v"(p[1]), "v"(p[2]), "v"(p[3]), "v"(p[4]), "v"(p[5]),
-                       "v"(p[6]), "v"(p[7]), "v"(p[8]), "v"(p[9]), "v"(p[10]), "v"(p[11]), "v"(p[12]), "v"(p[13]),
-                       "v"(p[14]), "v"(p[15]));
-    }
-#undef PB_F3
-#undef PB_FD
-#else
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-        const float f0 = w[0] * S.sF[3 * s][e], f1 = w[0] * S.sF[3 * s + 1][e], f2 = w[0] * S.sF[3 * s + 2][e];
-        spread_steps16<0>(S.P[wave], lane & 15, f0, f1, f2, acc[s][0], acc[s][1], acc[s][2]);
-    }
-#endif
+    for (int q = 0; q < NS; q++)
+        spread_fma48(S.P[wave], p, lane, w[0] * S.sF[3 * q][e], w[0] * S.sF[3 * q + 1][e], w[0] * S.sF[3 * q + 2][e],
+                     acc[q]);
     __builtin_amdgcn_wave_barrier();
 }
 
 template <typename T, int NS>
 __device__ __forceinline__ void spread_batch(spread_lds<T, NS> &S, int wave, int lane, int head, int cnt,
-                                             const int *sb, const pb_kgrid &g, int li, int lj, int lk,
+                                             const int *sb, const int *off, const pb_kgrid &g, int li, int lj, int lk,
                                              T (&acc)[NS][3])
 {
-    if constexpr (sizeof(T) == 4) {
-        spread_batch16<NS>(S, wave, lane, head, cnt, sb, g, acc);
+    if constexpr (spread_stage<T, NS>::PREW) {
+        spread_batch16w(S, wave, lane, head, cnt, off, acc);
+    } else if constexpr (sizeof(T) == 4) {
+        spread_batch16r<NS>(S, wave, lane, head, cnt, sb, g, acc);
     } else {
         static_assert(NS == 1, "FP64 spreads one force set");
         T &a0 = acc[0][0], &a1 = acc[0][1], &a2 = acc[0][2];
         __builtin_amdgcn_wave_barrier();
-        if (lane < cnt) spread_weights<T>(S, wave, lane, S.ring[wave][(head + lane) & 127], sb, g);
+        if (lane < cnt) spread_weights<T, NS>(S, wave, lane, S.ring[wave][(head + lane) & 127], sb, g);
         __builtin_amdgcn_wave_barrier();
         for (int h = 0; h < cnt; h++) {
             const T p = S.P[wave][h][4 * lj + lk];
@@ -250,25 +358,33 @@ __device__ __forceinline__ void spread_batch(spread_lds<T, NS> &S, int wave, int
     }
 }
 
-// The wavefront's two sub-blocks gather from the nst staged sources.
+// The wavefront's two sub-blocks (z halves of its 4 x 4 column of the region)
+// gather from the nst staged sources.
 template <typename T, int NS>
-__device__ __forceinline__ void spread_process(spread_lds<T, NS> &S, int wave, int lane, int nst, const int *sbxy,
-                                               int gz, const pb_kgrid &g, T (&acc)[2][NS][3])
+__device__ __forceinline__ void spread_process(spread_lds<T, NS> &S, int wave, int lane, int nst, const region_geo &R,
+                                               const pb_kgrid &g, T (&acc)[2][NS][3])
 {
     constexpr int HB = spread_cfg<T>::HB;
     const int li = lane >> 4, lj = (lane >> 2) & 3, lk = lane & 3;
+    const int hx = wave >> 1, hy = wave & 1;
 #pragma unroll
     for (int sz = 0; sz < 2; sz++) {
-        const int sb[3] = {sbxy[0], sbxy[1], gz + 4 * sz};
+        const int off[3] = {4 * hx, 4 * hy, 4 * sz};
+        const int sb[3] = {R.gA[0] + off[0], R.gA[1] + off[1], R.gA[2] + off[2]};
         if (sb[0] >= g.n[0] || sb[1] >= g.n[1] || sb[2] >= g.n[2]) continue; // uniform per wavefront
+        const unsigned need = (1u << hx) | (1u << (2 + hy)) | (1u << (4 + sz));
         int head = 0, pend = 0;
         for (int eb = 0; eb < nst; eb += 64) {
             const int e = eb + lane;
             bool in = e < nst;
+            if constexpr (spread_stage<T, NS>::PREW) {
+                in = in && (__float_as_uint(S.st[e][spread_stage<T, NS>::MASK]) & need) == need;
+            } else {
 #pragma unroll
-            for (int a = 0; a < 3; a++) {
-                const int f0 = in ? S.sf0[a][e] : 0;
-                in = in && f0 > sb[a] - g.P[a] && f0 <= sb[a] + 3;
+                for (int a = 0; a < 3; a++) {
+                    const int f0 = in ? S.sf0[a][e] : 0;
+                    in = in && f0 > sb[a] - g.P[a] && f0 <= sb[a] + 3;
+                }
             }
             const unsigned long long m = __ballot(in);
             if (in) {
@@ -277,63 +393,37 @@ __device__ __forceinline__ void spread_process(spread_lds<T, NS> &S, int wave, i
             }
             pend += __popcll(m);
             while (pend >= HB) {
-                spread_batch<T, NS>(S, wave, lane, head, HB, sb, g, li, lj, lk, acc[sz]);
+                spread_batch<T, NS>(S, wave, lane, head, HB, sb, off, g, li, lj, lk, acc[sz]);
                 head = (head + HB) & 127;
                 pend -= HB;
             }
         }
-        if (pend > 0) spread_batch<T, NS>(S, wave, lane, head, pend, sb, g, li, lj, lk, acc[sz]);
+        if (pend > 0) spread_batch<T, NS>(S, wave, lane, head, pend, sb, off, g, li, lj, lk, acc[sz]);
     }
 }
 
+// Candidates q0 .. q1 - 1 of region R (numbered in run order) spread onto the
+// region's points in acc (the calling workgroup's four wavefronts).
 template <typename T, int NS>
-__device__ __forceinline__ void spread_region(
-    const int rid, const int *__restrict__ cstart, const double4 *__restrict__ spos,
-    const double4 *const (&sfrc)[NS], const pb_cells &cells, const pb_kgrid &g,
-    typename cplx<T>::type4 *const (&grid4)[NS])
+__device__ __forceinline__ void spread_chunk(spread_lds<T, NS> &S, const region_geo &R, long long q0, long long q1,
+                                             const int *__restrict__ cstart, const double4 *__restrict__ spos,
+                                             const double4 *const (&sfrc)[NS], const pb_cells &cells,
+                                             const pb_kgrid &g, T (&acc)[2][NS][3])
 {
     constexpr int CAP = spread_lds<T, NS>::CAP;
-    __shared__ spread_lds<T, NS> S;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int nr0 = (g.n[0] + 7) / 8, nr1 = (g.n[1] + 7) / 8, nr2 = (g.n[2] + 7) / 8;
-    if (rid >= nr0 * nr1 * nr2) return; // whole workgroup, before any barrier
-    const int rr[3] = {rid / (nr1 * nr2), (rid / nr2) % nr1, rid % nr2};
-    int gA[3], gB[3], ulo[3], uhi[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        gA[a] = 8 * rr[a];
-        gB[a] = min(gA[a] + 7, g.n[a] - 1);
-        const double lo = (gA[a] - 0.5 * g.P[a]) * g.h[a], hi = (gB[a] + 0.5 * g.P[a]) * g.h[a];
-        ulo[a] = (int)floor(lo * cells.inv_h[a] - 1e-9);
-        uhi[a] = (int)floor(hi * cells.inv_h[a] + 1e-9);
-    }
-    const int sbxy[2] = {gA[0] + 4 * (wave >> 1), gA[1] + 4 * (wave & 1)};
-    const int nc0 = cells.nc[0], nc1 = cells.nc[1], nc2 = cells.nc[2];
-    const int ncx = uhi[0] - ulo[0] + 1, ncy = uhi[1] - ulo[1] + 1;
-    const int lz0 = floordiv(ulo[2], nc2), nzseg = floordiv(uhi[2], nc2) - lz0 + 1;
-    const long long nruns = (long long)ncx * ncy * nzseg;
-    T acc[2][NS][3];
 #pragma unroll
     for (int sz = 0; sz < 2; sz++)
 #pragma unroll
         for (int q = 0; q < NS; q++) acc[sz][q][0] = acc[sz][q][1] = acc[sz][q][2] = (T)0;
     int nst = 0;
-    for (long long rb = 0; rb < nruns; rb += 256) {
+    long long base = 0; // candidates of the runs before this chunk of runs
+    for (long long rb = 0; rb < R.nruns && base < q1; rb += 256) {
         // one candidate run per thread: column (u0, u1), z segment with one image
         const long long r = rb + tid;
         int j = 0, len = 0;
         unsigned code = 0;
-        if (r < nruns) {
-            const int seg = (int)(r % nzseg);
-            const long long col = r / nzseg;
-            const int u0 = ulo[0] + (int)(col / ncy), u1 = ulo[1] + (int)(col % ncy);
-            const int l0 = floordiv(u0, nc0), l1 = floordiv(u1, nc1), l2 = lz0 + seg;
-            const int zs = max(ulo[2], l2 * nc2) - l2 * nc2, ze = min(uhi[2], (l2 + 1) * nc2 - 1) - l2 * nc2;
-            const int colbase = ((u0 - l0 * nc0) * nc1 + (u1 - l1 * nc1)) * nc2;
-            j = cstart[colbase + zs];
-            len = cstart[colbase + ze + 1] - j;
-            code = (unsigned)(l0 + 4) | ((unsigned)(l1 + 4) << 4) | ((unsigned)(l2 + 4) << 8);
-        }
+        if (r < R.nruns) region_run(R, r, cells, cstart, j, len, code);
         // exclusive scan of the run lengths
         S.run_pre[tid + 1] = len;
         if (tid == 0) S.run_pre[0] = 0;
@@ -348,23 +438,26 @@ __device__ __forceinline__ void spread_region(
         S.run_code[tid] = code;
         __syncthreads();
         const int total = S.run_pre[256];
-        for (int qb = 0; qb < total; qb += 256) {
-            const int q = qb + tid;
+        // this chunk's share of [q0, q1) (uniform)
+        const long long lo = q0 > base ? q0 - base : 0, hi = q1 - base < total ? q1 - base : total;
+        for (long long qb = lo; qb < hi; qb += 256) {
+            const long long q = qb + tid;
             bool hit = false;
             int src = 0, f0[3] = {0, 0, 0};
             unsigned cq = 0;
             double xw[3] = {0, 0, 0};
-            if (q < total) {
-                int lo = 0, hi = 255; // last run with run_pre <= q
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (S.run_pre[mid] <= q)
-                        lo = mid;
+            if (q < hi) {
+                const int qi = (int)q;
+                int lo2 = 0, hi2 = 255; // last run with run_pre <= q
+                while (lo2 < hi2) {
+                    const int mid = (lo2 + hi2 + 1) >> 1;
+                    if (S.run_pre[mid] <= qi)
+                        lo2 = mid;
                     else
-                        hi = mid - 1;
+                        hi2 = mid - 1;
                 }
-                src = S.run_j[lo] + (q - S.run_pre[lo]);
-                cq = S.run_code[lo];
+                src = S.run_j[lo2] + (qi - S.run_pre[lo2]);
+                cq = S.run_code[lo2];
                 const double4 x = spos[src];
                 xw[0] = x.x;
                 xw[1] = x.y;
@@ -374,7 +467,7 @@ __device__ __forceinline__ void spread_region(
                 for (int a = 0; a < 3; a++) {
                     const int l = (int)((cq >> (4 * a)) & 15) - 4;
                     f0[a] = (int)floor(xw[a] * g.inv_h[a] - 0.5 * g.P[a]) + 1 + l * g.n[a];
-                    hit = hit && f0[a] > gA[a] - g.P[a] && f0[a] <= gB[a];
+                    hit = hit && f0[a] > R.gA[a] - g.P[a] && f0[a] <= R.gB[a];
                 }
             }
             // staged in a fixed order: wavefront by wavefront, lanes in order
@@ -388,39 +481,80 @@ __device__ __forceinline__ void spread_region(
                 tot += S.wcount[w];
             }
             if (nst + tot > CAP) { // no room: gather from what is staged, then restart
-                spread_process<T, NS>(S, wave, lane, nst, sbxy, gA[2], g, acc);
+                spread_process<T, NS>(S, wave, lane, nst, R, g, acc);
                 nst = 0;
                 __syncthreads();
             }
             if (hit) {
                 const int e = nst + wbase +
                               (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                S.sx[0][e] = xw[0];
-                S.sx[1][e] = xw[1];
-                S.sx[2][e] = xw[2];
-                S.sf0[0][e] = f0[0];
-                S.sf0[1][e] = f0[1];
-                S.sf0[2][e] = f0[2];
-                S.scode[e] = cq;
+                if constexpr (spread_stage<T, NS>::PREW) {
+                    // the weights at the region's 8 points per axis, once per source
+                    float *rec = S.st[e];
+                    unsigned mask = 0;
 #pragma unroll
-                for (int q = 0; q < NS; q++) {
-                    const double4 f = sfrc[q][src];
-                    S.sF[3 * q][e] = (T)(f.x * g.scale);
-                    S.sF[3 * q + 1][e] = (T)(f.y * g.scale);
-                    S.sF[3 * q + 2][e] = (T)(f.z * g.scale);
+                    for (int a = 0; a < 3; a++) {
+                        const int l = (int)((cq >> (4 * a)) & 15) - 4;
+                        float w8[8];
+#pragma unroll
+                        for (int p = 0; p < 8; p++) {
+                            const int gq = R.gA[a] + p, mm = gq - f0[a];
+                            float v = 0.0f;
+                            if (mm >= 0 && mm < g.P[a]) {
+                                const double dx = (double)(gq - l * g.n[a]) * g.h[a] - xw[a];
+                                v = __expf(-(float)(dx * dx) * (float)(1.0 / g.eta));
+                            }
+                            w8[p] = v;
+                        }
+                        *reinterpret_cast<float4 *>(rec + 8 * a) = make_float4(w8[0], w8[1], w8[2], w8[3]);
+                        *reinterpret_cast<float4 *>(rec + 8 * a + 4) = make_float4(w8[4], w8[5], w8[6], w8[7]);
+                        mask |= (f0[a] > R.gA[a] - g.P[a] && f0[a] <= R.gA[a] + 3 ? 1u : 0u) << (2 * a);
+                        mask |= (f0[a] > R.gA[a] + 4 - g.P[a] && f0[a] <= R.gA[a] + 7 ? 1u : 0u) << (2 * a + 1);
+                    }
+                    const double4 f = sfrc[0][src];
+                    rec[24] = (float)(f.x * g.scale);
+                    rec[25] = (float)(f.y * g.scale);
+                    rec[26] = (float)(f.z * g.scale);
+                    rec[spread_stage<T, NS>::MASK] = __uint_as_float(mask);
+                } else {
+                    S.sx[0][e] = xw[0];
+                    S.sx[1][e] = xw[1];
+                    S.sx[2][e] = xw[2];
+                    S.sf0[0][e] = f0[0];
+                    S.sf0[1][e] = f0[1];
+                    S.sf0[2][e] = f0[2];
+                    S.scode[e] = cq;
+#pragma unroll
+                    for (int q = 0; q < NS; q++) {
+                        const double4 f = sfrc[q][src];
+                        S.sF[3 * q][e] = (T)(f.x * g.scale);
+                        S.sF[3 * q + 1][e] = (T)(f.y * g.scale);
+                        S.sF[3 * q + 2][e] = (T)(f.z * g.scale);
+                    }
                 }
             }
             nst += tot;
             __syncthreads();
         }
-        __syncthreads(); // the run arrays are rewritten by the next chunk
+        base += total;
+        __syncthreads(); // the run arrays are rewritten by the next chunk of runs
     }
-    spread_process<T, NS>(S, wave, lane, nst, sbxy, gA[2], g, acc);
+    spread_process<T, NS>(S, wave, lane, nst, R, g, acc);
+}
+
+// The region's points of acc to the grid (every point of the region inside
+// the grid, exactly once).
+template <typename T, int NS>
+__device__ __forceinline__ void spread_store(const region_geo &R, const pb_kgrid &g, const T (&acc)[2][NS][3],
+                                             typename cplx<T>::type4 *const (&grid4)[NS])
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane >> 4, lj = (lane >> 2) & 3, lk = lane & 3;
+    const int sbx = R.gA[0] + 4 * (wave >> 1), sby = R.gA[1] + 4 * (wave & 1);
 #pragma unroll
     for (int sz = 0; sz < 2; sz++) {
-        const int p0 = sbxy[0] + li, p1 = sbxy[1] + lj, p2 = gA[2] + 4 * sz + lk;
-        if (sbxy[0] >= g.n[0] || sbxy[1] >= g.n[1] || gA[2] + 4 * sz >= g.n[2]) continue;
+        const int p0 = sbx + li, p1 = sby + lj, p2 = R.gA[2] + 4 * sz + lk;
+        if (sbx >= g.n[0] || sby >= g.n[1] || R.gA[2] + 4 * sz >= g.n[2]) continue;
 #pragma unroll
         for (int q = 0; q < NS; q++) {
             typename cplx<T>::type4 o;
@@ -433,28 +567,63 @@ __device__ __forceinline__ void spread_region(
     }
 }
 
+// Local index of a region point: ((x * 8) + y) * 8 + z.
+__device__ __forceinline__ int region_local(int wave, int lane, int sz)
+{
+    const int px = 4 * (wave >> 1) + (lane >> 4), py = 4 * (wave & 1) + ((lane >> 2) & 3), pz = 4 * sz + (lane & 3);
+    return (px * 8 + py) * 8 + pz;
+}
+
+// A chunk's partial sums to its slot (512 points).
+template <typename T>
+__device__ __forceinline__ void spread_store_part(const T (&acc)[2][1][3], typename cplx<T>::type4 *__restrict__ slot)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int sz = 0; sz < 2; sz++) {
+        typename cplx<T>::type4 o;
+        o.x = acc[sz][0][0];
+        o.y = acc[sz][0][1];
+        o.z = acc[sz][0][2];
+        o.w = (T)0;
+        slot[region_local(wave, lane, sz)] = o;
+    }
+}
+
+// Whole regions, one per workgroup: single system (block = region) and
+// grouped systems (map entry = (system, region)).
 template <typename T>
 __global__ __launch_bounds__(64 * kSpreadWaves) void k_spread_regions(
     const int *__restrict__ cstart, const double4 *__restrict__ spos,
     const double4 *__restrict__ sfrc, pb_cells cells, pb_kgrid g,
     typename cplx<T>::type4 *__restrict__ grid4)
 {
+    __shared__ spread_lds<T, 1> S;
     const double4 *const fr[1] = {sfrc};
     typename cplx<T>::type4 *const gr[1] = {grid4};
-    spread_region<T, 1>((int)blockIdx.x, cstart, spos, fr, cells, g, gr);
+    region_geo R;
+    if (!region_of((int)blockIdx.x, cells, g, R)) return; // whole workgroup
+    T acc[2][1][3];
+    spread_chunk<T, 1>(S, R, 0, LLONG_MAX, cstart, spos, fr, cells, g, acc);
+    spread_store<T, 1>(R, g, acc, gr);
 }
 
 template <typename T>
 __global__ __launch_bounds__(64 * kSpreadWaves) void k_gspread(const pb_sys *__restrict__ sys,
                                                                const int2 *__restrict__ map)
 {
+    __shared__ spread_lds<T, 1> S;
     const int2 m = pb_map_at(map, blockIdx.x);
-    const auto &S = ((pb_const_sys)sys)[m.x];
-    const double4 *const fr[1] = {S.sfrc};
-    typename cplx<T>::type4 *const gr[1] = {(typename cplx<T>::type4 *)S.grid4};
-    const pb_cells cells = S.cells;
-    const pb_kgrid g = S.g;
-    spread_region<T, 1>(m.y, S.sstart, S.spos, fr, cells, g, gr);
+    const auto &D = ((pb_const_sys)sys)[m.x];
+    const double4 *const fr[1] = {D.sfrc};
+    typename cplx<T>::type4 *const gr[1] = {(typename cplx<T>::type4 *)D.grid4};
+    const pb_cells cells = D.cells;
+    const pb_kgrid g = D.g;
+    region_geo R;
+    if (!region_of(m.y, cells, g, R)) return;
+    T acc[2][1][3];
+    spread_chunk<T, 1>(S, R, 0, LLONG_MAX, D.sstart, D.spos, fr, cells, g, acc);
+    spread_store<T, 1>(R, g, acc, gr);
 }
 
 // Grouped, NS force sets per workgroup: block b spreads sets NS map[b].x ..
@@ -464,20 +633,150 @@ template <int NS>
 __global__ __launch_bounds__(64 * kSpreadWaves) void k_gspread_sets(const pb_sys *__restrict__ sys,
                                                                     const int2 *__restrict__ map)
 {
+    __shared__ spread_lds<float, NS> S;
     const int2 m = pb_map_at(map, blockIdx.x);
-    const pb_const_sys S = (pb_const_sys)sys + NS * m.x;
+    const pb_const_sys D = (pb_const_sys)sys + NS * m.x;
     const double4 *fr[NS];
     float4 *gr[NS];
 #pragma unroll
     for (int q = 0; q < NS; q++) {
-        fr[q] = S[q].sfrc;
-        gr[q] = (float4 *)S[q].grid4;
+        fr[q] = D[q].sfrc;
+        gr[q] = (float4 *)D[q].grid4;
     }
     const double4 *const(&frc)[NS] = fr;
     float4 *const(&grd)[NS] = gr;
-    const pb_cells cells = S[0].cells;
-    const pb_kgrid g = S[0].g;
-    spread_region<float, NS>(m.y, S[0].sstart, S[0].spos, frc, cells, g, grd);
+    const pb_cells cells = D[0].cells;
+    const pb_kgrid g = D[0].g;
+    region_geo R;
+    if (!region_of(m.y, cells, g, R)) return;
+    float acc[2][NS][3];
+    spread_chunk<float, NS>(S, R, 0, LLONG_MAX, D[0].sstart, D[0].spos, frc, cells, g, acc);
+    spread_store<float, NS>(R, g, acc, grd);
+}
+
+// ---- balanced spreading of one system: count, items, reduction -----------------
+// Counters: tot[0] partial slots handed out, tot[1] split regions, tot[2] the
+// work counter (all zero before the count pass).
+__global__ void k_spread_reset(int *__restrict__ tot)
+{
+    if (threadIdx.x < 4) tot[threadIdx.x] = 0;
+}
+
+// Count: one wavefront per region adds up the lengths of its candidate runs.
+// A region of more than C candidates is split into nch = ceil(cand / C)
+// chunks: it takes nch consecutive partial slots (base = pbase[r], chunk c in
+// slot base + c, owner[base + c] = (r, c)) and joins the list of split regions.
+// Slots and list positions follow the order of the atomics, but each region's
+// chunks keep their order, so the sums do not depend on it.
+__global__ __launch_bounds__(256) void k_spread_count(const int *__restrict__ cstart, pb_cells cells, pb_kgrid g,
+                                                      int nreg, int C, int *__restrict__ cand, int *__restrict__ pbase,
+                                                      int *__restrict__ mlist, int2 *__restrict__ owner, int part_cap,
+                                                      int *__restrict__ tot)
+{
+    const int e = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    region_geo R;
+    if (e >= nreg || !region_of(e, cells, g, R)) return; // whole wavefront
+    int sum = 0;
+    for (long long r = lane; r < R.nruns; r += 64) {
+        int j, len;
+        unsigned code;
+        region_run(R, r, cells, cstart, j, len, code);
+        sum += len;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    const int nch = sum > C ? (sum - 1) / C + 1 : 1;
+    int base = 0;
+    if (lane == 0) {
+        cand[e] = sum;
+        if (nch > 1) {
+            base = atomicAdd(&tot[0], nch);
+            pbase[e] = base;
+            if (base + nch <= part_cap) mlist[atomicAdd(&tot[1], 1)] = e;
+        }
+    }
+    if (nch > 1) { // (uniform: sum is the same in every lane)
+        base = __shfl(base, 0, 64);
+        for (int c = lane; c < nch; c += 64)
+            if (base + c < part_cap) owner[base + c] = make_int2(e, c);
+    }
+}
+
+// Items: each workgroup takes the next item from the counter until none is
+// left.  Items 0 .. nslots - 1 are the chunks of split regions (owner[i]; the
+// partial goes to slot i), then item nslots + r is region r whole (skipped if
+// it was split).
+template <typename T>
+__global__ __launch_bounds__(64 * kSpreadWaves) void k_spread_items(
+    const int *__restrict__ cstart, const double4 *__restrict__ spos, const double4 *__restrict__ sfrc,
+    pb_cells cells, pb_kgrid g, typename cplx<T>::type4 *__restrict__ grid4, int nreg, int C,
+    const int *__restrict__ cand, const int2 *__restrict__ owner, int part_cap,
+    typename cplx<T>::type4 *__restrict__ part, int *__restrict__ tot)
+{
+    __shared__ spread_lds<T, 1> S;
+    __shared__ int s_item;
+    const double4 *const fr[1] = {sfrc};
+    typename cplx<T>::type4 *const gr[1] = {grid4};
+    const int nslots = min(tot[0], part_cap);
+    const int nitems = nslots + nreg;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(&tot[2], 1);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads(); // every thread has read s_item before it is written again
+        if (it >= nitems) break;
+        const bool chunk = it < nslots;
+        const int2 oc = chunk ? owner[it] : make_int2(it - nslots, 0);
+        const int ce = cand[oc.x];
+        if (!chunk && ce > C) continue; // split: its chunks are items of their own
+        region_geo R;
+        if (!region_of(oc.x, cells, g, R)) continue; // (cannot happen: items name regions)
+        T acc[2][1][3];
+        const long long q0 = chunk ? (long long)oc.y * C : 0;
+        const long long q1 = chunk ? (q0 + C < ce ? q0 + C : (long long)ce) : (long long)ce;
+        spread_chunk<T, 1>(S, R, q0, q1, cstart, spos, fr, cells, g, acc);
+        if (chunk)
+            spread_store_part<T>(acc, part + (size_t)it * 512);
+        else
+            spread_store<T, 1>(R, g, acc, gr);
+        __syncthreads(); // LDS is reused by the next item
+    }
+}
+
+// Reduction: the chunks of each split region added in chunk order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_spread_reduce(const int *__restrict__ tot, const int *__restrict__ mlist,
+                                                       const int *__restrict__ pbase, const int *__restrict__ cand,
+                                                       int C, const typename cplx<T>::type4 *__restrict__ part,
+                                                       pb_cells cells, pb_kgrid g,
+                                                       typename cplx<T>::type4 *__restrict__ grid4)
+{
+    const int nm = tot[1];
+    for (int k = blockIdx.x; k < nm; k += gridDim.x) {
+        const int e = mlist[k];
+        const int ce = cand[e];
+        const int nch = ce > C ? (ce - 1) / C + 1 : 1;
+        const int s0 = pbase[e];
+        region_geo R;
+        if (!region_of(e, cells, g, R)) continue;
+        for (int lp = threadIdx.x; lp < 512; lp += 256) {
+            const int p0 = R.gA[0] + (lp >> 6), p1 = R.gA[1] + ((lp >> 3) & 7), p2 = R.gA[2] + (lp & 7);
+            if (p0 >= g.n[0] || p1 >= g.n[1] || p2 >= g.n[2]) continue;
+            T sx = 0, sy = 0, sz = 0;
+            for (int c = 0; c < nch; c++) {
+                const typename cplx<T>::type4 v = part[(size_t)(s0 + c) * 512 + lp];
+                sx += v.x;
+                sy += v.y;
+                sz += v.z;
+            }
+            typename cplx<T>::type4 o;
+            o.x = sx;
+            o.y = sy;
+            o.z = sz;
+            o.w = (T)0;
+            grid4[((size_t)p0 * g.n[1] + p1) * g.n[2] + p2] = o;
+        }
+    }
 }
 
 // ---- interpolation (gather) --------------------------------------------------
@@ -1253,6 +1552,83 @@ int upload_twiddles(pb_context *c, hipStream_t s)
     return PB_OK;
 }
 
+// Upper bound on the candidate runs a source can appear in over all regions:
+// per axis, the largest number of (region, unwrapped cell) pairs that fall on
+// one cell (one cell of slack on each side for rounding), multiplied.
+long long spread_multiplicity(const pb_cells &cells, const pb_kgrid &g)
+{
+    long long K = 1;
+    for (int a = 0; a < 3; a++) {
+        const int nr = (g.n[a] + 7) / 8, nc = cells.nc[a];
+        std::vector<int> cnt(nc, 0);
+        for (int r = 0; r < nr; r++) {
+            const int gA = 8 * r, gB = std::min(gA + 7, g.n[a] - 1);
+            const double lo = (gA - 0.5 * g.P[a]) * g.h[a], hi = (gB + 0.5 * g.P[a]) * g.h[a];
+            const int ulo = (int)std::floor(lo * cells.inv_h[a] - 1e-9) - 1;
+            const int uhi = (int)std::floor(hi * cells.inv_h[a] + 1e-9) + 1;
+            for (int u = ulo; u <= uhi; u++) cnt[((u % nc) + nc) % nc]++;
+        }
+        K *= *std::max_element(cnt.begin(), cnt.end());
+    }
+    return K;
+}
+
+// Spreading of one system.  Large spreading work (ns P^3 >= 1e9 window
+// points, e.g. 1e6 sources at P = 16) splits regions of more than C
+// candidates into chunks of C (count, items, reduction), C doubled until the
+// partial slots any split could need fit 256 MiB (the split regions' chunks
+// number at most 2 cand_bound / C).  Smaller work, and the allocation-free
+// entry point (whose scratch holds no partials), take one workgroup per
+// region: beside the pair kernel, the three extra launches cost some 20-50 us
+// (measured at 1e5 sources, 128^3, P 12: 0.668 against 0.618 ms per step),
+// which only pays off against long spreading.
+template <typename T>
+int spread_balanced(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells, int ns,
+                    typename cplx<T>::type4 *grid4)
+{
+    using C4 = typename cplx<T>::type4;
+    const int nreg = ((g.n[0] + 7) / 8) * ((g.n[1] + 7) / 8) * ((g.n[2] + 7) / 8);
+    const int *cstart = (const int *)c->cell_start.ptr;
+    const double4 *spos = (const double4 *)c->spos.ptr, *sfrc = (const double4 *)c->sfrc.ptr;
+    if (c->arena || (double)std::max(ns, 0) * g.P[0] * g.P[1] * g.P[2] < 1e9) {
+        hipLaunchKernelGGL(k_spread_regions<T>, dim3((unsigned)nreg), dim3(64 * kSpreadWaves), 0, s, cstart, spos,
+                           sfrc, cells, g, grid4);
+        return PB_OK;
+    }
+    const long long cand_bound = (long long)std::max(ns, 1) * spread_multiplicity(cells, g);
+    const size_t slot = 512 * sizeof(C4);
+    long long C = 4096;
+    while ((2 * (cand_bound / C) + 1) * (long long)slot > (256LL << 20) && C < (1LL << 24)) C *= 2;
+    const long long part_cap = 2 * (cand_bound / C) + 1;
+    if (part_cap + nreg > INT_MAX / 2)
+        return pb_fail(c, PB_ERR_UNSUPPORTED, "too many sources for the spreading plan");
+    int st;
+    if ((st = pb_reserve(c, &c->sp_int, sizeof(int) * (3 * (size_t)nreg + 4)))) return st;
+    if ((st = pb_reserve(c, &c->sp_items, sizeof(int2) * (size_t)part_cap))) return st;
+    if ((st = pb_reserve(c, &c->sp_part, slot * (size_t)part_cap))) return st;
+    int *cand = (int *)c->sp_int.ptr, *mlist = cand + nreg, *pbase = mlist + nreg, *tot = pbase + nreg;
+    int2 *owner = (int2 *)c->sp_items.ptr;
+    C4 *part = (C4 *)c->sp_part.ptr;
+    static int per_cu = 0; // resident item workgroups per compute unit
+    if (!per_cu) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spread_items<T>, 64 * kSpreadWaves, 0) != hipSuccess ||
+            nb < 1)
+            nb = 2;
+        per_cu = nb;
+    }
+    const int nblk = (int)std::min<long long>(nreg + part_cap, (long long)per_cu * c->cu_count);
+    hipLaunchKernelGGL(k_spread_reset, dim3(1), dim3(64), 0, s, tot);
+    hipLaunchKernelGGL(k_spread_count, dim3((unsigned)((nreg + 3) / 4)), dim3(256), 0, s, cstart, cells, g, nreg,
+                       (int)C, cand, pbase, mlist, owner, (int)part_cap, tot);
+    hipLaunchKernelGGL(k_spread_items<T>, dim3((unsigned)nblk), dim3(64 * kSpreadWaves), 0, s, cstart, spos, sfrc,
+                       cells, g, grid4, nreg, (int)C, (const int *)cand, (const int2 *)owner, (int)part_cap, part, tot);
+    hipLaunchKernelGGL(k_spread_reduce<T>, dim3((unsigned)std::min(nreg, 4 * c->cu_count)), dim3(256), 0, s,
+                       (const int *)tot, (const int *)mlist, (const int *)pbase, (const int *)cand, (int)C,
+                       (const C4 *)part, cells, g, grid4);
+    return PB_OK;
+}
+
 template <typename T>
 int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells,
                  const pb_box &box, double a, double d, int ns, int nt, double *u, int accumulate)
@@ -1275,12 +1651,7 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
     if ((st = upload_twiddles<T>(c, s))) return st;
     const kmult km = pb_make_kmult(g, box, a, d);
     void *grid4 = c->grid.ptr, *spec = c->spec.ptr;
-    {
-        const long long nreg = (long long)((g.n[0] + 7) / 8) * ((g.n[1] + 7) / 8) * ((g.n[2] + 7) / 8);
-        hipLaunchKernelGGL(k_spread_regions<T>, dim3((unsigned)nreg), dim3(64 * kSpreadWaves), 0, s,
-                           (const int *)c->cell_start.ptr, (const double4 *)c->spos.ptr,
-                           (const double4 *)c->sfrc.ptr, cells, g, (C4 *)grid4);
-    }
+    if ((st = spread_balanced<T>(c, s, g, cells, ns, (C4 *)grid4))) return st;
     const C *tw = (const C *)c->twid[sizeof(T) == 8 ? 0 : 1].ptr;
     const void *tws[3] = {tw + (g.n[0] - 4), tw + (g.n[1] - 4), tw + (g.n[2] - 4)};
     pb_mark(c, s, 2);

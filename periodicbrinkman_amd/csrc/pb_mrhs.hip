@@ -256,9 +256,16 @@ __global__ __launch_bounds__(64 * kMWaves, 2) void k_pair_mrhs(
             if (r >= R) continue;
             double *ur = u + (size_t)r * 3 * nt + o;
 #pragma unroll
-            for (int a = 0; a < 3; a++) ur[a] += (double)acc[a][nb][v];
+            for (int a = 0; a < 3; a++) ur[a] = (double)acc[a][nb][v]; // each (set, target) once per call
         }
     }
+}
+
+// u[i] += ur[i], i < n
+__global__ void k_add(long long n, double *__restrict__ u, const double *__restrict__ ur)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u[i] += ur[i];
 }
 
 // Fs[j][b][rp] = f[rp][perm[j]][b] (zero for rp >= R): the forces of the
@@ -292,6 +299,11 @@ int pb_force_sets(pb_context *c, hipStream_t s, int ns, int r0, int nr, int RP, 
         hipLaunchKernelGGL(k_force_sets, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ns, r0, nr, RP,
                            (const float *)c->mrhs_f.ptr, frc);
     return pb_hip_check(c, hipGetLastError(), "force sets launch");
+}
+
+void pb_add(hipStream_t s, long long n, double *u, const double *ur)
+{
+    if (n > 0) hipLaunchKernelGGL(k_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, u, ur);
 }
 
 int pb_sort_forces(pb_context *c, hipStream_t s, int ns, int R, int RP, const float *f)

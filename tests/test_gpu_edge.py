@@ -102,3 +102,27 @@ def test_one_context_through_changing_cell_grids(ctx):
         want = po.velocity_ewald(y, x, f, p.L, p.alpha, p.d, p.eps, p.rc, p.grid, (p.P,) * 3, p.m)
         got = ctx.velocity(p, dev(x), dev(f), dev(y)).cpu().numpy()
         assert relL2(got, want) < 1e-10, (rc, relL2(got, want))
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-6)])
+def test_dense_clusters_split_spreading_regions(ctx, precision, tol):
+    # 3e5 sources in 8 clusters of sigma 0.004 (a quarter of a grid spacing at
+    # 64^3) with P 16: 3e5 x 16^3 window points take the balanced spreading,
+    # and each cluster's region holds ~4e4 candidates, so it is split into
+    # chunks whose partial sums are added in order.  The k-space part against
+    # the oracle's spectral sum on 256 targets (all sources), and the full
+    # velocity on the same targets.
+    import torch
+    n = 300_000
+    L = (1.0, 1.0, 1.0)
+    x, f = pb.clustered_system(pb.config_seed(45), n, 8, 0.004, L)
+    y = np.vstack([x[:128], pb.uniform_system(pb.config_seed(46), 1, 128, L)[2]])
+    p = pb.Params(L=L, alpha=10.0, eps=1e-3, d=0.05, rc=0.02, grid=(64, 64, 64), P=16,
+                  m=float(np.sqrt(np.pi * 16)), precision=precision)
+    want = po.kspace_spectral(y, x, f, p.L, p.alpha, p.d, p.grid, (p.P,) * 3, p.m)
+    got = ctx.velocity_kspace(p, dev(x), dev(f), dev(y)).cpu().numpy()
+    torch.cuda.synchronize()
+    assert relL2(got, want) < tol, relL2(got, want)
+    want_all = want + po.real_velocity_cutoff(y, x, f, p.L, p.alpha, p.d, p.eps, p.rc)
+    got_all = ctx.velocity(p, dev(x), dev(f), dev(y)).cpu().numpy()
+    assert relL2(got_all, want_all) < tol, relL2(got_all, want_all)
