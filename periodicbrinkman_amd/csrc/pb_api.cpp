@@ -748,11 +748,14 @@ int mrhs_kspace(pb_context *c, hipStream_t s, const pb_params *p, const pb_cells
         if ((st = pb_hip_check(c, hipStreamSynchronize(s), "descriptor upload"))) return st;
         c->mr_sys_host = sys;
     }
-    // maps: spread, the five FFT passes, the gather (XCD-renumbered: a multiple of 8)
+    // maps: spread, the FFT passes (five, or three with the fused plane
+    // passes), the gather (XCD-renumbered: a multiple of 8)
     const int nreg = ((g.n[0] + 7) / 8) * ((g.n[1] + 7) / 8) * ((g.n[2] + 7) / 8);
-    int per[7];
+    int passes[5];
+    const int npass = pb_fft_passes(PB_PREC_MIXED, g, passes);
+    int per[7] = {0, 0, 0, 0, 0, 0, 0};
     per[0] = nreg;
-    for (int pass = 0; pass < 5; pass++) per[1 + pass] = pb_group_fft_blocks(PB_PREC_MIXED, pass, g);
+    for (int k = 0; k < npass; k++) per[1 + k] = pb_group_fft_blocks(PB_PREC_MIXED, passes[k], g);
     const bool same = g.P[0] == g.P[1] && g.P[1] == g.P[2];
     const int Ps[8] = {4, 6, 8, 10, 12, 16, 20, 24};
     const int P = same && std::find(Ps, Ps + 8, g.P[0]) != Ps + 8 ? g.P[0] : 0; // 0: per-axis windows
@@ -786,9 +789,10 @@ int mrhs_kspace(pb_context *c, hipStream_t s, const pb_params *p, const pb_cells
         const int n8 = n / pb_spread_sets, rest = n - n8 * pb_spread_sets;
         pb_group_spread_sets(s, sd, map + off[0], per[0] * n8);
         if (rest) pb_group_spread(s, sd + n8 * pb_spread_sets, map + off[0], per[0] * rest, PB_PREC_MIXED);
-        for (int pass = 0; pass < 5; pass++) {
-            const int ax = (pass == 0 || pass == 4) ? 2 : (pass == 2 ? 0 : 1);
-            if ((st = pb_group_fft(s, sd, map + off[1 + pass], per[1 + pass] * n, PB_PREC_MIXED, pass, g.n[ax], tw)))
+        for (int k = 0; k < npass; k++) {
+            const int pass = passes[k];
+            if ((st = pb_group_fft(s, sd, map + off[1 + k], per[1 + k] * n, PB_PREC_MIXED, pass,
+                                   g.n[pb_fft_axis(pass)], tw)))
                 return pb_fail(c, st, "FFT size");
         }
         if ((st = pb_group_interp(s, sd, map + off[6], (int)ngather, PB_PREC_MIXED, P)))
@@ -1013,17 +1017,29 @@ int group_prepare(pb_context *c, hipStream_t s, const std::vector<int> &idx, con
             }
         add(4, pr, 0, 0, 0, m);
     }
-    for (int pass = 0; pass < 5; pass++) {
-        const int ax = (pass == 0 || pass == 4) ? 2 : (pass == 2 ? 0 : 1);
+    // FFT stages: stage k runs pass k of the five-pass systems and, at stages
+    // 0, 2 and 4, passes 5, 2 and 6 of the systems with fused plane passes
+    std::vector<int> fused(n);
+    for (int q = 0; q < n; q++) {
+        int ps[5];
+        fused[q] = pb_fft_passes(prec_of(q), sys[q].g, ps) == 3;
+    }
+    for (int stage = 0; stage < 5; stage++) {
+        const int fpass = stage == 0 ? 5 : (stage == 2 ? 2 : (stage == 4 ? 6 : -1));
         for (int pr : precs)
-            for (int len = 4; len <= 512; len *= 2) {
-                std::vector<int2> m;
-                for (int q = 0; q < n; q++)
-                    if (prec_of(q) == pr && sys[q].g.n[ax] == len) {
-                        const int blocks = pb_group_fft_blocks(pr, pass, sys[q].g);
-                        for (int b = 0; b < blocks; b++) m.push_back(make_int2(q, b));
-                    }
-                add(5, pr, pass, len, 0, m);
+            for (int v = 0; v < (fpass >= 0 && fpass != stage ? 2 : 1); v++) {
+                const int pass = v ? fpass : stage;
+                const int ax = pb_fft_axis(pass);
+                for (int len = 4; len <= 512; len *= 2) {
+                    std::vector<int2> m;
+                    for (int q = 0; q < n; q++)
+                        if (prec_of(q) == pr && sys[q].g.n[ax] == len &&
+                            (fused[q] ? pass == fpass : pass == stage)) {
+                            const int blocks = pb_group_fft_blocks(pr, pass, sys[q].g);
+                            for (int b = 0; b < blocks; b++) m.push_back(make_int2(q, b));
+                        }
+                    add(5, pr, pass, len, 0, m);
+                }
             }
     }
     {

@@ -365,6 +365,12 @@ int pb_group_fft(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, in
 int pb_group_interp(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, int precision, int P);
 // blocks of one system in each grouped launch
 int pb_group_fft_blocks(int precision, int pass, const pb_kgrid &g);
+// The FFT passes of grid g in order (pb_group_fft's pass numbers: 0 z forward,
+// 1 y forward, 2 x with the multiply, 3 y inverse, 4 z inverse, 5 and 6 the
+// fused (y, z) plane passes forward and inverse); returns their number (3 or
+// 5).  pb_fft_axis: the axis whose length a pass's launch is classed by.
+int pb_fft_passes(int precision, const pb_kgrid &g, int *passes);
+int pb_fft_axis(int pass);
 constexpr int pb_interp_waves = 4; // wavefronts per gather block
 // targets per gather wavefront for window P (P^2 window columns per target:
 // 4 targets fill whole rounds of 64 lanes where P^2 is not a multiple of 64;

@@ -1420,6 +1420,168 @@ __device__ __forceinline__ void fft_z_c2r(const int blk, const typename cplx<T>:
     }
 }
 
+// ---- fused (y, z) plane passes ----------------------------------------------
+// Where a whole x-plane of the three half spectra fits in LDS (N x N planes,
+// N <= kPlaneMax<T>), the z and y transforms of a plane run in one workgroup:
+// the forward pass reads the real plane once and writes its spectrum once, the
+// inverse pass the reverse, so the transform makes three trips through HBM
+// (plane, x with the multiply, plane) instead of five.  The arithmetic is that
+// of the separate passes: z pencils two at a time as one complex transform,
+// the half spectra (kz = 0 .. N/2) transposed into y pencils in LDS.
+template <typename T> struct plane_max { static constexpr int N = 64; };
+template <> struct plane_max<double> { static constexpr int N = 32; };
+
+template <typename T, int N> struct plane_lds {
+    static constexpr int NP = N + 1, H = N / 2 + 1;
+    typename cplx<T>::type buf[3 * H * NP]; // z pencils [3 (N/2)][NP], then y pencils [3 H][NP]
+    typename cplx<T>::type tw[N];
+};
+
+template <typename T, int N>
+__device__ __forceinline__ void fft_yz_fwd(const int i0, const typename cplx<T>::type4 *__restrict__ R4,
+                                           typename cplx<T>::type *__restrict__ S, int n0,
+                                           const typename cplx<T>::type *__restrict__ gtw)
+{
+    using C = typename cplx<T>::type;
+    using C4 = typename cplx<T>::type4;
+    constexpr int NP = N + 1, H = N / 2 + 1;
+    __shared__ plane_lds<T, N> L;
+    C *buf = L.buf;
+    load_twiddles<C, N>(L.tw, gtw);
+    const C4 *P4 = R4 + (size_t)i0 * N * N;
+    for (int idx = threadIdx.x; idx < N * N; idx += 256) {
+        const int i1 = idx / N, z = idx % N, pp = i1 >> 1, side = i1 & 1;
+        const C4 f = P4[idx];
+        C a;
+        a.x = f.x;
+        a.y = f.y;
+        buf[(3 * pp + side) * NP + z] = a;
+        if (side == 0)
+            buf[(3 * pp + 2) * NP + z].x = f.z;
+        else
+            buf[(3 * pp + 2) * NP + z].y = f.z;
+    }
+    __syncthreads();
+    fft_stages<C, N, 1, 3 * (N / 2) * N>(buf, false, L.tw);
+    // half spectra of pencil i1 at kz = k, moved to y pencil (f, k) in place
+    constexpr int NI = N * H, KI = (NI + 255) / 256;
+    C o[KI][3];
+#pragma unroll
+    for (int kb = 0; kb < KI; kb++) {
+        const int idx = (int)threadIdx.x + 256 * kb;
+        if (NI % 256 == 0 || idx < NI) {
+            const int i1 = idx / H, k = idx % H, pp = i1 >> 1, side = i1 & 1, km = (N - k) & (N - 1);
+            const C z1 = buf[(3 * pp + side) * NP + k], z1m = buf[(3 * pp + side) * NP + km];
+            const C z3 = buf[(3 * pp + 2) * NP + k], z3m = buf[(3 * pp + 2) * NP + km];
+            o[kb][0].x = (T)0.5 * (z1.x + z1m.x);
+            o[kb][0].y = (T)0.5 * (z1.y - z1m.y);
+            o[kb][1].x = (T)0.5 * (z1.y + z1m.y);
+            o[kb][1].y = (T)0.5 * (z1m.x - z1.x);
+            if (side == 0) {
+                o[kb][2].x = (T)0.5 * (z3.x + z3m.x);
+                o[kb][2].y = (T)0.5 * (z3.y - z3m.y);
+            } else {
+                o[kb][2].x = (T)0.5 * (z3.y + z3m.y);
+                o[kb][2].y = (T)0.5 * (z3m.x - z3.x);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kb = 0; kb < KI; kb++) {
+        const int idx = (int)threadIdx.x + 256 * kb;
+        if (NI % 256 == 0 || idx < NI) {
+            const int i1 = idx / H, k = idx % H;
+#pragma unroll
+            for (int f = 0; f < 3; f++) buf[(f * H + k) * NP + i1] = o[kb][f];
+        }
+    }
+    __syncthreads();
+    fft_stages<C, N, 1, 3 * H * N>(buf, false, L.tw);
+    const size_t fs = (size_t)n0 * N * H;
+    for (int idx = threadIdx.x; idx < 3 * N * H; idx += 256) {
+        const int f = idx / (N * H), r = idx % (N * H), i1 = r / H, kz = r % H;
+        S[f * fs + ((size_t)i0 * N + i1) * H + kz] = buf[(f * H + kz) * NP + i1];
+    }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void fft_yz_inv(const int i0, const typename cplx<T>::type *__restrict__ S,
+                                           trip<T> *__restrict__ U3, int n0,
+                                           const typename cplx<T>::type *__restrict__ gtw)
+{
+    using C = typename cplx<T>::type;
+    constexpr int NP = N + 1, H = N / 2 + 1;
+    __shared__ plane_lds<T, N> L;
+    C *buf = L.buf;
+    load_twiddles<C, N>(L.tw, gtw);
+    const size_t fs = (size_t)n0 * N * H;
+    for (int idx = threadIdx.x; idx < 3 * N * H; idx += 256) {
+        const int f = idx / (N * H), r = idx % (N * H), i1 = r / H, kz = r % H;
+        buf[(f * H + kz) * NP + i1] = S[f * fs + ((size_t)i0 * N + i1) * H + kz];
+    }
+    __syncthreads();
+    fft_stages<C, N, 1, 3 * H * N>(buf, true, L.tw);
+    // z pencils of the pair (2 pp, 2 pp + 1) at k: U(N - k) = conj U(k)
+    constexpr int NI = (N / 2) * N, KI = (NI + 255) / 256;
+    C o[KI][3];
+#pragma unroll
+    for (int kb = 0; kb < KI; kb++) {
+        const int idx = (int)threadIdx.x + 256 * kb;
+        if (NI % 256 == 0 || idx < NI) {
+            const int pp = idx / N, k = idx % N;
+            const bool up = k >= H;
+            const int kk = up ? N - k : k;
+            C v[3][2];
+#pragma unroll
+            for (int f = 0; f < 3; f++)
+#pragma unroll
+                for (int sd = 0; sd < 2; sd++) {
+                    v[f][sd] = buf[(f * H + kk) * NP + 2 * pp + sd];
+                    if (up) v[f][sd].y = -v[f][sd].y;
+                }
+#pragma unroll
+            for (int sd = 0; sd < 2; sd++) { // ux + i uy
+                o[kb][sd].x = v[0][sd].x - v[1][sd].y;
+                o[kb][sd].y = v[0][sd].y + v[1][sd].x;
+            }
+            o[kb][2].x = v[2][0].x - v[2][1].y; // uz_a + i uz_b
+            o[kb][2].y = v[2][0].y + v[2][1].x;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kb = 0; kb < KI; kb++) {
+        const int idx = (int)threadIdx.x + 256 * kb;
+        if (NI % 256 == 0 || idx < NI) {
+            const int pp = idx / N, k = idx % N;
+#pragma unroll
+            for (int q = 0; q < 3; q++) buf[(3 * pp + q) * NP + k] = o[kb][q];
+        }
+    }
+    __syncthreads();
+    fft_stages<C, N, 1, 3 * (N / 2) * N>(buf, true, L.tw);
+    trip<T> *P3 = U3 + (size_t)i0 * N * N;
+    for (int idx = threadIdx.x; idx < N * N; idx += 256) {
+        const int i1 = idx / N, z = idx % N, pp = i1 >> 1, side = i1 & 1;
+        const C a = buf[(3 * pp + side) * NP + z], c3 = buf[(3 * pp + 2) * NP + z];
+        P3[idx] = trip<T>{a.x, a.y, side == 0 ? c3.x : c3.y};
+    }
+}
+
+template <typename T, int N, bool INV>
+__global__ __launch_bounds__(256) void k_fft_yz(const typename cplx<T>::type4 *__restrict__ R4,
+                                                typename cplx<T>::type *__restrict__ S, int n0,
+                                                const typename cplx<T>::type *__restrict__ gtw)
+{
+    if constexpr (N <= plane_max<T>::N) {
+        if (INV)
+            fft_yz_inv<T, N>((int)blockIdx.x, S, (trip<T> *)R4, n0, gtw);
+        else
+            fft_yz_fwd<T, N>((int)blockIdx.x, R4, S, n0, gtw);
+    }
+}
+
 template <typename T, int N>
 __global__ __launch_bounds__(256) void k_fft_z_r2c(const typename cplx<T>::type4 *__restrict__ R4,
                                                    typename cplx<T>::type *__restrict__ S, int n0,
@@ -1461,7 +1623,14 @@ __global__ __launch_bounds__(256) void k_gfft(const pb_sys *__restrict__ sys, co
     const int2 m = pb_map_at(map, blockIdx.x);
     const auto &S = ((pb_const_sys)sys)[m.x];
     const int h2 = S.g.n[2] / 2 + 1;
-    if constexpr (PASS == 0)
+    if constexpr (PASS == 5 || PASS == 6) {
+        if constexpr (N <= plane_max<T>::N) {
+            if (PASS == 5)
+                fft_yz_fwd<T, N>(m.y, (const typename cplx<T>::type4 *)S.grid4, (C *)S.spec, S.g.n[0], tw);
+            else
+                fft_yz_inv<T, N>(m.y, (const C *)S.spec, (trip<T> *)S.grid4, S.g.n[0], tw);
+        }
+    } else if constexpr (PASS == 0)
         fft_z_r2c<T, N>(m.y, (const typename cplx<T>::type4 *)S.grid4, (C *)S.spec, S.g.n[0], S.g.n[1], tw);
     else if constexpr (PASS == 1 || PASS == 3)
         fft_y<T, N>(m.y, (C *)S.spec, S.g.n[0], h2, PASS == 3, tw);
@@ -1480,7 +1649,16 @@ void launch_axis(hipStream_t s, int pass, const pb_kgrid &g, void *grid4, void *
     using C4 = typename cplx<T>::type4;
     const int h2 = g.n[2] / 2 + 1;
     const C *t = (const C *)tw;
-    if (pass == 0 || pass == 4) { // z
+    if (pass == 5 || pass == 6) { // (y, z) planes, one workgroup per x-plane
+        if constexpr (N <= plane_max<T>::N) {
+            if (pass == 5)
+                hipLaunchKernelGGL((k_fft_yz<T, N, false>), dim3(g.n[0]), dim3(256), 0, s, (const C4 *)grid4,
+                                   (C *)spec, g.n[0], t);
+            else
+                hipLaunchKernelGGL((k_fft_yz<T, N, true>), dim3(g.n[0]), dim3(256), 0, s, (const C4 *)grid4,
+                                   (C *)spec, g.n[0], t);
+        }
+    } else if (pass == 0 || pass == 4) { // z
         constexpr int PP = pbudget<T, 0>::E / (3 * N) > 0 ? pbudget<T, 0>::E / (3 * N) : 1;
         const int npair = g.n[0] * (g.n[1] / 2);
         dim3 gd((npair + PP - 1) / PP);
@@ -1503,13 +1681,27 @@ void launch_axis(hipStream_t s, int pass, const pb_kgrid &g, void *grid4, void *
     }
 }
 
-// pass: 0 z forward (r2c), 1 y forward, 2 x forward + multiply + inverse,
-// 3 y inverse, 4 z inverse (c2r)
+// The passes of a transform in order: 0 z forward (r2c), 1 y forward, 2 x
+// forward + multiply + inverse, 3 y inverse, 4 z inverse (c2r); or, for
+// n1 = n2 planes that fit in LDS, 5 (y, z) forward, 2, 6 (y, z) inverse.
+template <typename T>
+int fft_passes_t(const pb_kgrid &g, int *passes)
+{
+    if (g.n[1] == g.n[2] && g.n[2] <= plane_max<T>::N) {
+        passes[0] = 5;
+        passes[1] = 2;
+        passes[2] = 6;
+        return 3;
+    }
+    for (int k = 0; k < 5; k++) passes[k] = k;
+    return 5;
+}
+
 template <typename T>
 int fft_pass(hipStream_t s, int pass, const pb_kgrid &g, void *grid4, void *spec, const kmult &km,
              const void *const *tws)
 {
-    const int ax = (pass == 0 || pass == 4) ? 2 : (pass == 2 ? 0 : 1);
+    const int ax = pb_fft_axis(pass);
     const void *tw = tws[ax];
     switch (g.n[ax]) {
     case 4: launch_axis<T, 4>(s, pass, g, grid4, spec, km, tw); break;
@@ -1655,9 +1847,12 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
     const C *tw = (const C *)c->twid[sizeof(T) == 8 ? 0 : 1].ptr;
     const void *tws[3] = {tw + (g.n[0] - 4), tw + (g.n[1] - 4), tw + (g.n[2] - 4)};
     pb_mark(c, s, 2);
-    for (int pass = 0; pass < 5; pass++) {
+    int passes[5];
+    const int np = fft_passes_t<T>(g, passes);
+    for (int k = 0; k < np; k++) {
+        const int pass = passes[k];
         if (pass == 2) pb_mark(c, s, 3);
-        if (pass == 3) pb_mark(c, s, 4);
+        if (k > 0 && passes[k - 1] == 2) pb_mark(c, s, 4);
         if ((st = fft_pass<T>(s, pass, g, grid4, spec, km, tws))) return pb_fail(c, st, "FFT size");
     }
     pb_mark(c, s, 5);
@@ -1704,7 +1899,10 @@ int fft_blocks_t(int pass, const pb_kgrid &g)
 {
     const long long h2 = g.n[2] / 2 + 1;
     long long work, per;
-    if (pass == 0 || pass == 4) {
+    if (pass == 5 || pass == 6) {
+        per = 1;
+        work = g.n[0];
+    } else if (pass == 0 || pass == 4) {
         per = std::max(1, pbudget<T, 0>::E / (3 * g.n[2]));
         work = (long long)g.n[0] * (g.n[1] / 2);
     } else if (pass == 1 || pass == 3) {
@@ -1728,7 +1926,13 @@ void launch_gfft(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, in
     case 1: hipLaunchKernelGGL((k_gfft<T, N, 1>), gd, bd, 0, s, sys, map, t); break;
     case 2: hipLaunchKernelGGL((k_gfft<T, N, 2>), gd, bd, 0, s, sys, map, t); break;
     case 3: hipLaunchKernelGGL((k_gfft<T, N, 3>), gd, bd, 0, s, sys, map, t); break;
-    default: hipLaunchKernelGGL((k_gfft<T, N, 4>), gd, bd, 0, s, sys, map, t); break;
+    case 4: hipLaunchKernelGGL((k_gfft<T, N, 4>), gd, bd, 0, s, sys, map, t); break;
+    default:
+        if constexpr (N <= plane_max<T>::N) {
+            if (pass == 5) hipLaunchKernelGGL((k_gfft<T, N, 5>), gd, bd, 0, s, sys, map, t);
+            else hipLaunchKernelGGL((k_gfft<T, N, 6>), gd, bd, 0, s, sys, map, t);
+        }
+        break;
     }
 }
 
@@ -1770,6 +1974,16 @@ int group_interp_t(hipStream_t s, const pb_sys *sys, const int2 *map, int nblk, 
     return PB_OK;
 }
 } // namespace
+
+int pb_fft_axis(int pass)
+{
+    return pass == 1 || pass == 3 ? 1 : (pass == 2 ? 0 : 2);
+}
+
+int pb_fft_passes(int precision, const pb_kgrid &g, int *passes)
+{
+    return precision == PB_PREC_MIXED ? fft_passes_t<float>(g, passes) : fft_passes_t<double>(g, passes);
+}
 
 int pb_group_fft_blocks(int precision, int pass, const pb_kgrid &g)
 {
