@@ -8,6 +8,7 @@
 #include "pb_internal.h"
 
 #include <algorithm>
+#include <functional>
 #include <cstdint>
 #include <mutex>
 #include <thread>
@@ -700,8 +701,11 @@ size_t pad256(size_t n) { return (n + 255) & ~(size_t)255; }
 // 64 sets per grouped launch (fewer when their grids would pass 4 GB), each
 // set a system of the group with the shared cells, sorted points and k grid
 // and its own forces, grid, spectrum and block of u (overwritten).
+// fork is called once, after the first chunk's maps and force sets are
+// enqueued: work the caller puts beside the k space from there on does not
+// hold up these small launches.
 int mrhs_kspace(pb_context *c, hipStream_t s, const pb_params *p, const pb_cells &cells, const pb_box &box,
-                const pb_kgrid &g, int ns, int nt, int R, int RP, double *u)
+                const pb_kgrid &g, int ns, int nt, int R, int RP, double *u, const std::function<int()> &fork)
 {
     int st;
     const size_t gbytes = pad256((size_t)g.ng * 16);                                   // float4 per point
@@ -784,6 +788,7 @@ int mrhs_kspace(pb_context *c, hipStream_t s, const pb_params *p, const pb_cells
         }
         const pb_sys *sd = (const pb_sys *)c->mr_sys.ptr + r0;
         if ((st = pb_force_sets(c, s, ns, r0, n, RP, (double4 *)c->mr_frc.ptr))) return st;
+        if (r0 == 0 && (st = fork())) return st;
         // spreading: groups of pb_spread_sets sets per workgroup (one staging
         // and one set of window weights for all of them), then the rest one by one
         const int n8 = n / pb_spread_sets, rest = n - n8 * pb_spread_sets;
@@ -849,14 +854,21 @@ int pb_velocity_multirhs(pb_context *c, const pb_params *p, int ns, const double
         if (!c->pair_event[k] &&
             (st = pb_hip_check(c, hipEventCreateWithFlags(&c->pair_event[k], hipEventDisableTiming), "event")))
             return st;
-    if ((st = pb_hip_check(c, hipEventRecord(c->pair_event[0], s), "fork"))) return st;
-    if ((st = pb_hip_check(c, hipStreamWaitEvent(c->pair_stream, c->pair_event[0], 0), "fork"))) return st;
-    // from here every return joins the pair stream back into s
-    st = pb_run_mrhs_real(c, c->pair_stream, cells, kp, ntab, c->table_deg, box, ns, nt, R, RP,
-                          (double *)c->mrhs_u.ptr);
-    const int jst = pb_hip_check(c, hipEventRecord(c->pair_event[1], c->pair_stream), "join");
-    if (!st && !jst) st = mrhs_kspace(c, s, p, cells, box, g, ns, nt, R, RP, u);
-    const int wst = jst ? jst : pb_hip_check(c, hipStreamWaitEvent(s, c->pair_event[1], 0), "join");
+    bool forked = false;
+    int jst = 0;
+    auto fork = [&]() -> int {
+        int q;
+        if ((q = pb_hip_check(c, hipEventRecord(c->pair_event[0], s), "fork"))) return q;
+        if ((q = pb_hip_check(c, hipStreamWaitEvent(c->pair_stream, c->pair_event[0], 0), "fork"))) return q;
+        forked = true; // from here every return joins the pair stream back into s
+        q = pb_run_mrhs_real(c, c->pair_stream, cells, kp, ntab, c->table_deg, box, ns, nt, R, RP,
+                             (double *)c->mrhs_u.ptr);
+        jst = pb_hip_check(c, hipEventRecord(c->pair_event[1], c->pair_stream), "join");
+        return q ? q : jst;
+    };
+    st = mrhs_kspace(c, s, p, cells, box, g, ns, nt, R, RP, u, fork);
+    const int wst = !forked ? 0 : jst ? jst : pb_hip_check(c, hipStreamWaitEvent(s, c->pair_event[1], 0), "join");
+    if (!forked && !st) st = pb_fail(c, PB_ERR_INVALID, "multi-RHS real space not launched");
     if (st || wst) return st ? st : wst;
     pb_add(s, nu, u, (const double *)c->mrhs_u.ptr);
     return pb_hip_check(c, hipGetLastError(), "multi-RHS add");

@@ -812,7 +812,7 @@ __device__ __forceinline__ void interp_wave(const int i0, int nt, const double4 
     constexpr int TW = pb_interp_tw(P), NQ = P * P, NR = (TW * NQ + 63) / 64;
     static_assert(TW == 1 || NQ >= 64, "a round spans at most two targets");
     __shared__ T wts[kInterpWaves][TW][3][P];
-    __shared__ int off[kInterpWaves][TW][3][P]; // wrapped index times its stride
+    __shared__ unsigned off[kInterpWaves][TW][3][P]; // wrapped index times its stride, in bytes
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (i0 >= nt) return; // whole wavefront; no block barriers below
     const int nv = min(TW, nt - i0); // targets of this wavefront (the others repeat the last)
@@ -828,7 +828,8 @@ __device__ __forceinline__ void interp_wave(const int i0, int nt, const double4 
             wts[wave][w][a][p] = (T)__expf(-(float)(dx * dx) * (float)(1.0 / g.eta));
         int j = f0 + p;
         j = j < 0 ? j + g.n[a] : (j >= g.n[a] ? j - g.n[a] : j);
-        off[wave][w][a][p] = a == 0 ? j * g.n[1] * g.n[2] : (a == 1 ? j * g.n[2] : j);
+        off[wave][w][a][p] = (unsigned)sizeof(trip<T>) *
+                             (unsigned)(a == 0 ? j * g.n[1] * g.n[2] : (a == 1 ? j * g.n[2] : j));
     }
     __builtin_amdgcn_wave_barrier();
     T acc[TW][3];
@@ -844,16 +845,23 @@ __device__ __forceinline__ void interp_wave(const int i0, int nt, const double4 
             const int w = hi ? whi : wlo, qq = q - w * NQ;
             const int py = qq / P, pz = qq % P;
             const T wyz = wts[wave][w][1][py] * wts[wave][w][2][pz];
-            const int oyz = off[wave][w][1][py] + off[wave][w][2][pz];
+            const unsigned oyz = off[wave][w][1][py] + off[wave][w][2][pz];
+            // the column's x sum with the x weights, then times wy wz: one
+            // add (a 32-bit byte offset on the grid's base) and three FMAs
+            // per point
+            const char *col = reinterpret_cast<const char *>(U3);
             T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
             for (int px = 0; px < P; px++) {
-                const T wq = wts[wave][w][0][px] * wyz;
-                const trip<T> v = U3[(unsigned)(off[wave][w][0][px] + oyz)];
-                s0 += wq * v.x;
-                s1 += wq * v.y;
-                s2 += wq * v.z;
+                const T wx = wts[wave][w][0][px];
+                const trip<T> v = *reinterpret_cast<const trip<T> *>(col + (off[wave][w][0][px] + oyz));
+                s0 += wx * v.x;
+                s1 += wx * v.y;
+                s2 += wx * v.z;
             }
+            s0 *= wyz;
+            s1 *= wyz;
+            s2 *= wyz;
             if (hi) {
                 acc[whi][0] += s0;
                 acc[whi][1] += s1;

@@ -296,23 +296,35 @@ __global__ void k_sort_forces(int ns, int R, int RP, const int *__restrict__ per
     Fs[q] = rp < R ? f[(size_t)rp * 3 * ns + 3 * (size_t)perm[j] + b] : 0.0f;
 }
 
-__global__ void k_force_sets(int ns, int r0, int nr, int RP, const float *__restrict__ Fs, double4 *__restrict__ frc)
+// Forces of sets r0 .. r0 + nr - 1 from the sorted FP32 array Fs[j][3][RP]
+// to the per-set FP64 arrays frc[r][j] the spreading reads: tiles of 64
+// sources x 16 sets through LDS, so that the reads (runs of sets) and the
+// writes (runs of sources) are both contiguous.
+__global__ __launch_bounds__(256) void k_force_sets(int ns, int r0, int nr, int RP, const float *__restrict__ Fs,
+                                                    double4 *__restrict__ frc)
 {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (long long)ns * nr) return;
-    const int r = (int)(q / ns), j = (int)(q % ns);
-    const float *p = Fs + (size_t)j * 3 * RP + r0 + r;
-    frc[q] = make_double4((double)p[0], (double)p[RP], (double)p[2 * RP], 0.0);
+    __shared__ float T[64][3][17];
+    const int j0 = (int)blockIdx.x * 64, t0 = (int)blockIdx.y * 16;
+    for (int idx = threadIdx.x; idx < 64 * 3 * 16; idx += 256) {
+        const int r = idx % 16, jb = idx / 16, b = jb % 3, j = jb / 3;
+        const int jj = j0 + j, rr = t0 + r;
+        T[j][b][r] = (jj < ns && rr < nr) ? Fs[(size_t)jj * 3 * RP + (size_t)b * RP + r0 + rr] : 0.0f;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 64 * 16; idx += 256) {
+        const int j = idx % 64, r = idx / 64;
+        const int jj = j0 + j, rr = t0 + r;
+        if (jj < ns && rr < nr) frc[(size_t)rr * ns + jj] = make_double4(T[j][0][r], T[j][1][r], T[j][2][r], 0.0);
+    }
 }
 
 } // namespace
 
 int pb_force_sets(pb_context *c, hipStream_t s, int ns, int r0, int nr, int RP, double4 *frc)
 {
-    const long long n = (long long)ns * nr;
-    if (n > 0)
-        hipLaunchKernelGGL(k_force_sets, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ns, r0, nr, RP,
-                           (const float *)c->mrhs_f.ptr, frc);
+    if (ns > 0 && nr > 0)
+        hipLaunchKernelGGL(k_force_sets, dim3((unsigned)((ns + 63) / 64), (unsigned)((nr + 15) / 16)), dim3(256), 0,
+                           s, ns, r0, nr, RP, (const float *)c->mrhs_f.ptr, frc);
     return pb_hip_check(c, hipGetLastError(), "force sets launch");
 }
 

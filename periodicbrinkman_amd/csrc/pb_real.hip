@@ -42,8 +42,9 @@ struct wave_lds {
     double y[64][3];
     double acc[64][3];
     float4 stage[64];  // candidates of the current run: position - tile origin (FP32), index bits
-    uint2 q[2 * kQ];   // [0, kQ): FP64 ring; [kQ, 2 kQ): FP32 far ring
+    uint2 q[2 * kQ];   // [0, kQ): FP64 ring; [kQ, 2 kQ): FP32 far ring (without one: all FP64)
     uint2 qr[kQ];      // r < rcommon, moved out of evaluated FP64 batches
+    uint2 trash[64];   // where a lane whose test fails writes (the ring write takes no branch)
 };
 
 __device__ __forceinline__ unsigned lane_prefix(unsigned long long m)
@@ -116,7 +117,7 @@ struct batch_consts {
     double rcommon, rcommon2;
 };
 
-template <bool TABLE, int DEG>
+template <bool TABLE, int DEG, int RM>
 __device__ __forceinline__ void batch64(wave_lds &W, const int lane, const int cnt, const int head,
                                         const int rh, int &rn, unsigned &nin, const double4 *__restrict__ spos,
                                         const double4 *__restrict__ sfrc, const pb_box &box, const double rc2,
@@ -129,7 +130,7 @@ __device__ __forceinline__ void batch64(wave_lds &W, const int lane, const int c
     double d0 = 0, d1 = 0, d2 = 0, r2 = 0;
     double4 fs = make_double4(0, 0, 0, 0);
     if (on) {
-        e = W.q[(head + lane) & (kQ - 1)];
+        e = W.q[(head + lane) & RM];
         double s0, s1, s2;
         decode(e, box, t, s0, s1, s2);
         const double4 xs = spos[e.x];
@@ -286,15 +287,23 @@ __device__ __forceinline__ void pair_body(
     const double rc2 = cells.rc2;
     const float rc2f = (float)(rc2 * (1.0 + 1e-4)), rfar2f = (float)rfar2;
 
+    // FP64 ring index mask: kQ entries beside the far ring, else 2 kQ (the
+    // ring is then checked every two candidates: at most 63 + 128 pending)
+    constexpr int RM = MIXED ? kQ - 1 : 2 * kQ - 1;
     int qh = 0, qn = 0, fh = 0, fn = 0, rh = 0, rn = 0;
     unsigned n64 = 0, n32 = 0, ntest = 0, nread = 0;
     // evaluate every full ring (or, at the end, every non-empty one)
     auto flush = [&](const bool last) {
         while (qn >= 64 || (last && qn > 0)) {
             const int c = qn < 64 ? qn : 64;
-            batch64<TABLE, DEG>(W, lane, c, qh, rh, rn, n64, spos, sfrc, box, rc2, K, tab);
-            qh = (qh + c) & (kQ - 1);
+            batch64<TABLE, DEG, RM>(W, lane, c, qh, rh, rn, n64, spos, sfrc, box, rc2, K, tab);
+            qh = (qh + c) & RM;
             qn -= c;
+            if (rn >= 64) { // a batch adds at most 64: the rare ring (kQ) never overflows
+                batch_rare(W, lane, 64, rh, spos, sfrc, box, kpp);
+                rh = (rh + 64) & (kQ - 1);
+                rn -= 64;
+            }
         }
         while (rn >= 64 || (last && rn > 0)) {
             const int c = rn < 64 ? rn : 64;
@@ -318,10 +327,11 @@ __device__ __forceinline__ void pair_body(
     auto append = [&](const unsigned jk, const float r2, const unsigned code) {
         const bool in = r2 < rc2f;
         const unsigned long long m = __ballot(in);
-        if (!MIXED) {
-            if (in) W.q[(qh + qn + lane_prefix(m)) & (kQ - 1)] = make_uint2(jk, code);
+        if (!MIXED) { // (the caller flushes)
+            const unsigned slot = (unsigned)(qh + qn) + lane_prefix(m);
+            uint2 *dst = in ? &W.q[slot & RM] : &W.trash[lane];
+            *dst = make_uint2(jk, code);
             qn += __popcll(m);
-            if (qn >= 64) flush(false);
         } else {
             const bool far = in && r2 >= rfar2f;
             const unsigned long long mf = __ballot(far);
@@ -412,11 +422,13 @@ __device__ __forceinline__ void pair_body(
                         const float rb2 = b0 * b0 + b1 * b1 + b2 * b2;
                         append((unsigned)__float_as_int(sa.w), ra2, code);
                         append((unsigned)__float_as_int(sb.w), rb2, code);
+                        if (!MIXED && qn >= 64) flush(false);
                     }
                     if (k < nkeep) {
                         const float4 sa = W.stage[k];
                         const float a0 = y0 - sa.x, a1 = y1 - sa.y, a2 = y2 - sa.z;
                         append((unsigned)__float_as_int(sa.w), a0 * a0 + a1 * a1 + a2 * a2, code);
+                        if (!MIXED && qn >= 64) flush(false);
                     }
                     __builtin_amdgcn_wave_barrier();
                 }
