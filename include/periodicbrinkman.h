@@ -113,7 +113,7 @@ typedef struct pb_params {
     double rc;     /* real-space cutoff (> 0; pb_tune fills it when <= 0) */
     int grid[3];   /* FFT grid (pb_tune fills zeros) */
     int P;         /* Gaussian window points per axis */
-    double m;      /* window half-width in standard deviations (<= 0: sqrt(pi P)) */
+    double m;      /* window half-width in standard deviations (<= 0: sqrt(pi P - 5), P >= 4) */
     int precision; /* PB_PREC_FP64 or PB_PREC_MIXED */
     double tol;    /* target relative error for pb_tune */
     int kernel;    /* PB_KERNEL_TABLE, PB_KERNEL_DIRECT or PB_KERNEL_TABLE_FAR32 */
@@ -135,8 +135,8 @@ int pb_tune(pb_params *p, int ns, int nt);
  * kept.  The report (may be NULL) holds the prediction. */
 typedef struct pb_tune_report {
     double err_real;   /* e^{-(rc/d)^2}: real-space truncation */
-    double err_kspace; /* e^{-(d^2/4 - eta/2) kmax^2}: dropped modes */
-    double err_window; /* e^{-pi P/2}: window truncation and aliasing (m = sqrt(pi P)) */
+    double err_kspace; /* e^{-(X + 7.5)}, X = (d^2/4 - eta/2) kmax^2: dropped modes */
+    double err_window; /* e^{-m^2/2}: window truncation (m = sqrt(pi P - 5)) */
     double pairs;      /* predicted (target, source image) pairs inside rc */
     double t_real_ms, t_kspace_ms, t_bin_ms, t_total_ms; /* predicted phase times */
     int candidates;    /* (grid, P) combinations scored */

@@ -194,7 +194,7 @@ int make_kgrid(pb_context *c, const pb_params *p, pb_kgrid *g, const int *Paxis 
     }
     int st;
     if ((st = check_kgrid(c, *g))) return st;
-    double m = p->m > 0 ? p->m : std::sqrt(M_PI * p->P);
+    double m = p->m > 0 ? p->m : pb_window_m(p->P);
     double eta = p->P * g->h[0] / m; // fftBrinkman.h:113-114
     g->eta = eta * eta / 2;
     double sc = 1.0 / (g->eta * M_PI);

@@ -150,6 +150,8 @@ struct pb_sys {
 // returns that error (tn = 0: no table within the LDS budget).
 double pb_build_table(pb_kparams *k, double z0, double z1, std::vector<double> &coef, int deg = PB_TAB_DEG,
                       double tol = 4e-15);
+// Window half-width (in standard deviations) of P points when none is given.
+double pb_window_m(int P);
 void pb_fill_radial(pb_radial *r, double a, double d);
 void pb_fill_kparams(pb_kparams *k, double a, double d, double e, double rfar);
 void pb_fill_box(pb_box *b, const double *L);

@@ -258,6 +258,23 @@ void pb_fill_box(pb_box *b, const double *L)
     }
 }
 
+double pb_window_m(int P)
+{
+    return std::sqrt(std::fmax(M_PI * P - 5.0, 0.5 * M_PI * P));
+}
+
+namespace {
+double window_m2(int P)
+{
+    const double m = pb_window_m(P);
+    return m * m;
+}
+double kspace_X(double lt)
+{
+    return std::fmax(lt - 7.5, 0.4 * lt);
+}
+} // namespace
+
 extern "C" {
 
 void pb_params_default(pb_params *p)
@@ -270,14 +287,22 @@ void pb_params_default(pb_params *p)
     p->tol = 1e-6;
 }
 
-// Error model (relative).  Real space: truncation ~ e^{-(rc/d)^2}.  k space:
+// Error model (relative L2 of the velocity), calibrated against converged
+// sums of the FP64 oracle (1e5 uniform points on 128^3 with P 12, 512 uneven
+// systems, 1e6 clustered points on 256^3 with P 16, tolerances 4e-6 to 1e-11).
+// Real space: truncation ~ e^{-(rc/d)^2}, rc = d sqrt(ln(1/tol) + 1).  k space:
 // spreading and gathering each multiply mode k by e^{-eta k^2/4} and the
-// multiply divides both out, so the modes the grid drops carry
-// e^{-(d^2/4 - eta/2) k^2}; requiring this at the Nyquist wavenumber k_max
-// gives d^2 = 4 [ln(1/tol)/k_max^2 + eta/2].  The window half-width m balances
-// truncation of the Gaussian (e^{-m^2/2}) against aliasing (e^{-pi^2 P^2/(2 m^2)}):
-// m = sqrt(pi P).  The grid (when not given) gives about two grid points per
-// particle, rounded up to a power of two per axis.
+// multiply divides both out, so a dropped mode carries e^{-(d^2/4 - eta/2) k^2};
+// at the Nyquist wavenumber that is e^{-X}, X = (d^2/4 - eta/2) k_max^2.  The
+// velocity sits mostly at small k: the dropped modes' share of it measured
+// e^{-(X + 7.5)} or less, so X = ln(1/tol) - 7.5, and never below 0.4 ln(1/tol)
+// (at loose tolerances the error falls off slower than e^{-X}).  Window: the
+// Gaussian cut at the window's edge leaves e^{-m^2/2}; aliasing, which the
+// deconvolution amplifies, falls with smaller m, and the measured optimum is
+// m^2 = pi P - 5 (the unamplified balance would be m^2 = pi P).  The grid
+// (when not given) gives about two grid points per particle, rounded up to a
+// power of two per axis.
+
 int pb_tune(pb_params *p, int ns, int nt)
 {
     if (!p || !(p->alpha > 0) || !(p->L[0] > 0) || !(p->L[1] > 0) || !(p->L[2] > 0) || p->P < 2)
@@ -295,13 +320,13 @@ int pb_tune(pb_params *p, int ns, int nt)
             p->grid[i] = g;
         }
     }
-    if (!(p->m > 0)) p->m = std::sqrt(M_PI * p->P);
+    if (!(p->m > 0)) p->m = std::sqrt(window_m2(p->P));
     if (!(p->d > 0)) {
         double kmax = 1e300;
         for (int i = 0; i < 3; i++) kmax = std::fmin(kmax, M_PI * p->grid[i] / p->L[i]);
         double w = p->P * (p->L[0] / p->grid[0]) / p->m;
         double eta = w * w / 2;
-        p->d = 2.0 * std::sqrt((lt + 2.0) / (kmax * kmax) + eta / 2);
+        p->d = 2.0 * std::sqrt(kspace_X(lt) / (kmax * kmax) + eta / 2);
     }
     if (!(p->rc > 0)) p->rc = p->d * std::sqrt(lt + 1.0);
     if (p->rc > Lmin * 3) p->rc = Lmin * 3;
@@ -344,8 +369,8 @@ int pb_autotune(pb_params *p, int ns, int nt, pb_tune_report *rep)
     pb_tune_report br{};
     int scored = 0;
     for (int P : Ps) {
-        const double m = std::sqrt(M_PI * P);
-        const double ew = std::exp(-M_PI * P / 2);
+        const double m = std::sqrt(window_m2(P));
+        const double ew = std::exp(-m * m / 2);
         if (ew > tol) continue;
         // grids: one spacing h for all axes, each axis rounded up to a power of two
         for (int k = 2; k <= 9; k++) {
@@ -362,7 +387,7 @@ int pb_autotune(pb_params *p, int ns, int nt, pb_tune_report *rep)
             double kmax = 1e300, hx = p->L[0] / g[0];
             for (int i = 0; i < 3; i++) kmax = std::fmin(kmax, M_PI * g[i] / p->L[i]);
             const double w = P * hx / m, eta = w * w / 2;
-            const double d = 2.0 * std::sqrt((lt + 2.0) / (kmax * kmax) + eta / 2);
+            const double d = 2.0 * std::sqrt(kspace_X(lt) / (kmax * kmax) + eta / 2);
             const double rc = d * std::sqrt(lt + 1.0);
             if (rc > 3 * Lmin) continue;
             const double pairs = (double)nt * ns / vol * (4.0 / 3.0) * M_PI * rc * rc * rc;
@@ -384,7 +409,7 @@ int pb_autotune(pb_params *p, int ns, int nt, pb_tune_report *rep)
                 bestp.d = d;
                 bestp.rc = rc;
                 br.err_real = std::exp(-(rc / d) * (rc / d));
-                br.err_kspace = std::exp(-(d * d / 4 - eta / 2) * kmax * kmax);
+                br.err_kspace = std::exp(-((d * d / 4 - eta / 2) * kmax * kmax + 7.5));
                 br.err_window = ew;
                 br.pairs = pairs;
                 br.t_real_ms = t_real * 1e3;

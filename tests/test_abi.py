@@ -51,10 +51,15 @@ def test_tuner_error_model():
     p = pb.Params(alpha=10.0, eps=1e-3, grid=(128, 128, 128), P=12, tol=1e-6).tune(100000, 100000)
     kmax = math.pi * 128
     eta = (12 / 128 / p.m) ** 2 / 2
-    assert abs(p.m - math.sqrt(math.pi * 12)) < 1e-12
-    # the dropped k-space modes carry e^{-(d^2/4 - eta/2) k^2} <= tol e^{-2}
-    assert math.exp(-(p.d ** 2 / 4 - eta / 2) * kmax ** 2) <= 1e-6 * math.exp(-2) * 1.0001
+    assert abs(p.m - math.sqrt(math.pi * 12 - 5)) < 1e-12
+    # the dropped modes' weight at k_max, e^{-X}, is tol e^{7.5} (X = ln(1/tol) - 7.5)
+    X = (p.d ** 2 / 4 - eta / 2) * kmax ** 2
+    assert X == pytest.approx(math.log(1e6) - 7.5, rel=1e-12)
     assert math.exp(-(p.rc / p.d) ** 2) <= 1e-6 / math.e * 1.0001
+    # loose tolerances keep X >= 0.4 ln(1/tol)
+    r = pb.Params(alpha=10.0, eps=1e-3, grid=(128, 128, 128), P=12, tol=1e-3).tune(100000, 100000)
+    X = (r.d ** 2 / 4 - eta / 2) * kmax ** 2
+    assert X == pytest.approx(0.4 * math.log(1e3), rel=1e-12)
     q = pb.Params(alpha=1.0, tol=1e-8).tune(5000, 5000)
     assert all(g >= 16 and (g & (g - 1)) == 0 for g in q.grid)
     with pytest.raises(pb.PBError):

@@ -18,9 +18,9 @@ def _tuned(tol, L=(1.0, 1.0, 1.0), ns=100_000, nt=100_000, alpha=10.0, precision
 def test_bounds_met(tol):
     p, rep = _tuned(tol)
     assert rep["candidates"] > 0
-    assert max(rep["err_real"], rep["err_kspace"], rep["err_window"]) <= tol
-    # the error model's own relations (window m = sqrt(pi P), rc = d sqrt(ln(1/tol) + 1))
-    assert math.isclose(p.m, math.sqrt(math.pi * p.P), rel_tol=1e-12)
+    assert max(rep["err_real"], rep["err_kspace"], rep["err_window"]) <= tol * (1 + 1e-12)
+    # the error model's own relations (window m = sqrt(pi P - 5), rc = d sqrt(ln(1/tol) + 1))
+    assert math.isclose(p.m, math.sqrt(math.pi * p.P - 5), rel_tol=1e-12)
     assert math.isclose(p.rc, p.d * math.sqrt(math.log(1 / tol) + 1), rel_tol=1e-12)
     assert all(g & (g - 1) == 0 and 4 <= g <= 512 for g in p.grid)
     assert p.P in (4, 6, 8, 10, 12, 16, 20, 24) and p.P <= min(p.grid)

@@ -67,7 +67,8 @@ def test_multirhs_dense_staging_matches_single_sets():
     x, _, _ = pb.uniform_system(pb.config_seed(8), n, 0, L)
     rng = np.random.default_rng(9)
     F = rng.standard_normal((R, n, 3)).astype(np.float32)
-    p = pb.Params(L=L, alpha=5.0, eps=1e-3, d=0.07, rc=0.1, grid=(32, 32, 32), P=12, precision="mixed")
+    p = pb.Params(L=L, alpha=5.0, eps=1e-3, d=0.07, rc=0.1, grid=(32, 32, 32), P=12, m=float(np.sqrt(np.pi * 12)),
+                  precision="mixed")
     ctx = pb.Context(0)
     dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")
     X = dev(x)
@@ -90,7 +91,8 @@ def test_multirhs_windows_match_single_sets(P, grid):
     x, _, y = pb.uniform_system(pb.config_seed(10), ns, nt, L)
     rng = np.random.default_rng(11)
     F = rng.standard_normal((R, ns, 3)).astype(np.float32)
-    p = pb.Params(L=L, alpha=8.0, eps=1e-3, d=0.05, rc=0.12, grid=(grid,) * 3, P=P, precision="mixed")
+    p = pb.Params(L=L, alpha=8.0, eps=1e-3, d=0.05, rc=0.12, grid=(grid,) * 3, P=P, m=float(np.sqrt(np.pi * P)),
+                  precision="mixed")
     ctx = pb.Context(0)
     dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")
     X, Y = dev(x), dev(y)
@@ -111,7 +113,8 @@ def test_multirhs_repeated_calls_follow_the_forces():
     L = (1.0, 1.0, 1.0)
     x, _, y = pb.uniform_system(pb.config_seed(12), ns, nt, L)
     rng = np.random.default_rng(13)
-    p = pb.Params(L=L, alpha=8.0, eps=1e-3, d=0.05, rc=0.12, grid=(32,) * 3, P=12, precision="mixed")
+    p = pb.Params(L=L, alpha=8.0, eps=1e-3, d=0.05, rc=0.12, grid=(32,) * 3, P=12, m=float(np.sqrt(np.pi * 12)),
+                  precision="mixed")
     ctx = pb.Context(0)
     dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")
     X, Y = dev(x), dev(y)
@@ -164,7 +167,8 @@ def test_invalid_grid_or_window_rejected_before_any_work(grid, P):
     n = 50
     x = torch.rand((n, 3), dtype=torch.float64, device="cuda")
     F = torch.rand((2, n, 3), dtype=torch.float32, device="cuda")
-    p = pb.Params(alpha=10.0, eps=1e-3, d=0.05, rc=0.2, grid=grid, P=P, precision="mixed")
+    p = pb.Params(alpha=10.0, eps=1e-3, d=0.05, rc=0.2, grid=grid, P=P, m=float(np.sqrt(np.pi * P)),
+                  precision="mixed")
     u = torch.full((n, 3), 3.0, dtype=torch.float64, device="cuda")
     U = torch.full((2, n, 3), 3.0, dtype=torch.float64, device="cuda")
     calls = [lambda: ctx.velocity(p, x, x, x, u), lambda: ctx.velocity_kspace(p, x, x, x, u),
@@ -182,7 +186,8 @@ def test_invalid_grid_or_window_rejected_before_any_work(grid, P):
 def test_python_wrappers_reject_mismatched_shapes():
     import torch
     ctx = pb.Context(0)
-    p = pb.Params(alpha=10.0, eps=1e-3, d=0.05, rc=0.2, grid=(16, 16, 16), P=8, precision="mixed")
+    p = pb.Params(alpha=10.0, eps=1e-3, d=0.05, rc=0.2, grid=(16, 16, 16), P=8, m=float(np.sqrt(np.pi * 8)),
+                  precision="mixed")
     x = torch.rand((40, 3), dtype=torch.float64, device="cuda")
     for args in [(x, x[:39], x), (x, x, x, torch.empty((39, 3), dtype=torch.float64, device="cuda"))]:
         with pytest.raises(pb.PBError) as ei:
