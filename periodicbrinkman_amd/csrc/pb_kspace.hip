@@ -424,18 +424,23 @@ __device__ __forceinline__ void spread_chunk(spread_lds<T, NS> &S, const region_
         int j = 0, len = 0;
         unsigned code = 0;
         if (r < R.nruns) region_run(R, r, cells, cstart, j, len, code);
-        // exclusive scan of the run lengths
-        S.run_pre[tid + 1] = len;
-        if (tid == 0) S.run_pre[0] = 0;
-        __syncthreads();
-        for (int off = 1; off < 256; off <<= 1) {
-            const int v = tid + 1 > off ? S.run_pre[tid + 1 - off] : 0;
-            __syncthreads();
-            S.run_pre[tid + 1] += v;
-            __syncthreads();
+        // exclusive scan of the run lengths: within each wavefront by
+        // shuffles, then the wavefront totals (one block barrier)
+        int incl = len;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += v;
         }
+        if (lane == 63) S.wcount[wave] = incl;
         S.run_j[tid] = j;
         S.run_code[tid] = code;
+        __syncthreads();
+        int wpre = 0;
+#pragma unroll
+        for (int w = 0; w < kSpreadWaves; w++) wpre += w < wave ? S.wcount[w] : 0;
+        S.run_pre[tid + 1] = wpre + incl;
+        if (tid == 0) S.run_pre[0] = 0;
         __syncthreads();
         const int total = S.run_pre[256];
         // this chunk's share of [q0, q1) (uniform)
