@@ -57,7 +57,6 @@ CLUSTER_TOL = 3e-6
 # 3x3 apply) plus six exp/erfc at 20 FLOP each.
 FLOP_PER_PAIR = 200.0
 PEAK_FP64, PEAK_FP32 = 78.6, 157.3  # TFLOP/s, MI355X FP64 (vector = matrix) and FP32 peaks
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
 COPY_BYTES = 1 << 29  # copy-calibration size (512 MiB read + 512 MiB written)
 
 
@@ -559,17 +558,13 @@ def leg_kernel_rooflines(workload, spec):
     return out
 
 
-def pair_traffic(workload):
-    """HBM bytes per pair-kernel launch from the committed counter summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or None."""
-    try:
-        with open(TRAFFIC_FILE) as fh:
-            t = json.load(fh)
-    except (OSError, ValueError):
-        return None
-    if t.get("workload") != workload:
-        return None
-    return t.get("bytes_per_launch")
+def pair_traffic(kernel="k_pair_table<false, 6>"):
+    """HBM bytes per launch of the headline's pair kernel from the committed
+    counter summary (FETCH_SIZE + WRITE_SIZE of bench.py --profile headline,
+    the same parameters as the timed mixed form, scaled by what the counters
+    read for a copy of known size), or None."""
+    e = kernel_counters("headline").get(kernel)
+    return e.get("hbm_bytes") if e else None
 
 
 def main():
@@ -705,7 +700,7 @@ def main():
         "phases_ms": ph_mixed,
         "pairs": {"fp64": n64, "fp32": n32},
         "roofline": {"bound": "valu-fp64", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": pair_traffic(workload),
+                     "frac": achieved / peak, "traffic": pair_traffic(),
                      "kernel": "k_pair_table (real-space pair kernel)",
                      "per_kernel": headline_kernel_rooflines(ph_mixed, pairs),
                      "note": "the FP64 vector-ALU roof (78.6 TFLOP/s on MI355X) of the pair kernel, which "
@@ -725,13 +720,14 @@ def main():
             ("pair kernel, degree-8 class", ["k_gpair_table<false, 8>"], "valu-fp64", None),
             ("spread", ["k_gspread<float>"], "valu-fp32", None),
             ("gather", ["k_ginterp<float, 10>"], "valu-fp32", None),
-            ("FFT passes, 64^3 systems", ["k_gfft<float, 64, %d>" % q for q in range(5)], "hbm", None),
-            ("FFT passes, 32^3 systems", ["k_gfft<float, 32, %d>" % q for q in range(5)], "hbm", None)])
+            ("FFT passes, 64^3 systems", ["k_gfft<float, 64, %d>" % q for q in (5, 2, 6)], "hbm", None),
+            ("FFT passes, 32^3 systems", ["k_gfft<float, 32, %d>" % q for q in (5, 2, 6)], "hbm", None)])
     if not args.no_more:
         result["clustered_form"] = clustered_leg(ctx, 3)
         result["clustered_form"]["per_kernel_roofline"] = leg_kernel_rooflines("clustered", [
             ("pair kernel", ["k_pair_table<false, 6>"], "valu-fp64", None),
-            ("spread", ["k_spread_regions<float>"], "valu-fp32", None),
+            ("spread (count, chunks and regions, chunk reduction)",
+             ["k_spread_count", "k_spread_items<float>", "k_spread_reduce<float>"], "valu-fp32", None),
             ("gather", ["k_interp<float, 16>"], "valu-fp32", None)])
         result["multirhs_form"] = multirhs_leg(ctx, 3)
         mr = result["multirhs_form"]
