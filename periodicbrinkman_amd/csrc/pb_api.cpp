@@ -228,7 +228,7 @@ const pb_kparams &cached_kparams(double a, double d, double e, double rfar)
 // [rcommon, rc]: coefficients in coef (empty without a table); the table
 // pointers of kp are left for the caller.
 // Degree of the radial table for a precision: the FP64 form keeps the table
-// at FP64 rounding (4e-15), the mixed form needs 1e-10 and takes degree 6
+// at FP64 rounding (4e-15), the mixed form needs PB_TAB_TOL_MIXED and takes degree 6
 // (112 bytes of coefficients per pair instead of 208, 14 FMAs instead of 26)
 // where that fits the LDS budget (large alpha d needs short intervals), else 8,
 // else 12.
@@ -247,9 +247,9 @@ void host_kparams(const pb_params *p, double rfar, pb_kparams *kp, std::vector<d
     if (p->kernel != PB_KERNEL_DIRECT && p->rc > kp->rcommon) {
         const bool mixed = p->precision == PB_PREC_MIXED;
         const double z0 = kp->rcommon / p->d, z1 = p->rc / p->d * (1 + 1e-9);
-        pb_build_table(kp, z0, z1, coef, table_degree(p), mixed ? 1e-10 : 4e-15);
-        if (kp->tn == 0 && mixed) pb_build_table(kp, z0, z1, coef, PB_TAB_DEG_MID, 1e-10);
-        if (kp->tn == 0 && mixed) pb_build_table(kp, z0, z1, coef, PB_TAB_DEG, 1e-10);
+        pb_build_table(kp, z0, z1, coef, table_degree(p), mixed ? PB_TAB_TOL_MIXED : 4e-15);
+        if (kp->tn == 0 && mixed) pb_build_table(kp, z0, z1, coef, PB_TAB_DEG_MID, PB_TAB_TOL_MIXED);
+        if (kp->tn == 0 && mixed) pb_build_table(kp, z0, z1, coef, PB_TAB_DEG, PB_TAB_TOL_MIXED);
         if (kp->tn == 0) coef.clear();
     }
 }
