@@ -13,7 +13,7 @@ that is the accuracy the benchmark reports.
             (realShells home + layer to M = 1, fftBrinkman) in full;
   config 2: 1e5 sources -> 1e5 targets, alpha 10, eps 1e-3, 128^3, P 12;
   config 4: 512 systems of uneven size, autotuned, one batched call;
-  config 5: 2e5 particles x 64 FP32 force sets, 128^3, P 12.
+  config 5: 2e5 particles x 64 FP32 force sets, 128^3, P 10 (autotuned).
 """
 import numpy as np
 import pytest
@@ -179,7 +179,10 @@ def test_config5_multirhs_2e5_by_64(ctx):
     n, R = 200_000, 64
     x, _, _ = pb.uniform_system(pb.config_seed(4), n, 0)
     Fh = pb.Rng(pb.config_seed(4) + (1 << 32)).normal(3 * n * R).reshape(R, n, 3).astype(np.float32)
-    p = pb.Params(alpha=10.0, eps=1e-3, grid=(128, 128, 128), P=12, tol=4e-6, precision="mixed").tune(n, n)
+    # the autotuner's choice at tol 2e-6 (the configuration's 128^3 grid, window P 10)
+    p = pb.Params(alpha=10.0, eps=1e-3, tol=2e-6, precision="mixed")
+    p.autotune(n, n)
+    assert tuple(p.grid) == (128, 128, 128) and p.P == 10
     X = dev(x)
     say("config 5 multi-RHS call")
     U = ctx.velocity_multirhs(p, X, dev(Fh, np.float32), X)
