@@ -319,11 +319,12 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
     if (do_k && (st = make_kgrid(c, p, &g))) return st; // (validated before anything is enqueued)
     if ((st = pb_bin_all(c, s, cells, box, ns, xn, f, nt, y))) return st;
     pb_mark(c, s, 1);
-    // Both halves: the pair kernel on a stream of its own, concurrently with
-    // the k-space pipeline (each adds to a zeroed u with atomics; the pair
-    // kernel's latency-bound waves and the k-space kernels' tails and launch
-    // gaps fill each other's idle compute units).  Not while every phase is
-    // timed (the phases would overlap).
+    // Both halves: the pair kernel on a stream of its own, forked from the
+    // k-space pipeline after the spread and concurrent with the rest of it
+    // (each adds to a zeroed u with atomics; the pair kernel's latency-bound
+    // waves and the FFT's and gather's memory-bound ones fill each other's
+    // idle issue slots).  Not while every phase is timed (the phases would
+    // overlap).
     overlap = overlap && do_real && do_k && nt > 0 && c->timing != 1;
     if (overlap) {
         if (!c->pair_stream &&
@@ -334,14 +335,26 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
                 (st = pb_hip_check(c, hipEventCreateWithFlags(&c->pair_event[k], hipEventDisableTiming), "event")))
                 return st;
         pb_zero_u(s, nt, u);
-        if ((st = pb_hip_check(c, hipEventRecord(c->pair_event[0], s), "fork"))) return st;
-        if ((st = pb_hip_check(c, hipStreamWaitEvent(c->pair_stream, c->pair_event[0], 0), "fork"))) return st;
-        // from here every return joins the pair stream back into s, so no
-        // work of this call is left running beside the caller's next call
-        st = pb_run_real(c, c->pair_stream, cells, kp, ntab, tdeg, box, ns, nt, u, true, far32_of(p, ntab));
-        const int jst = pb_hip_check(c, hipEventRecord(c->pair_event[1], c->pair_stream), "join");
-        if (!st && !jst) st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u, 2, p->precision);
-        const int wst = jst ? jst : pb_hip_check(c, hipStreamWaitEvent(s, c->pair_event[1], 0), "join");
+        bool forked = false;
+        int jst = 0;
+        const std::function<int()> fork = [&]() -> int {
+            int q;
+            if ((q = pb_hip_check(c, hipEventRecord(c->pair_event[0], s), "fork"))) return q;
+            if ((q = pb_hip_check(c, hipStreamWaitEvent(c->pair_stream, c->pair_event[0], 0), "fork"))) return q;
+            // from here every return joins the pair stream back into s, so no
+            // work of this call is left running beside the caller's next call
+            forked = true;
+            q = pb_run_real(c, c->pair_stream, cells, kp, ntab, tdeg, box, ns, nt, u, true, far32_of(p, ntab));
+            jst = pb_hip_check(c, hipEventRecord(c->pair_event[1], c->pair_stream), "join");
+            return q ? q : jst;
+        };
+        // forked after the spread: beside the spread (VALU- and LDS-bound
+        // like it) the two only share the compute units, beside the FFT
+        // (memory-bound) and the gather (texture-path-bound) it fills their
+        // idle issue slots
+        st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u, 2, p->precision, &fork, 1);
+        const int wst = !forked ? 0 : jst ? jst : pb_hip_check(c, hipStreamWaitEvent(s, c->pair_event[1], 0), "join");
+        if (!forked && !st) st = pb_fail(c, PB_ERR_INVALID, "real space not launched");
         return st ? st : wst;
     }
     bool acc = false;

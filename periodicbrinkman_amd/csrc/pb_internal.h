@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -397,7 +398,7 @@ int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kp
 // may be adding to u at the same time)
 int pb_run_kspace(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells,
                   const pb_box &box, double a, double d, int ns, int nt, double *u, int accumulate,
-                  int precision);
+                  int precision, const std::function<int()> *fork = nullptr, int fork_at = 0);
 // The k table goes to c->dense_tmp (built on the device on stream s).
 int pb_run_dense(pb_context *c, hipStream_t s, double *dA, int lda, const double *y, int nt, const double *x,
                  int ns, const pb_kparams &kp, const pb_box &box, const int *lo_real, const int *hi_real,

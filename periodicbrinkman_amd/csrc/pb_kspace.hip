@@ -11,6 +11,8 @@
 // wrapped positions are always computed in FP64 with the oracle's formula.
 #include "pb_internal.h"
 
+#include <functional>
+
 #include <climits>
 #include <cmath>
 #include <vector>
@@ -1836,7 +1838,8 @@ int spread_balanced(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_ce
 
 template <typename T>
 int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells,
-                 const pb_box &box, double a, double d, int ns, int nt, double *u, int accumulate)
+                 const pb_box &box, double a, double d, int ns, int nt, double *u, int accumulate,
+                 const std::function<int()> *fork, int fork_at)
 {
     using C = typename cplx<T>::type;
     using C4 = typename cplx<T>::type4;
@@ -1856,7 +1859,13 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
     if ((st = upload_twiddles<T>(c, s))) return st;
     const kmult km = pb_make_kmult(g, box, a, d);
     void *grid4 = c->grid.ptr, *spec = c->spec.ptr;
+    // the caller's fork (the pair kernel on its own stream) at point fork_at:
+    // 0 before the spread, 1 after it, 2 after the forward transform, 3 before
+    // the gather
+    auto fork_here = [&](int at) { return fork && fork_at == at ? (*fork)() : PB_OK; };
+    if ((st = fork_here(0))) return st;
     if ((st = spread_balanced<T>(c, s, g, cells, ns, (C4 *)grid4))) return st;
+    if ((st = fork_here(1))) return st;
     const C *tw = (const C *)c->twid[sizeof(T) == 8 ? 0 : 1].ptr;
     const void *tws[3] = {tw + (g.n[0] - 4), tw + (g.n[1] - 4), tw + (g.n[2] - 4)};
     pb_mark(c, s, 2);
@@ -1864,11 +1873,15 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
     const int np = fft_passes_t<T>(g, passes);
     for (int k = 0; k < np; k++) {
         const int pass = passes[k];
-        if (pass == 2) pb_mark(c, s, 3);
+        if (pass == 2) {
+            if ((st = fork_here(2))) return st;
+            pb_mark(c, s, 3);
+        }
         if (k > 0 && passes[k - 1] == 2) pb_mark(c, s, 4);
         if ((st = fft_pass<T>(s, pass, g, grid4, spec, km, tws))) return pb_fail(c, st, "FFT size");
     }
     pb_mark(c, s, 5);
+    if ((st = fork_here(3))) return st;
     if (nt > 0) launch_interp<T>(s, nt, g, c->tpos.ptr, c->tperm.ptr, (const trip<T> *)grid4, u, accumulate);
     if (c->timing != 2) pb_mark(c, s, 6); // (mode 2: the pair kernel records boundary 6 itself)
     return pb_hip_check(c, hipGetLastError(), "k-space launch");
@@ -1878,11 +1891,11 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
 
 int pb_run_kspace(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells,
                   const pb_box &box, double a, double d, int ns, int nt, double *u, int accumulate,
-                  int precision)
+                  int precision, const std::function<int()> *fork, int fork_at)
 {
     if (precision == PB_PREC_MIXED)
-        return run_kspace_t<float>(c, s, g, cells, box, a, d, ns, nt, u, accumulate);
-    return run_kspace_t<double>(c, s, g, cells, box, a, d, ns, nt, u, accumulate);
+        return run_kspace_t<float>(c, s, g, cells, box, a, d, ns, nt, u, accumulate, fork, fork_at);
+    return run_kspace_t<double>(c, s, g, cells, box, a, d, ns, nt, u, accumulate, fork, fork_at);
 }
 
 pb_kmult pb_make_kmult(const pb_kgrid &g, const pb_box &box, double a, double d)
