@@ -41,15 +41,16 @@ constexpr int kQ = 128;   // ring capacity (power of two)
 struct wave_lds {
     double y[64][3];
     double acc[64][3];
-    float4 stage[64];  // candidates of the current run: position - tile origin (FP32), index bits
+    float4 stage[64];  // candidates of the current run: c = position - tile origin (FP32), |c|^2
+    unsigned stage_j[64]; // their source indices
     uint2 q[2 * kQ];   // [0, kQ): FP64 ring; [kQ, 2 kQ): FP32 far ring (without one: all FP64)
     uint2 qr[kQ];      // r < rcommon, moved out of evaluated FP64 batches
     uint2 trash[64];   // where a lane whose test fails writes (the ring write takes no branch)
 };
 
-__device__ __forceinline__ unsigned lane_prefix(unsigned long long m)
+__device__ __forceinline__ unsigned lane_prefix(unsigned long long m, unsigned base = 0u)
 {
-    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, base));
 }
 
 // Uniform copy of a value every lane holds (kept in SGPRs).
@@ -271,10 +272,14 @@ __device__ __forceinline__ void pair_body(
     const double bmax0 = wave_max(p.x), bmax1 = wave_max(p.y), bmax2 = wave_max(p.z);
     // the candidate test runs in FP32 on positions relative to the tile's
     // lower corner, against rc enlarged by 1e-4 (relative) so that every pair
-    // in range passes; the batches apply the exact cutoff in FP64.  Inactive
-    // lanes are parked far away, so the test never passes for them.
-    const float y0 = active ? (float)(p.x - bmin0) : 1e18f, y1 = active ? (float)(p.y - bmin1) : 1e18f,
-                y2 = active ? (float)(p.z - bmin2) : 1e18f;
+    // in range passes; the batches apply the exact cutoff in FP64.  With c a
+    // candidate and y the lane's target, |y - c|^2 < R^2 is tested as
+    // |c|^2 - 2 c.y < R^2 - |y|^2: three FMAs per candidate (|c|^2 staged with
+    // it), the right side per lane.  Coordinates are below ~2.5 rc, so the
+    // rounding of the expanded form stays near 1e-6 rc^2, far inside the
+    // 1e-4 enlargement.  Inactive lanes get a bound no candidate meets.
+    const float y0 = (float)(p.x - bmin0), y1 = (float)(p.y - bmin1), y2 = (float)(p.z - bmin2);
+    const float ty0 = -2.0f * y0, ty1 = -2.0f * y1, ty2 = -2.0f * y2, yy = y0 * y0 + y1 * y1 + y2 * y2;
     __builtin_amdgcn_wave_barrier();
 
     const int nc0 = cells.nc[0], nc1 = cells.nc[1], nc2 = cells.nc[2];
@@ -286,6 +291,7 @@ __device__ __forceinline__ void pair_body(
     const int ncols = (uhi0 - ulo0 + 1) * ny;
     const double rc2 = cells.rc2;
     const float rc2f = (float)(rc2 * (1.0 + 1e-4)), rfar2f = (float)rfar2;
+    const float qin = active ? rc2f - yy : -1e30f, qfar = rfar2f - yy; // test bounds on |c|^2 - 2 c.y
 
     // FP64 ring index mask: kQ entries beside the far ring, else 2 kQ (the
     // ring is then checked every two candidates: at most 63 + 128 pending)
@@ -324,16 +330,16 @@ __device__ __forceinline__ void pair_body(
     // append the lanes of one tested candidate that pass the FP32 test to the
     // rings (with the FP32 far field: r >= rfar by the FP32 distance to the
     // far ring; the FP32 decision near rfar only picks the precision)
-    auto append = [&](const unsigned jk, const float r2, const unsigned code) {
-        const bool in = r2 < rc2f;
+    auto append = [&](const unsigned jk, const float q, const unsigned code) {
+        const bool in = q < qin;
         const unsigned long long m = __ballot(in);
         if (!MIXED) { // (the caller flushes)
-            const unsigned slot = (unsigned)(qh + qn) + lane_prefix(m);
+            const unsigned slot = lane_prefix(m, (unsigned)(qh + qn));
             uint2 *dst = in ? &W.q[slot & RM] : &W.trash[lane];
             *dst = make_uint2(jk, code);
             qn += __popcll(m);
         } else {
-            const bool far = in && r2 >= rfar2f;
+            const bool far = in && q >= qfar;
             const unsigned long long mf = __ballot(far);
             const unsigned pm = lane_prefix(m), pf = lane_prefix(mf);
             if (in) {
@@ -408,26 +414,27 @@ __device__ __forceinline__ void pair_body(
                     nread += nk;
                     ntest += nkeep;
                     __builtin_amdgcn_wave_barrier();
-                    if (keep)
-                        W.stage[lane_prefix(km)] = make_float4((float)(sx - bmin0), (float)(sy - bmin1),
-                                                               (float)(sz - bmin2), __int_as_float(j + lane));
+                    if (keep) {
+                        const float c0 = (float)(sx - bmin0), c1 = (float)(sy - bmin1), c2 = (float)(sz - bmin2);
+                        const unsigned slot = lane_prefix(km);
+                        W.stage[slot] = make_float4(c0, c1, c2, c0 * c0 + c1 * c1 + c2 * c2);
+                        W.stage_j[slot] = (unsigned)(j + lane);
+                    }
                     __builtin_amdgcn_wave_barrier();
                     // two candidates per step: independent dependency chains
                     int k = 0;
                     for (; k + 1 < nkeep; k += 2) {
                         const float4 sa = W.stage[k], sb = W.stage[k + 1]; // broadcast reads
-                        const float a0 = y0 - sa.x, a1 = y1 - sa.y, a2 = y2 - sa.z;
-                        const float b0 = y0 - sb.x, b1 = y1 - sb.y, b2 = y2 - sb.z;
-                        const float ra2 = a0 * a0 + a1 * a1 + a2 * a2;
-                        const float rb2 = b0 * b0 + b1 * b1 + b2 * b2;
-                        append((unsigned)__float_as_int(sa.w), ra2, code);
-                        append((unsigned)__float_as_int(sb.w), rb2, code);
+                        const uint2 jab = *reinterpret_cast<const uint2 *>(&W.stage_j[k]);
+                        const float qa = fmaf(ty0, sa.x, fmaf(ty1, sa.y, fmaf(ty2, sa.z, sa.w)));
+                        const float qb = fmaf(ty0, sb.x, fmaf(ty1, sb.y, fmaf(ty2, sb.z, sb.w)));
+                        append(jab.x, qa, code);
+                        append(jab.y, qb, code);
                         if (!MIXED && qn >= 64) flush(false);
                     }
                     if (k < nkeep) {
                         const float4 sa = W.stage[k];
-                        const float a0 = y0 - sa.x, a1 = y1 - sa.y, a2 = y2 - sa.z;
-                        append((unsigned)__float_as_int(sa.w), a0 * a0 + a1 * a1 + a2 * a2, code);
+                        append(W.stage_j[k], fmaf(ty0, sa.x, fmaf(ty1, sa.y, fmaf(ty2, sa.z, sa.w))), code);
                         if (!MIXED && qn >= 64) flush(false);
                     }
                     __builtin_amdgcn_wave_barrier();
