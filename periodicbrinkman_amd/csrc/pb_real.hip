@@ -48,6 +48,14 @@ struct wave_lds {
     uint2 trash[64];   // where a lane whose test fails writes (the ring write takes no branch)
 };
 
+// Element j of a source array through a 32-bit byte offset on the array's
+// (uniform) base: a scalar-base load, no 64-bit address arithmetic per lane
+// (source counts stay far below 2^27).
+__device__ __forceinline__ double4 src_at(const double4 *__restrict__ a, unsigned j)
+{
+    return *reinterpret_cast<const double4 *>(reinterpret_cast<const char *>(a) + (j << 5));
+}
+
 __device__ __forceinline__ unsigned lane_prefix(unsigned long long m, unsigned base = 0u)
 {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, base));
@@ -134,8 +142,8 @@ __device__ __forceinline__ void batch64(wave_lds &W, const int lane, const int c
         e = W.q[(head + lane) & RM];
         double s0, s1, s2;
         decode(e, box, t, s0, s1, s2);
-        const double4 xs = spos[e.x];
-        fs = sfrc[e.x];
+        const double4 xs = src_at(spos, e.x);
+        fs = src_at(sfrc, e.x);
         d0 = W.y[t][0] - (xs.x + s0);
         d1 = W.y[t][1] - (xs.y + s1);
         d2 = W.y[t][2] - (xs.z + s2);
@@ -173,8 +181,8 @@ __device__ __forceinline__ void batch_rare(wave_lds &W, const int lane, const in
         int t;
         double s0, s1, s2;
         decode(e, box, t, s0, s1, s2);
-        const double4 xs = spos[e.x];
-        const double4 fs = sfrc[e.x];
+        const double4 xs = src_at(spos, e.x);
+        const double4 fs = src_at(sfrc, e.x);
         const double d0 = W.y[t][0] - (xs.x + s0);
         const double d1 = W.y[t][1] - (xs.y + s1);
         const double d2 = W.y[t][2] - (xs.z + s2);
@@ -200,7 +208,7 @@ __device__ __forceinline__ void batch32(wave_lds &W, const int lane, const int c
         e = W.q[kQ + ((head + lane) & (kQ - 1))];
         double s0, s1, s2;
         decode(e, box, t, s0, s1, s2);
-        const double4 xs = spos[e.x];
+        const double4 xs = src_at(spos, e.x);
         dd0 = W.y[t][0] - (xs.x + s0);
         dd1 = W.y[t][1] - (xs.y + s1);
         dd2 = W.y[t][2] - (xs.z + s2);
@@ -208,7 +216,7 @@ __device__ __forceinline__ void batch32(wave_lds &W, const int lane, const int c
     }
     nin += __popcll(__ballot(on));
     if (on) {
-        const double4 fs = sfrc[e.x];
+        const double4 fs = src_at(sfrc, e.x);
         const float d0 = (float)dd0, d1 = (float)dd1, d2 = (float)dd2;
         const float f0 = (float)fs.x, f1 = (float)fs.y, f2 = (float)fs.z;
         const float r = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
