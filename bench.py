@@ -505,6 +505,12 @@ def _entry(name, kernels, ms, counters, bound, algorithmic=None, note=None):
     busy = [x["mfma_busy_frac"] for x in c if x.get("mfma_busy_frac")]
     if bound == "mfma" and busy:
         e["mfma_busy_frac"] = busy[0]
+    # busy fractions of the units a launch can saturate (counter runs, one
+    # kernel at a time; the largest over `kernels`): the vector ALU issue,
+    # and the texture address (TA) and data (TD) units of the vector memory
+    # path, through which every global load passes
+    if all("td_busy_frac" in x for x in c):
+        e["unit_busy"] = {u: max(x[u + "_busy_frac"] for x in c) for u in ("valu", "ta", "td")}
     if algorithmic:
         work, unit, peak = algorithmic
         e["algorithmic"] = {"per_launch": work, "rate": work / t / (1e12 if unit == "TFLOP/s" else 1e9),
@@ -550,7 +556,8 @@ def headline_kernel_rooflines(ph, pairs, prec="float", pair_kernel="k_pair_table
                "algorithmic: 2 transforms x 2 (read + write) x 3 fields x n^2 (n/2 + 1) x 8 B"),
         _entry("gather", ["k_interp<%s, 12>" % prec], ph["gather"], cnt, "valu-fp32",
                (gather_flop, "TFLOP/s", PEAK_FP32),
-               "bound in practice by the L1 return of 12-byte window points (P^3 per target)"),
+               "bound in practice by the vector memory path: the texture address and data units are busy "
+               "most of the kernel's cycles (unit_busy) for the 12-byte loads of the P^3 window points per target"),
         _entry("binning", ["k_fill_int2", "k_bin_count2", "k_bin_scan2", "k_bin_scatter2"], ph["binning"], cnt,
                "hbm"),
     ]
@@ -565,8 +572,15 @@ def leg_kernel_rooflines(workload, spec):
     out = []
     for label, kernels, bound, alg in spec:
         ms = sum(cnt[k]["trace_avg_us"] for k in kernels if k in cnt) / 1e3
-        e = _entry(label, kernels, ms, cnt, bound, alg, "duration: rocprofv3 trace average")
+        e = _entry(label, kernels, ms, cnt, bound, alg,
+                   "duration: rocprofv3 trace average (launches beside other kernels included); at_min_duration: "
+                   "the same work over the trace minimum, the kernel with the GPU to itself")
         if e:
+            ms_min = sum(cnt[k]["trace_min_us"] for k in kernels if k in cnt) / 1e3
+            m = _entry(label, kernels, ms_min, cnt, bound, alg)
+            e["at_min_duration"] = {"ms": ms_min, "achieved": m["achieved"], "frac": m["frac"]}
+            if "algorithmic" in m:
+                e["at_min_duration"]["algorithmic_frac"] = m["algorithmic"]["frac"]
             out.append(e)
     return out
 
