@@ -244,3 +244,39 @@ def test_phase_timing_modes(ctx):
     assert pair["pair"] > 0 and all(v == 0 for k, v in pair.items() if k != "pair"), pair
     assert real["binning"] > 0 and real["pair"] > 0, real
     assert all(real[k] == 0 for k in ("spread", "fft_forward", "multiply", "fft_inverse", "gather")), real
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_pair_count_exact_at_the_cutoff(ctx, precision):
+    """Every (target, source image) pair with r < rc is evaluated and no other:
+    the kernel's FP32 candidate test (enlarged bound) must pass all pairs the
+    exact FP64 cutoff keeps.  Targets sit at rc (1 -+ 1e-9 .. 1e-5) from a
+    source, where a dropped pair would change u only at the truncation level;
+    the count is compared with a brute-force FP64 count over all images."""
+    import torch
+    L = (1.0, 1.0, 1.0)
+    rng = np.random.default_rng(5)
+    ns = 1500
+    x = rng.random((ns, 3))
+    f = rng.standard_normal((ns, 3))
+    p = pb.Params(L=L, alpha=10.0, eps=1e-3, grid=(32, 32, 32), P=12, tol=1e-7,
+                  precision=precision).tune(ns, 600)
+    rc = p.rc
+    deltas = np.array([1e-9, 1e-7, 1e-5, -1e-9, -1e-7, -1e-5])
+    k = rng.integers(0, ns, 600)
+    u = rng.standard_normal((600, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    y = (x[k] + rc * (1 - deltas[np.arange(600) % 6])[:, None] * u) % 1.0
+    ctx.velocity_real(p, dev(x), dev(f), dev(y))
+    torch.cuda.synchronize()
+    got = sum(ctx.last_pair_counts())
+    # brute force on the wrapped positions, all 27 images (rc < L / 2)
+    assert rc < 0.5
+    xw, yw = x % 1.0, y % 1.0
+    want = 0
+    for s0 in (-1, 0, 1):
+        for s1 in (-1, 0, 1):
+            for s2 in (-1, 0, 1):
+                d = yw[:, None, :] - (xw[None, :, :] + np.array([s0, s1, s2], dtype=np.float64))
+                want += int(((d * d).sum(-1) < rc * rc).sum())
+    assert got == want
