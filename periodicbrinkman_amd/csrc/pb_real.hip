@@ -11,14 +11,15 @@
 // candidate sources: the (x, y) cell columns within rc of B, each trimmed in
 // z to sqrt(rc^2 - gap^2), which are contiguous runs of the z-major sorted
 // sources.  The columns are dealt round-robin to `nsplit` wavefronts, which
-// meet in u through FP64 atomics.  The lanes of a wavefront set up its columns
-// in parallel (culling, image shifts, first run of sources); the wavefront
-// then walks the runs one source at a time: the source is a uniform scalar
-// load and is tested against the 64 lane targets.
+// meet in u through FP64 atomics.  The lanes of a wavefront set up 64 of its
+// columns in parallel (culling, image shifts, first run of sources); the
+// candidates of the lanes' current runs, numbered in lane order, are loaded
+// 64 at a time (one per lane, a round drawing on as many columns as it takes
+// to fill), kept when within rc of the tile box, and staged in LDS.
 //
-// Each candidate is tested against the 64 lane targets in FP32 (positions
-// relative to the tile's lower corner, rc enlarged by 1e-4 relative, so every
-// pair in range passes); the passing (target, source, image) triples are
+// Each staged candidate is broadcast to the 64 lane targets and tested in
+// FP32 (positions relative to the tile's lower corner, rc enlarged by 1e-4
+// relative, so every pair in range passes); the passing (target, source, image) triples are
 // compacted with a ballot into an LDS ring, one LDS write per tested source;
 // each time 64 are pending every lane evaluates one, so the evaluation runs
 // with full lanes however sparse the test is.  A batch applies the exact
@@ -41,11 +42,15 @@ constexpr int kQ = 128;   // ring capacity (power of two)
 struct wave_lds {
     double y[64][3];
     double acc[64][3];
-    float4 stage[64];  // candidates of the current run: c = position - tile origin (FP32), |c|^2
-    unsigned stage_j[64]; // their source indices
+    float4 stage[64];  // staged candidates: c = position - tile origin (FP32), |c|^2
+    uint2 stage_jc[64]; // their source indices and image codes ((l0+4) | (l1+4) << 3 | (l2+4) << 6) << 6
     uint2 q[2 * kQ];   // [0, kQ): FP64 ring; [kQ, 2 kQ): FP32 far ring (without one: all FP64)
     uint2 qr[kQ];      // r < rcommon, moved out of evaluated FP64 batches
-    uint2 trash[64];   // where a lane whose test fails writes (the ring write takes no branch)
+    union {
+        int2 col_ej[64];  // per column lane: first candidate of its run in the round, first source of the run
+        uint2 trash[64];  // (while testing) where a lane whose test fails writes: the ring write takes no branch
+    };
+    unsigned col_code[64]; // per column lane: image code of its run
 };
 
 // Element j of a source array through a 32-bit byte offset on the array's
@@ -54,6 +59,19 @@ struct wave_lds {
 __device__ __forceinline__ double4 src_at(const double4 *__restrict__ a, unsigned j)
 {
     return *reinterpret_cast<const double4 *>(reinterpret_cast<const char *>(a) + (j << 5));
+}
+
+// Inclusive prefix sum over the 64 lanes by DPP (shifts within rows of 16,
+// then the row totals broadcast to the rows above): no LDS round trips.
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false); // row_bcast:15 into rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false); // row_bcast:31 into rows 2, 3
+    return v;
 }
 
 __device__ __forceinline__ unsigned lane_prefix(unsigned long long m, unsigned base = 0u)
@@ -68,11 +86,6 @@ __device__ __forceinline__ double uniform(double v)
     const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
     const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-__device__ __forceinline__ int lane_read(int v, int i)
-{
-    return __builtin_amdgcn_readlane(v, i);
 }
 
 __device__ __forceinline__ double wave_min(double v)
@@ -363,10 +376,14 @@ __device__ __forceinline__ void pair_body(
     };
 
     // this wavefront's columns are split, split + nsplit, ...; the lanes set
-    // up 64 of them at a time
+    // up 64 of them at a time, one column per lane.  A column's z range runs
+    // through one or more periodic images, each a run of consecutive sources.
+    // The candidates of the lanes' current runs are numbered in lane order and
+    // staged 64 at a time, so a staging round draws on many columns (the runs
+    // are short: a few sources each at the benchmark density).
     for (int cb = split; cb < ncols; cb += 64 * nsplit) {
         const int colv = cb + nsplit * lane;
-        int zs = 0, ze = -1, l0 = 0, l1 = 0, colbase = 0, l2 = 0, zend = -1, ja = 0, jea = 0;
+        int ze = -1, l0 = 0, l1 = 0, colbase = 0, l2 = 0, zend = -1, j = 0, je = 0;
         bool ok = colv < ncols;
         if (ok) {
             const int u0 = ulo0 + colv / ny, u1 = ulo1 + colv % ny;
@@ -376,85 +393,116 @@ __device__ __forceinline__ void pair_body(
             ok = g2 < rcx2;
             if (ok) {
                 const double rem = sqrt(rcx2 - g2);
-                zs = (int)floor((bmin2 - rem) * cells.inv_h[2]);
+                const int zs = (int)floor((bmin2 - rem) * cells.inv_h[2]);
                 ze = (int)floor((bmax2 + rem) * cells.inv_h[2]);
                 l0 = floordiv(u0, nc0);
                 l1 = floordiv(u1, nc1);
                 colbase = ((u0 - l0 * nc0) * nc1 + (u1 - l1 * nc1)) * nc2;
                 l2 = floordiv(zs, nc2);
                 zend = min(ze, (l2 + 1) * nc2 - 1);
-                ja = cstart[colbase + zs - l2 * nc2];
-                jea = cstart[colbase + zend - l2 * nc2 + 1];
+                j = cstart[colbase + zs - l2 * nc2];
+                je = cstart[colbase + zend - l2 * nc2 + 1];
             }
         }
-        unsigned long long cm = __ballot(ok);
-        while (cm) {
-            const int i = __builtin_ctzll(cm);
-            cm &= cm - 1;
-            // lane i's column, uniform from here
-            const int cl0 = lane_read(l0, i), cl1 = lane_read(l1, i), cze = lane_read(ze, i);
-            const int cbase = lane_read(colbase, i);
-            int cl2 = lane_read(l2, i), czend = lane_read(zend, i);
-            int j = lane_read(ja, i), je = lane_read(jea, i);
-            const double sh0 = cl0 * box.L[0], sh1 = cl1 * box.L[1];
-            const unsigned code01 = (unsigned)lane | ((unsigned)(cl0 + 4) << 6) | ((unsigned)(cl1 + 4) << 9);
-            for (;;) {
-                const double sh2 = cl2 * box.L[2];
-                const unsigned code = code01 | ((unsigned)(cl2 + 4) << 12);
-                for (; j < je; j += 64) {
-                    // stage the next 64 sources of the run at this image, keeping
-                    // those within rc of the tile box, compacted
-                    const int nk = min(64, je - j);
-                    bool keep = false;
-                    double sx = 0, sy = 0, sz = 0;
-                    if (lane < nk) {
-                        const double4 xs = spos[j + lane];
-                        sx = xs.x + sh0;
-                        sy = xs.y + sh1;
-                        sz = xs.z + sh2;
-                        const double g0 = fmax(0.0, fmax(bmin0 - sx, sx - bmax0));
-                        const double g1 = fmax(0.0, fmax(bmin1 - sy, sy - bmax1));
-                        const double g2 = fmax(0.0, fmax(bmin2 - sz, sz - bmax2));
-                        keep = g0 * g0 + g1 * g1 + g2 * g2 < rcx2;
-                    }
-                    const unsigned long long km = __ballot(keep);
-                    const int nkeep = __popcll(km);
-                    nread += nk;
-                    ntest += nkeep;
+        const unsigned code01 = ((unsigned)(l0 + 4) | ((unsigned)(l1 + 4) << 3)) << 6;
+        for (;;) {
+            // lanes whose run is used up move to the next image's run of their column
+            bool adv = ok && j >= je && zend < ze;
+            while (__ballot(adv)) {
+                if (adv) {
+                    const int zsn = zend + 1;
+                    l2 = floordiv(zsn, nc2);
+                    zend = min(ze, (l2 + 1) * nc2 - 1);
+                    j = cstart[colbase + zsn - l2 * nc2];
+                    je = cstart[colbase + zend - l2 * nc2 + 1];
+                }
+                adv = ok && j >= je && zend < ze;
+            }
+            const int len = ok && je > j ? je - j : 0;
+            const unsigned long long nz = __ballot(len > 0);
+            if (!nz) break;
+            const unsigned rcode = code01 | ((unsigned)(l2 + 4) << 12);
+            // the first column with candidates (uniform): when it alone holds
+            // the round's candidates (long runs: dense sources) they are its
+            // next nk sources; else the candidates are numbered over the
+            // lanes' runs (prefix sum) and each lane finds its column
+            const int c0 = __builtin_ctzll(nz);
+            const int len0 = __builtin_amdgcn_readlane(len, c0);
+            int nk = 64, excl = 0;
+            bool one = len0 >= 64;
+            if (!one) {
+                const int incl = wave_incl_scan(len);
+                nk = min(64, __builtin_amdgcn_readlane(incl, 63));
+                excl = incl - len;
+                one = len0 >= nk;
+                if (!one) {
                     __builtin_amdgcn_wave_barrier();
-                    if (keep) {
-                        const float c0 = (float)(sx - bmin0), c1 = (float)(sy - bmin1), c2 = (float)(sz - bmin2);
-                        const unsigned slot = lane_prefix(km);
-                        W.stage[slot] = make_float4(c0, c1, c2, c0 * c0 + c1 * c1 + c2 * c2);
-                        W.stage_j[slot] = (unsigned)(j + lane);
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                    // two candidates per step: independent dependency chains
-                    int k = 0;
-                    for (; k + 1 < nkeep; k += 2) {
-                        const float4 sa = W.stage[k], sb = W.stage[k + 1]; // broadcast reads
-                        const uint2 jab = *reinterpret_cast<const uint2 *>(&W.stage_j[k]);
-                        const float qa = fmaf(ty0, sa.x, fmaf(ty1, sa.y, fmaf(ty2, sa.z, sa.w)));
-                        const float qb = fmaf(ty0, sb.x, fmaf(ty1, sb.y, fmaf(ty2, sb.z, sb.w)));
-                        append(jab.x, qa, code);
-                        append(jab.y, qb, code);
-                        if (!MIXED && qn >= 64) flush(false);
-                    }
-                    if (k < nkeep) {
-                        const float4 sa = W.stage[k];
-                        append(W.stage_j[k], fmaf(ty0, sa.x, fmaf(ty1, sa.y, fmaf(ty2, sa.z, sa.w))), code);
-                        if (!MIXED && qn >= 64) flush(false);
-                    }
+                    W.col_ej[lane] = make_int2(excl, j);
+                    W.col_code[lane] = rcode;
                     __builtin_amdgcn_wave_barrier();
                 }
-                if (czend >= cze) break;
-                // the z range runs into the next image: next run of sources
-                const int zsn = czend + 1;
-                cl2 = floordiv(zsn, nc2);
-                czend = min(cze, (cl2 + 1) * nc2 - 1);
-                j = cstart[cbase + zsn - cl2 * nc2];
-                je = cstart[cbase + czend - cl2 * nc2 + 1];
             }
+            bool keep = false;
+            double sx = 0, sy = 0, sz = 0;
+            int src = 0;
+            unsigned ccode = 0;
+            if (lane < nk) {
+                if (one) {
+                    src = __builtin_amdgcn_readlane(j, c0) + lane;
+                    ccode = (unsigned)__builtin_amdgcn_readlane((int)rcode, c0);
+                } else {
+                    // candidate `lane`: the last column whose first candidate is
+                    // at or before it (columns without candidates share the
+                    // next one's start and are passed over)
+                    int c = 0; // (c + step stays at or below 63)
+#pragma unroll
+                    for (int step = 32; step >= 1; step >>= 1)
+                        if (W.col_ej[c + step].x <= lane) c += step;
+                    const int2 ej = W.col_ej[c];
+                    src = ej.y + (lane - ej.x);
+                    ccode = W.col_code[c];
+                }
+                const double4 xs = spos[src];
+                sx = xs.x + (double)((int)((ccode >> 6) & 7) - 4) * box.L[0];
+                sy = xs.y + (double)((int)((ccode >> 9) & 7) - 4) * box.L[1];
+                sz = xs.z + (double)((int)((ccode >> 12) & 7) - 4) * box.L[2];
+                const double g0 = fmax(0.0, fmax(bmin0 - sx, sx - bmax0));
+                const double g1 = fmax(0.0, fmax(bmin1 - sy, sy - bmax1));
+                const double g2 = fmax(0.0, fmax(bmin2 - sz, sz - bmax2));
+                keep = g0 * g0 + g1 * g1 + g2 * g2 < rcx2;
+            }
+            // the lanes' runs move past the candidates this round takes
+            j += one ? (lane == c0 ? nk : 0) : min(len, max(0, 64 - excl));
+            const unsigned long long km = __ballot(keep);
+            const int nkeep = __popcll(km);
+            nread += nk;
+            ntest += nkeep;
+            __builtin_amdgcn_wave_barrier();
+            if (keep) {
+                const float c0f = (float)(sx - bmin0), c1f = (float)(sy - bmin1), c2f = (float)(sz - bmin2);
+                const unsigned slot = lane_prefix(km);
+                W.stage[slot] = make_float4(c0f, c1f, c2f, c0f * c0f + c1f * c1f + c2f * c2f);
+                W.stage_jc[slot] = make_uint2((unsigned)src, ccode);
+            }
+            __builtin_amdgcn_wave_barrier();
+            // two candidates per step: independent dependency chains
+            int k = 0;
+            for (; k + 1 < nkeep; k += 2) {
+                const float4 sa = W.stage[k], sb = W.stage[k + 1]; // broadcast reads
+                const uint4 jc = *reinterpret_cast<const uint4 *>(&W.stage_jc[k]);
+                const float qa = fmaf(ty0, sa.x, fmaf(ty1, sa.y, fmaf(ty2, sa.z, sa.w)));
+                const float qb = fmaf(ty0, sb.x, fmaf(ty1, sb.y, fmaf(ty2, sb.z, sb.w)));
+                append(jc.x, qa, jc.y | (unsigned)lane);
+                append(jc.z, qb, jc.w | (unsigned)lane);
+                if (!MIXED && qn >= 64) flush(false);
+            }
+            if (k < nkeep) {
+                const float4 sa = W.stage[k];
+                const uint2 jc = W.stage_jc[k];
+                append(jc.x, fmaf(ty0, sa.x, fmaf(ty1, sa.y, fmaf(ty2, sa.z, sa.w))), jc.y | (unsigned)lane);
+                if (!MIXED && qn >= 64) flush(false);
+            }
+            __builtin_amdgcn_wave_barrier();
         }
     }
     flush(true);
