@@ -21,6 +21,8 @@ namespace {
 
 constexpr int kPmax = 32;
 
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
 // Three components packed without padding (the inverse transform's output:
 // the gather reads 12 or 24 bytes per window point instead of 16 or 32).
 template <typename T> struct trip {
@@ -62,16 +64,20 @@ __device__ __forceinline__ int wrapi(int i, int n)
 // time and stages the hits in LDS in a fixed order; each wavefront then takes
 // the staged sources that touch its sub-block in batches.
 //
-// FP32, 16 sources per batch: at staging, the thread that found a source
-// evaluates its window weights at the region's 8 points per axis (24 values,
-// zero outside the window) and records which 4-point halves of the region
-// the window touches; in a batch, lane (i, h) reads source h's weight at
-// offset i of the sub-block on each axis, keeps wx[i] F in registers and forms
-// P[h][4 i + k] = wy[i] wz[k] in LDS; the lane at point (i, j, k) then adds,
-// per source, P[h][4 j + k] times wx F of lane h of its own row, taken by a
-// DPP row broadcast folded into the multiply-add (three per source).  FP64, 32
-// per batch: one lane per source evaluates the sub-block's 4+4+4 weights and
-// stores P[j][k] = wy[j] wz[k] and wx[i] F in LDS.
+// FP32, one force set: at staging, the thread that found a source evaluates
+// its window weights at the region's 8 points per axis (24 values, zero
+// outside the window) and records which 4-point halves of the region the
+// window touches; each wavefront then takes the staged sources that touch its
+// 4 x 4 x 8 column (both z halves) 32 at a time as matrix products on the
+// MFMA units (A = wx F over (x, component), B = wy wz over (y, z), the
+// sources the inner dimension; spread_mfma_col).  FP32 with several force
+// sets (16 sources per batch, weights evaluated per batch): lane (i, h)
+// evaluates source h's weights at offset i of the sub-block, keeps wx[i] F in
+// registers and forms P[h][4 i + k] = wy[i] wz[k] in LDS; the lane at point
+// (i, j, k) then adds, per source, P[h][4 j + k] times wx F of lane h of its
+// own row, taken by a DPP row broadcast folded into the multiply-add.  FP64,
+// 32 per batch: one lane per source evaluates the sub-block's 4+4+4 weights
+// and stores P[j][k] = wy[j] wz[k] and wx[i] F in LDS.
 //
 // Work: a region whose candidates are few is one work item.  Dense regions
 // (clustered sources) are split into chunks of C candidates, each a work item,
@@ -123,15 +129,27 @@ template <> struct spread_stage<float, 1> {
 
 // NS > 1 (FP32 only): NS force sets spread together from one staging of the
 // sources (the many-right-hand-side k space).
-template <typename T, int NS = 1> struct spread_lds : spread_batch_lds<T>, spread_stage<T, NS> {
+// The per-batch arrays of the FP64 and several-force-set forms (the one-set
+// FP32 form reads the staged records directly).
+struct spread_no_batch_lds {};
+template <typename T, int NS> struct spread_batch_for {
+    using type = spread_batch_lds<T>;
+};
+template <> struct spread_batch_for<float, 1> {
+    using type = spread_no_batch_lds;
+};
+
+template <typename T, int NS = 1>
+struct spread_lds : spread_batch_for<T, NS>::type, spread_stage<T, NS> {
     static constexpr int CAP = spread_stage<T, NS>::CAP, HB = spread_cfg<T>::HB;
-    static_assert(CAP >= 256, "a chunk of 256 candidates must fit the staging");
+    static_assert(CAP == 256, "a chunk of 256 candidates fits the staging; staged indices are bytes");
     int run_j[256];         // first source of each candidate run of the chunk
     int run_pre[257];       // exclusive prefix of the run lengths
     unsigned run_code[256]; // image shifts (l0 + 4) | (l1 + 4) << 4 | (l2 + 4) << 8
     int wcount[kSpreadWaves];
-    int ring[kSpreadWaves][128]; // staged indices touching the wavefront's sub-block
-                                 // (< HB pending + 64 appended)
+    // staged indices (< CAP) touching the wavefront's columns: fewer than a
+    // batch pending plus 64 appended
+    unsigned char ring[kSpreadWaves][128];
 };
 
 // Region rid: its first and last grid points per axis and its candidate runs
@@ -273,28 +291,6 @@ __device__ __forceinline__ void spread_fma48(const float (&P)[16][16], const flo
 #endif
 }
 
-// FP32 batch of up to 16 staged records (one force set) on the sub-block at
-// offsets off[a] (0 or 4) of the region.  Lane (i, h) = (lane / 16, lane % 16)
-// reads record h's weights at offset i (wx zero for h >= cnt), keeps wx F and
-// forms P[h][4 i + k] = wy[i] wz[k].
-__device__ __forceinline__ void spread_batch16w(spread_lds<float, 1> &S, int wave, int lane, int head, int cnt,
-                                                const int *off, float (&acc)[1][3])
-{
-    const int i = lane >> 4, h = lane & 15;
-    __builtin_amdgcn_wave_barrier();
-    const int e = S.ring[wave][(head + (h < cnt ? h : 0)) & 127];
-    const float *rec = S.st[e];
-    const float wx = h < cnt ? rec[off[0] + i] : 0.0f;
-    const float wy = rec[8 + off[1] + i];
-    const float4 z = *reinterpret_cast<const float4 *>(rec + 16 + off[2]);
-    *reinterpret_cast<float4 *>(&S.P[wave][h][4 * i]) = make_float4(wy * z.x, wy * z.y, wy * z.z, wy * z.w);
-    __builtin_amdgcn_wave_barrier();
-    float p[16];
-    spread_p16(S.P[wave], lane, p);
-    spread_fma48(S.P[wave], p, lane, wx * rec[24], wx * rec[25], wx * rec[26], acc[0]);
-    __builtin_amdgcn_wave_barrier();
-}
-
 // FP32 batch of up to 16 staged sources, NS force sets, weights evaluated
 // here: lane (i, h) evaluates source h's weight at offset i of the sub-block
 // on each axis (zero outside the window or for h >= cnt), keeps wx[i] F,
@@ -339,9 +335,7 @@ __device__ __forceinline__ void spread_batch(spread_lds<T, NS> &S, int wave, int
                                              const int *sb, const int *off, const pb_kgrid &g, int li, int lj, int lk,
                                              T (&acc)[NS][3])
 {
-    if constexpr (spread_stage<T, NS>::PREW) {
-        spread_batch16w(S, wave, lane, head, cnt, off, acc);
-    } else if constexpr (sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 4) {
         spread_batch16r<NS>(S, wave, lane, head, cnt, sb, g, acc);
     } else {
         static_assert(NS == 1, "FP64 spreads one force set");
@@ -360,11 +354,93 @@ __device__ __forceinline__ void spread_batch(spread_lds<T, NS> &S, int wave, int
     }
 }
 
+// FP32, one force set, both z halves of the wavefront's 4 x 4 x 8 column at
+// once: up to 16 NB staged records as 4 NB steps of two
+// v_mfma_f32_16x16x4_f32 each, one per z half h:
+//   D_h[4 i + c][4 j + k] += sum over the step's four records r of
+//                            A[4 i + c][r] B_h[r][4 j + k],
+//   A = wx[i] F[c] (x offset off0; row c = 3 zero), B_h = wy[j] wz[4 h + k]
+// (y offset off1).  Lane l supplies A[l % 16][l / 16] and B_h[l / 16][l % 16]
+// of record 4 s + l / 16 and holds D_h[4 (l / 16) + c][l % 16] in d[h][c]:
+// the point (i, j, k) = (l / 16, (l / 4) % 4, l % 4) of the sub-block,
+// component c, as in the rest of the spread.  Records past cnt add zero.
+template <int NB>
+__device__ __forceinline__ void spread_mfma_col(spread_lds<float, 1> &S, int wave, int lane, int head, int cnt,
+                                                int off0, int off1, floatx4 (&d)[2])
+{
+    constexpr int NSTEP = 4 * NB;
+    const int g = lane >> 4, hi = (lane >> 2) & 3, lo = lane & 3;
+    __builtin_amdgcn_wave_barrier();
+    int e[NSTEP];
+#pragma unroll
+    for (int s = 0; s < NSTEP; s++) e[s] = S.ring[wave][(head + (4 * s + g < cnt ? 4 * s + g : 0)) & 127];
+    float wx[NSTEP], fc[NSTEP], wy[NSTEP], wz0[NSTEP], wz1[NSTEP];
+#pragma unroll
+    for (int s = 0; s < NSTEP; s++) {
+        const float *rec = S.st[e[s]];
+        wx[s] = rec[off0 + hi];
+        fc[s] = rec[24 + lo]; // lo = 3: the mask word, not used
+        wy[s] = rec[8 + off1 + hi];
+        wz0[s] = rec[16 + lo];
+        wz1[s] = rec[20 + lo];
+    }
+#pragma unroll
+    for (int s = 0; s < NSTEP; s++) {
+        const float a = 4 * s + g < cnt && lo < 3 ? wx[s] * fc[s] : 0.0f;
+        d[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, wy[s] * wz0[s], d[0], 0, 0, 0);
+        d[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, wy[s] * wz1[s], d[1], 0, 0, 0);
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// The FP32 one-force-set form of spread_process: the wavefront takes each
+// staged source whose window touches its 4 x 4 column (either z half) once.
+__device__ __forceinline__ void spread_process_col(spread_lds<float, 1> &S, int wave, int lane, int nst,
+                                                   const region_geo &R, const pb_kgrid &g, float (&acc)[2][1][3])
+{
+    constexpr int NB = 2;
+    const int hx = wave >> 1, hy = wave & 1;
+    if (R.gA[0] + 4 * hx >= g.n[0] || R.gA[1] + 4 * hy >= g.n[1]) return; // uniform per wavefront
+    const unsigned need = (1u << hx) | (1u << (2 + hy));
+    floatx4 d[2];
+#pragma unroll
+    for (int sz = 0; sz < 2; sz++) d[sz] = floatx4{acc[sz][0][0], acc[sz][0][1], acc[sz][0][2], 0.0f};
+    int head = 0, pend = 0;
+    for (int eb = 0; eb < nst; eb += 64) {
+        const int e = eb + lane;
+        bool in = e < nst;
+        if (in) {
+            const unsigned m = __float_as_uint(S.st[e][spread_stage<float, 1>::MASK]);
+            in = (m & need) == need && (m & 0x30u) != 0;
+        }
+        const unsigned long long m = __ballot(in);
+        if (in) {
+            const unsigned pre = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            S.ring[wave][(head + pend + pre) & 127] = e;
+        }
+        pend += __popcll(m);
+        while (pend >= 16 * NB) {
+            spread_mfma_col<NB>(S, wave, lane, head, 16 * NB, 4 * hx, 4 * hy, d);
+            head = (head + 16 * NB) & 127;
+            pend -= 16 * NB;
+        }
+    }
+    if (pend > 16) spread_mfma_col<2>(S, wave, lane, head, pend, 4 * hx, 4 * hy, d);
+    else if (pend > 0) spread_mfma_col<1>(S, wave, lane, head, pend, 4 * hx, 4 * hy, d);
+#pragma unroll
+    for (int sz = 0; sz < 2; sz++) {
+        acc[sz][0][0] = d[sz][0];
+        acc[sz][0][1] = d[sz][1];
+        acc[sz][0][2] = d[sz][2];
+    }
+}
+
 // The wavefront's two sub-blocks (z halves of its 4 x 4 column of the region)
-// gather from the nst staged sources.
+// gather, one after the other, from the nst staged sources (FP64; FP32 with
+// several force sets).
 template <typename T, int NS>
-__device__ __forceinline__ void spread_process(spread_lds<T, NS> &S, int wave, int lane, int nst, const region_geo &R,
-                                               const pb_kgrid &g, T (&acc)[2][NS][3])
+__device__ __forceinline__ void spread_process_sub(spread_lds<T, NS> &S, int wave, int lane, int nst,
+                                                   const region_geo &R, const pb_kgrid &g, T (&acc)[2][NS][3])
 {
     constexpr int HB = spread_cfg<T>::HB;
     const int li = lane >> 4, lj = (lane >> 2) & 3, lk = lane & 3;
@@ -374,19 +450,14 @@ __device__ __forceinline__ void spread_process(spread_lds<T, NS> &S, int wave, i
         const int off[3] = {4 * hx, 4 * hy, 4 * sz};
         const int sb[3] = {R.gA[0] + off[0], R.gA[1] + off[1], R.gA[2] + off[2]};
         if (sb[0] >= g.n[0] || sb[1] >= g.n[1] || sb[2] >= g.n[2]) continue; // uniform per wavefront
-        const unsigned need = (1u << hx) | (1u << (2 + hy)) | (1u << (4 + sz));
         int head = 0, pend = 0;
         for (int eb = 0; eb < nst; eb += 64) {
             const int e = eb + lane;
             bool in = e < nst;
-            if constexpr (spread_stage<T, NS>::PREW) {
-                in = in && (__float_as_uint(S.st[e][spread_stage<T, NS>::MASK]) & need) == need;
-            } else {
 #pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    const int f0 = in ? S.sf0[a][e] : 0;
-                    in = in && f0 > sb[a] - g.P[a] && f0 <= sb[a] + 3;
-                }
+            for (int a = 0; a < 3; a++) {
+                const int f0 = in ? S.sf0[a][e] : 0;
+                in = in && f0 > sb[a] - g.P[a] && f0 <= sb[a] + 3;
             }
             const unsigned long long m = __ballot(in);
             if (in) {
@@ -402,6 +473,18 @@ __device__ __forceinline__ void spread_process(spread_lds<T, NS> &S, int wave, i
         }
         if (pend > 0) spread_batch<T, NS>(S, wave, lane, head, pend, sb, off, g, li, lj, lk, acc[sz]);
     }
+}
+
+// The wavefronts' share of the nst staged sources onto their columns of the
+// region (acc: each wavefront's two 4 x 4 x 4 sub-blocks, z halves).
+template <typename T, int NS>
+__device__ __forceinline__ void spread_process(spread_lds<T, NS> &S, int wave, int lane, int nst, const region_geo &R,
+                                               const pb_kgrid &g, T (&acc)[2][NS][3])
+{
+    if constexpr (spread_stage<T, NS>::PREW)
+        spread_process_col(S, wave, lane, nst, R, g, acc);
+    else
+        spread_process_sub(S, wave, lane, nst, R, g, acc);
 }
 
 // Candidates q0 .. q1 - 1 of region R (numbered in run order) spread onto the
@@ -453,6 +536,7 @@ __device__ __forceinline__ void spread_chunk(spread_lds<T, NS> &S, const region_
             int src = 0, f0[3] = {0, 0, 0};
             unsigned cq = 0;
             double xw[3] = {0, 0, 0};
+            double4 fw = {0, 0, 0, 0};
             if (q < hi) {
                 const int qi = (int)q;
                 int lo2 = 0, hi2 = 255; // last run with run_pre <= q
@@ -465,6 +549,7 @@ __device__ __forceinline__ void spread_chunk(spread_lds<T, NS> &S, const region_
                 }
                 src = S.run_j[lo2] + (qi - S.run_pre[lo2]);
                 cq = S.run_code[lo2];
+                if constexpr (spread_stage<T, NS>::PREW) fw = sfrc[0][src]; // loaded beside the position
                 const double4 x = spos[src];
                 xw[0] = x.x;
                 xw[1] = x.y;
@@ -497,28 +582,28 @@ __device__ __forceinline__ void spread_chunk(spread_lds<T, NS> &S, const region_
                               (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
                 if constexpr (spread_stage<T, NS>::PREW) {
                     // the weights at the region's 8 points per axis, once per source
+                    // (branch-free: each weight evaluated, zero outside the window)
                     float *rec = S.st[e];
                     unsigned mask = 0;
+                    const float c2 = (float)(-1.4426950408889634 / g.eta); // e^{-t} = 2^{-t log2 e}
 #pragma unroll
                     for (int a = 0; a < 3; a++) {
                         const int l = (int)((cq >> (4 * a)) & 15) - 4;
+                        double dx = (double)(R.gA[a] - l * g.n[a]) * g.h[a] - xw[a];
+                        const int m0 = R.gA[a] - f0[a];
                         float w8[8];
 #pragma unroll
                         for (int p = 0; p < 8; p++) {
-                            const int gq = R.gA[a] + p, mm = gq - f0[a];
-                            float v = 0.0f;
-                            if (mm >= 0 && mm < g.P[a]) {
-                                const double dx = (double)(gq - l * g.n[a]) * g.h[a] - xw[a];
-                                v = __expf(-(float)(dx * dx) * (float)(1.0 / g.eta));
-                            }
-                            w8[p] = v;
+                            const float v = __builtin_amdgcn_exp2f((float)(dx * dx) * c2);
+                            w8[p] = (unsigned)(m0 + p) < (unsigned)g.P[a] ? v : 0.0f;
+                            dx += g.h[a];
                         }
                         *reinterpret_cast<float4 *>(rec + 8 * a) = make_float4(w8[0], w8[1], w8[2], w8[3]);
                         *reinterpret_cast<float4 *>(rec + 8 * a + 4) = make_float4(w8[4], w8[5], w8[6], w8[7]);
                         mask |= (f0[a] > R.gA[a] - g.P[a] && f0[a] <= R.gA[a] + 3 ? 1u : 0u) << (2 * a);
                         mask |= (f0[a] > R.gA[a] + 4 - g.P[a] && f0[a] <= R.gA[a] + 7 ? 1u : 0u) << (2 * a + 1);
                     }
-                    const double4 f = sfrc[0][src];
+                    const double4 f = fw;
                     rec[24] = (float)(f.x * g.scale);
                     rec[25] = (float)(f.y * g.scale);
                     rec[26] = (float)(f.z * g.scale);
