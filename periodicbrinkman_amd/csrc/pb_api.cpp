@@ -317,7 +317,9 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
     if ((st = make_cells(c, p, &cells))) return st;
     pb_kgrid g;
     if (do_k && (st = make_kgrid(c, p, &g))) return st; // (validated before anything is enqueued)
-    if ((st = pb_bin_all(c, s, cells, box, ns, xn, f, nt, y))) return st;
+    overlap = overlap && do_real && do_k && nt > 0 && c->timing != 1;
+    // (the overlapped form's output zeroed by the binning's first launch)
+    if ((st = pb_bin_all(c, s, cells, box, ns, xn, f, nt, y, overlap ? u : nullptr))) return st;
     pb_mark(c, s, 1);
     // Both halves: the pair kernel on a stream of its own, forked from the
     // k-space pipeline after the spread and concurrent with the rest of it
@@ -325,7 +327,6 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
     // waves and the FFT's and gather's memory-bound ones fill each other's
     // idle issue slots).  Not while every phase is timed (the phases would
     // overlap).
-    overlap = overlap && do_real && do_k && nt > 0 && c->timing != 1;
     if (overlap) {
         if (!c->pair_stream &&
             (st = pb_hip_check(c, hipStreamCreateWithFlags(&c->pair_stream, hipStreamNonBlocking), "stream")))
@@ -334,7 +335,6 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
             if (!c->pair_event[k] &&
                 (st = pb_hip_check(c, hipEventCreateWithFlags(&c->pair_event[k], hipEventDisableTiming), "event")))
                 return st;
-        pb_zero_u(s, nt, u);
         bool forked = false;
         int jst = 0;
         const std::function<int()> fork = [&]() -> int {

@@ -192,19 +192,24 @@ __global__ void k_write_tiles(int nq, const int *__restrict__ tstart, const int 
 }
 
 // ---- fused binning of sources and targets (small key spaces) -----------------
-// Four launches instead of seventeen: one fill, one count and one scatter
-// over both point sets, and one scan launch of two workgroups (one per count
-// array; the target one also writes the tiles).  Used when ncell + nkey_t <=
-// kFusedKeys.
+// Five launches instead of seventeen: one fill (which can also zero the
+// output), one count and one scatter over both point sets, and the two
+// launches of the chunked scans (the target one also writes the tiles).  Used
+// when ncell + nkey_t <= kFusedKeys.
 constexpr int kFusedKeys = 1 << 18;
 
-__global__ void k_fill_int2(int n0, int *__restrict__ a0, int n1, int *__restrict__ a1)
+// Zeroes both count arrays and, with u, the nu doubles of the output (one
+// launch instead of two).
+__global__ void k_fill_int2(int n0, int *__restrict__ a0, int n1, int *__restrict__ a1, long long nu,
+                            double *__restrict__ u)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n0)
         a0[i] = 0;
     else if (i - n0 < n1)
         a1[i - n0] = 0;
+    else if (i - n0 - n1 < nu)
+        u[i - n0 - n1] = 0.0;
 }
 
 __device__ __forceinline__ int cell_key(const double *__restrict__ pos, size_t i, const pb_box &box,
@@ -385,18 +390,151 @@ __device__ void scan_counts(const int *__restrict__ count, int n, int *__restric
     }
 }
 
-__global__ __launch_bounds__(1024) void k_bin_scan2(const int *__restrict__ scount, int ncell,
-                                                    int *__restrict__ sstart, const int *__restrict__ tcount,
-                                                    int nkey_t, int *__restrict__ tstart,
-                                                    int2 *__restrict__ tiles, int *__restrict__ ntiles)
+
+
+// ---- chunked scans of the fused binning ------------------------------------
+// The two count arrays (sources: cell counts; targets: brick-Morton key
+// counts, whose quarter bricks also become tiles) are scanned in two launches
+// of one workgroup per chunk of kChunk keys: the chunk totals of counts and
+// tiles, then each chunk's scan offset by the totals of the chunks before it
+// in the same array.  Many workgroups in flight instead of one per array
+// working through the array in rounds.
+constexpr int kChunk = 4096; // 256 threads x 16 keys; a multiple of a brick's 64 keys
+
+__device__ __forceinline__ void load16(const int *__restrict__ count, int n, int i0, int (&v)[16])
 {
-    __shared__ int wsum[16];
-    if (blockIdx.x == 0) // the two scans run side by side, one workgroup each
-        scan_counts(scount, ncell, sstart, wsum, nullptr, nullptr);
-    else
-        scan_counts(tcount, nkey_t, tstart, wsum, tiles, ntiles);
+    if (i0 + 16 <= n) {
+        const int4 *c4 = reinterpret_cast<const int4 *>(count + i0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int4 t = c4[q];
+            v[4 * q] = t.x;
+            v[4 * q + 1] = t.y;
+            v[4 * q + 2] = t.z;
+            v[4 * q + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; q++) v[q] = i0 + q < n ? count[i0 + q] : 0;
+    }
 }
 
+// Exclusive scan across the 256 threads of the workgroup; total to every thread.
+__device__ __forceinline__ int scan256(int v, int *wsum, int &total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int sk = wsum[k];
+        pre += k < w ? sk : 0;
+        tot += sk;
+    }
+    __syncthreads(); // wsum is rewritten by the next call
+    total = tot;
+    return pre + x - v;
+}
+
+// Tiles of the thread's quarter brick (16 target keys summing to c; the 4
+// quarters of a brick are 4 consecutive lanes).
+__device__ __forceinline__ int thread_tiles(int c, int &span)
+{
+    const int h = c + __shfl_xor(c, 1, 64);
+    const int B = h + __shfl_xor(h, 2, 64);
+    return quarter_tiles((int)threadIdx.x & 3, c, h, B, span);
+}
+
+// blocks [0, nchs): source chunks; [nchs, nchs + ncht): target chunks.
+__global__ __launch_bounds__(256) void k_bin_sums2(const int *__restrict__ scount, int ncell,
+                                                   const int *__restrict__ tcount, int nkey_t, int nchs,
+                                                   int2 *__restrict__ csum)
+{
+    __shared__ int wsum[4];
+    const bool tgt = (int)blockIdx.x >= nchs; // uniform per workgroup
+    const int ch = tgt ? (int)blockIdx.x - nchs : (int)blockIdx.x;
+    int v[16];
+    load16(tgt ? tcount : scount, tgt ? nkey_t : ncell, ch * kChunk + 16 * (int)threadIdx.x, v);
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) sum += v[q];
+    int span, nq = 0;
+    if (tgt) nq = thread_tiles(sum, span);
+    int tot, ttot;
+    scan256(sum, wsum, tot);
+    scan256(nq, wsum, ttot);
+    if (threadIdx.x == 0) csum[blockIdx.x] = make_int2(tot, ttot);
+}
+
+__global__ __launch_bounds__(256) void k_bin_apply2(const int *__restrict__ scount, int ncell, int *__restrict__ sstart,
+                                                    const int *__restrict__ tcount, int nkey_t, int *__restrict__ tstart,
+                                                    int nchs, int ncht, const int2 *__restrict__ csum,
+                                                    int2 *__restrict__ tiles, int *__restrict__ ntiles)
+{
+    __shared__ int wsum[4];
+    const bool tgt = (int)blockIdx.x >= nchs; // uniform per workgroup
+    const int ch = tgt ? (int)blockIdx.x - nchs : (int)blockIdx.x;
+    const int n = tgt ? nkey_t : ncell, nch = tgt ? ncht : nchs, cbase = tgt ? nchs : 0;
+    int *__restrict__ start = tgt ? tstart : sstart;
+    // totals of the chunks before this one
+    int pc = 0, pt = 0;
+    for (int k = (int)threadIdx.x; k < ch; k += 256) {
+        const int2 t = csum[cbase + k];
+        pc += t.x;
+        pt += t.y;
+    }
+    int carry, tcarry;
+    scan256(pc, wsum, carry);
+    scan256(pt, wsum, tcarry);
+    const int i0 = ch * kChunk + 16 * (int)threadIdx.x;
+    int v[16];
+    load16(tgt ? tcount : scount, n, i0, v);
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) sum += v[q];
+    int tot;
+    const int first = scan256(sum, wsum, tot) + carry;
+    int run = first;
+    if (i0 + 16 <= n) {
+        int4 *o4 = reinterpret_cast<int4 *>(start + i0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            int4 t;
+            t.x = run;
+            run += v[4 * q];
+            t.y = run;
+            run += v[4 * q + 1];
+            t.z = run;
+            run += v[4 * q + 2];
+            t.w = run;
+            run += v[4 * q + 3];
+            o4[q] = t;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            if (i0 + q < n) start[i0 + q] = run;
+            run += v[q];
+        }
+    }
+    const bool last = ch == nch - 1 && threadIdx.x == 0;
+    if (last) start[n] = carry + tot;
+    if (tgt) {
+        int span;
+        const int nq = thread_tiles(sum, span);
+        int ttot;
+        const int o = scan256(nq, wsum, ttot) + tcarry;
+        for (int k = 0; k < nq; k++) tiles[o + k] = make_int2(first + 64 * k, min(64, span - 64 * k));
+        if (last) *ntiles = tcarry + ttot;
+    }
+}
 
 // ---- grouped binning: every system of a batched call in one launch each ----
 __global__ __launch_bounds__(256) void k_gbin_count(const pb_sys *__restrict__ sys, const int2 *__restrict__ map)
@@ -531,10 +669,11 @@ int pb_bin_points(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_
 }
 
 int pb_bin_all(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_box &box, int ns, const double *xn,
-               const double *f, int nt, const double *y)
+               const double *f, int nt, const double *y, double *zero_u)
 {
     if (!pb_bin_fused_ok(cells)) {
         int st;
+        if (zero_u) pb_zero_u(s, nt, zero_u);
         if ((st = pb_bin_points(c, s, cells, box, ns, xn, f, true))) return st;
         return pb_bin_points(c, s, cells, box, nt, y, nullptr, false);
     }
@@ -557,17 +696,24 @@ int pb_bin_all(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_box
     if ((st = pb_reserve(c, &c->tperm, sizeof(int) * ntn))) return st;
     if ((st = pb_reserve(c, &c->qoff, sizeof(int) * (nq + 1)))) return st;
     if ((st = pb_reserve(c, &c->tiles, sizeof(int2) * mt))) return st;
+    const int nchs = (ncell + kChunk - 1) / kChunk, ncht = (nkt + kChunk - 1) / kChunk;
+    if ((st = pb_reserve(c, &c->scan_tmp, sizeof(int2) * (size_t)(nchs + ncht)))) return st;
     int *scount = (int *)c->cell_count.ptr, *tcount = (int *)c->tcell_count.ptr;
-    const int nfill = ncell + nkt;
-    hipLaunchKernelGGL(k_fill_int2, dim3((nfill + 255) / 256), dim3(256), 0, s, ncell, scount, nkt, tcount);
+    const long long nu = zero_u ? 3LL * std::max(nt, 0) : 0;
+    const long long nfill = (long long)ncell + nkt + nu;
+    hipLaunchKernelGGL(k_fill_int2, dim3((unsigned)((nfill + 255) / 256)), dim3(256), 0, s, ncell, scount, nkt, tcount,
+                       nu, zero_u);
     const long long npts = (long long)ns + nt;
     if (npts > 0)
         hipLaunchKernelGGL(k_bin_count2, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, ns, xn, nt, y, box,
                            cells, scount, (int *)c->cell_of_s.ptr, (int *)c->rank_s.ptr, tcount,
                            (int *)c->cell_of_t.ptr, (int *)c->rank_t.ptr);
-    hipLaunchKernelGGL(k_bin_scan2, dim3(2), dim3(1024), 0, s, (const int *)scount, ncell, (int *)c->cell_start.ptr,
-                       (const int *)tcount, nkt, (int *)c->tcell_start.ptr, (int2 *)c->tiles.ptr,
-                       (int *)c->qoff.ptr + nq);
+    int2 *csum = (int2 *)c->scan_tmp.ptr;
+    hipLaunchKernelGGL(k_bin_sums2, dim3((unsigned)(nchs + ncht)), dim3(256), 0, s, (const int *)scount, ncell,
+                       (const int *)tcount, nkt, nchs, csum);
+    hipLaunchKernelGGL(k_bin_apply2, dim3((unsigned)(nchs + ncht)), dim3(256), 0, s, (const int *)scount, ncell,
+                       (int *)c->cell_start.ptr, (const int *)tcount, nkt, (int *)c->tcell_start.ptr, nchs, ncht,
+                       (const int2 *)csum, (int2 *)c->tiles.ptr, (int *)c->qoff.ptr + nq);
     if (npts > 0)
         hipLaunchKernelGGL(k_bin_scatter2, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, ns, xn, f, nt, y,
                            box, (const int *)c->cell_start.ptr, (const int *)c->cell_of_s.ptr,

@@ -301,10 +301,11 @@ int pb_run_mrhs_real(pb_context *c, hipStream_t s, const pb_cells &cells, const 
 // tile count is left on the device, *max_tiles bounds it.  Built by
 // pb_bin_points for targets.
 int pb_build_tiles(pb_context *c, hipStream_t s, const pb_cells &cells, int nt, int *max_tiles);
-// Sources and targets together (with the tiles), in four launches when the
-// key space is small, else the two pb_bin_points calls.
+// Sources and targets together (with the tiles), in five launches when the
+// key space is small, else the two pb_bin_points calls; with zero_u the 3 nt
+// doubles of the output are zeroed too (by the first launch).
 int pb_bin_all(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_box &box, int ns, const double *xn,
-               const double *f, int nt, const double *y);
+               const double *f, int nt, const double *y, double *zero_u = nullptr);
 
 // Workgroups are dispatched to the 8 XCDs of the device round-robin; this
 // renumbering gives each XCD one contiguous range of logical ids, so
