@@ -322,11 +322,11 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
     if ((st = pb_bin_all(c, s, cells, box, ns, xn, f, nt, y, overlap ? u : nullptr))) return st;
     pb_mark(c, s, 1);
     // Both halves: the pair kernel on a stream of its own, forked from the
-    // k-space pipeline after the spread and concurrent with the rest of it
-    // (each adds to a zeroed u with atomics; the pair kernel's latency-bound
-    // waves and the FFT's and gather's memory-bound ones fill each other's
-    // idle issue slots).  Not while every phase is timed (the phases would
-    // overlap).
+    // k-space pipeline and concurrent with the rest of it (each adds to a
+    // zeroed u with atomics; the pair kernel's latency-bound waves and the
+    // k-space kernels' matrix-core-, memory- and texture-path-bound ones fill
+    // each other's idle issue slots).  Not while every phase is timed (the
+    // phases would overlap).
     if (overlap) {
         if (!c->pair_stream &&
             (st = pb_hip_check(c, hipStreamCreateWithFlags(&c->pair_stream, hipStreamNonBlocking), "stream")))
@@ -348,11 +348,16 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
             jst = pb_hip_check(c, hipEventRecord(c->pair_event[1], c->pair_stream), "join");
             return q ? q : jst;
         };
-        // forked after the spread: beside the spread (VALU- and LDS-bound
-        // like it) the two only share the compute units, beside the FFT
-        // (memory-bound) and the gather (texture-path-bound) it fills their
-        // idle issue slots
-        st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u, 2, p->precision, &fork, 1);
+        // forked before the spread, whose FP32 form runs on the matrix cores
+        // and leaves the vector units to the pair kernel (at 1e5 -> 1e5, 128^3,
+        // P 12: mixed 0.476 against 0.487 ms with the fork after the spread,
+        // closed forms 1.106 against 1.178 ms); after the spread for the FP64
+        // table form, whose FP64 spread contends with the pair kernel for the
+        // vector units (1.063 against 1.081 ms), and for the balanced spread
+        // of large work (pb_run_kspace moves the fork there)
+        const bool fp64_table = p->precision != PB_PREC_MIXED && ntab > 0;
+        st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u, 2, p->precision, &fork,
+                           fp64_table ? 1 : 0);
         const int wst = !forked ? 0 : jst ? jst : pb_hip_check(c, hipStreamWaitEvent(s, c->pair_event[1], 0), "join");
         if (!forked && !st) st = pb_fail(c, PB_ERR_INVALID, "real space not launched");
         return st ? st : wst;
@@ -1107,7 +1112,7 @@ int group_prepare(pb_context *c, hipStream_t s, const std::vector<int> &idx, con
 }
 
 // The group's launches on stream g (pair kernels forked to two pair streams
-// after the spread and joined back); u of the group's targets must be zero.
+// before the spread and joined back); u of the group's targets must be zero.
 int group_enqueue(pb_context *c, hipStream_t g)
 {
     int st;
@@ -1153,12 +1158,15 @@ int group_enqueue(pb_context *c, hipStream_t g)
         }
         return PB_OK;
     };
-    // fork after the spread, as for one system: beside the spread the pair
-    // kernels only time-share the compute units (with the fork after the
-    // binning: 8.91 against 8.75 ms for the 512-system call)
+    // fork before the first spread launch (the 512-system call: mixed 8.40
+    // against 8.57 ms with the fork after the last one, closed forms 18.6
+    // against 20.7 ms, the FP64 table form unchanged at 16.9 ms)
     int fork_before = -1, i = 0;
     for (const auto &L : c->group_launches) {
-        if (L.kind == 4) fork_before = i + 1;
+        if (L.kind == 4) {
+            fork_before = i;
+            break;
+        }
         i++;
     }
     i = 0;

@@ -1874,6 +1874,11 @@ long long spread_multiplicity(const pb_cells &cells, const pb_kgrid &g)
 // region: beside the pair kernel, the three extra launches cost some 20-50 us
 // (measured at 1e5 sources, 128^3, P 12: 0.668 against 0.618 ms per step),
 // which only pays off against long spreading.
+bool spread_split(const pb_context *c, const pb_kgrid &g, int ns)
+{
+    return !c->arena && (double)std::max(ns, 0) * g.P[0] * g.P[1] * g.P[2] >= 1e9;
+}
+
 template <typename T>
 int spread_balanced(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells, int ns,
                     typename cplx<T>::type4 *grid4)
@@ -1882,7 +1887,7 @@ int spread_balanced(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_ce
     const int nreg = ((g.n[0] + 7) / 8) * ((g.n[1] + 7) / 8) * ((g.n[2] + 7) / 8);
     const int *cstart = (const int *)c->cell_start.ptr;
     const double4 *spos = (const double4 *)c->spos.ptr, *sfrc = (const double4 *)c->sfrc.ptr;
-    if (c->arena || (double)std::max(ns, 0) * g.P[0] * g.P[1] * g.P[2] < 1e9) {
+    if (!spread_split(c, g, ns)) {
         hipLaunchKernelGGL(k_spread_regions<T>, dim3((unsigned)nreg), dim3(64 * kSpreadWaves), 0, s, cstart, spos,
                            sfrc, cells, g, grid4);
         return PB_OK;
@@ -1946,7 +1951,11 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
     void *grid4 = c->grid.ptr, *spec = c->spec.ptr;
     // the caller's fork (the pair kernel on its own stream) at point fork_at:
     // 0 before the spread, 1 after it, 2 after the forward transform, 3 before
-    // the gather
+    // the gather; not before the balanced spread of large work, whose item
+    // workgroups (as many as fit the device) beside the pair kernel were
+    // slower (1e6 clustered sources, 256^3, P 16: 45.2-45.8 against 44.4-44.8
+    // ms per call with the fork after it)
+    if (fork_at == 0 && spread_split(c, g, ns)) fork_at = 1;
     auto fork_here = [&](int at) { return fork && fork_at == at ? (*fork)() : PB_OK; };
     if ((st = fork_here(0))) return st;
     if ((st = spread_balanced<T>(c, s, g, cells, ns, (C4 *)grid4))) return st;
