@@ -538,8 +538,10 @@ def headline_kernel_rooflines(ph, pairs, prec="float", pair_kernel="k_pair_table
                (FLOP_PER_PAIR * pairs, "TFLOP/s", PEAK_FP64)),
         _entry("pair kernel (alone)", [pair_kernel], ph["pair_alone"], cnt, "valu-fp64",
                (FLOP_PER_PAIR * pairs, "TFLOP/s", PEAK_FP64)),
-        _entry("spread", ["k_spread_regions<%s>" % prec], ph["spread"], cnt, "valu-fp32",
-               (spread_flop, "TFLOP/s", PEAK_FP32),
+        _entry("spread", ["k_spread_regions<%s>" % prec], ph["spread"], cnt, "mfma" if prec == "float" else "valu-fp64",
+               (spread_flop, "TFLOP/s", PEAK_FP32 if prec == "float" else PEAK_FP64),
+               "the one-set FP32 spread's multiply-adds run on the matrix cores (v_mfma_f32_16x16x4_f32; "
+               "counted.mfma_fp32_tflops), its window weights and staging on the vector ALUs; "
                "algorithmic: 7 P^3 FLOP per source; its bytes (24 B per source + 12 B per grid point) are in "
                "counted.hbm_gbps's terms %.1f MB" % ((24 * NS + 12 * n3) / 1e6)),
         _entry("FFT forward (z r2c + y)", ["k_fft_z_r2c<%s, 128>" % prec, "k_fft_y<%s, 128>" % prec],
@@ -558,8 +560,8 @@ def headline_kernel_rooflines(ph, pairs, prec="float", pair_kernel="k_pair_table
                (gather_flop, "TFLOP/s", PEAK_FP32),
                "bound in practice by the vector memory path: the texture address and data units are busy "
                "most of the kernel's cycles (unit_busy) for the 12-byte loads of the P^3 window points per target"),
-        _entry("binning", ["k_fill_int2", "k_bin_count2", "k_bin_scan2", "k_bin_scatter2"], ph["binning"], cnt,
-               "hbm"),
+        _entry("binning", ["k_fill_int2", "k_bin_count2", "k_bin_sums2", "k_bin_apply2", "k_bin_scatter2"],
+               ph["binning"], cnt, "hbm"),
     ]
     return [e for e in out if e]
 
@@ -744,7 +746,7 @@ def main():
         result["batched_form"] = batched_leg(ctx, args.steps, args.warmup)
         result["batched_form"]["per_kernel_roofline"] = leg_kernel_rooflines("batched", [
             ("pair kernel (degree-6 table; every system's class)", ["k_gpair_table<false, 6>"], "valu-fp64", None),
-            ("spread", ["k_gspread<float>"], "valu-fp32", None),
+            ("spread", ["k_gspread<float>"], "mfma", None),
             ("gather", ["k_ginterp<float, 10>"], "valu-fp32", None),
             ("FFT passes, 64^3 systems", ["k_gfft<float, 64, %d>" % q for q in (5, 2, 6)], "hbm", None),
             ("FFT passes, 32^3 systems", ["k_gfft<float, 32, %d>" % q for q in (5, 2, 6)], "hbm", None)])
@@ -753,7 +755,7 @@ def main():
         result["clustered_form"]["per_kernel_roofline"] = leg_kernel_rooflines("clustered", [
             ("pair kernel", ["k_pair_table<false, 6>"], "valu-fp64", None),
             ("spread (count, chunks and regions, chunk reduction)",
-             ["k_spread_count", "k_spread_items<float>", "k_spread_reduce<float>"], "valu-fp32", None),
+             ["k_spread_count", "k_spread_items<float>", "k_spread_reduce<float>"], "mfma", None),
             ("gather", ["k_interp<float, 16>"], "valu-fp32", None)])
         result["multirhs_form"] = multirhs_leg(ctx, 3)
         mr = result["multirhs_form"]
