@@ -337,27 +337,43 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
                 return st;
         bool forked = false;
         int jst = 0;
-        const std::function<int()> fork = [&]() -> int {
+        // Fork point: before the spread, whose FP32 form runs on the matrix
+        // cores and leaves the vector units to the pair kernel (at 1e5 -> 1e5,
+        // 128^3, P 12: mixed 0.476 against 0.487 ms with the fork after the
+        // spread, closed forms 1.106 against 1.178 ms); after the spread for
+        // the FP64 table form, whose FP64 spread contends with the pair kernel
+        // for the vector units (1.063 against 1.081 ms), and for the balanced
+        // spread of large work (its item workgroups fill the device: 1e6
+        // clustered sources, 256^3, P 16, 44.4-44.8 against 45.2-45.8 ms).
+        // Which of the two goes first onto the device after the fork decides
+        // which gets the compute units: the launch on the stream that waits
+        // for the fork event starts a few microseconds later.  The pair
+        // kernel goes to the pair stream (the k space first) except with the
+        // balanced spread, where the pair kernel is the longer path (the
+        // clustered system: 42.1 against 45.1 ms per call with the k space
+        // first) and stays on s, the rest of the k space moving to the pair
+        // stream.
+        const bool split = pb_spread_split(c, g, ns);
+        const bool fp64_table = p->precision != PB_PREC_MIXED && ntab > 0;
+        const std::function<int(hipStream_t &)> fork = [&](hipStream_t &ks) -> int {
             int q;
             if ((q = pb_hip_check(c, hipEventRecord(c->pair_event[0], s), "fork"))) return q;
             if ((q = pb_hip_check(c, hipStreamWaitEvent(c->pair_stream, c->pair_event[0], 0), "fork"))) return q;
             // from here every return joins the pair stream back into s, so no
             // work of this call is left running beside the caller's next call
             forked = true;
+            if (split) {
+                ks = c->pair_stream;
+                return pb_run_real(c, s, cells, kp, ntab, tdeg, box, ns, nt, u, true, far32_of(p, ntab));
+            }
             q = pb_run_real(c, c->pair_stream, cells, kp, ntab, tdeg, box, ns, nt, u, true, far32_of(p, ntab));
             jst = pb_hip_check(c, hipEventRecord(c->pair_event[1], c->pair_stream), "join");
             return q ? q : jst;
         };
-        // forked before the spread, whose FP32 form runs on the matrix cores
-        // and leaves the vector units to the pair kernel (at 1e5 -> 1e5, 128^3,
-        // P 12: mixed 0.476 against 0.487 ms with the fork after the spread,
-        // closed forms 1.106 against 1.178 ms); after the spread for the FP64
-        // table form, whose FP64 spread contends with the pair kernel for the
-        // vector units (1.063 against 1.081 ms), and for the balanced spread
-        // of large work (pb_run_kspace moves the fork there)
-        const bool fp64_table = p->precision != PB_PREC_MIXED && ntab > 0;
         st = pb_run_kspace(c, s, g, cells, box, p->alpha, p->d, ns, nt, u, 2, p->precision, &fork,
-                           fp64_table ? 1 : 0);
+                           fp64_table || split ? 1 : 0);
+        if (forked && split) // (the k space's last launch is on the pair stream)
+            jst = pb_hip_check(c, hipEventRecord(c->pair_event[1], c->pair_stream), "join");
         const int wst = !forked ? 0 : jst ? jst : pb_hip_check(c, hipStreamWaitEvent(s, c->pair_event[1], 0), "join");
         if (!forked && !st) st = pb_fail(c, PB_ERR_INVALID, "real space not launched");
         return st ? st : wst;

@@ -396,10 +396,16 @@ int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kp
                 size_t table_doubles, int table_deg, const pb_box &box, int ns, int nt, double *u, bool accumulate,
                 bool far32);
 // accumulate: 0 overwrite u, 1 add to u, 2 add with atomics (another kernel
-// may be adding to u at the same time)
+// may be adding to u at the same time).  fork, when given, is called at point
+// fork_at (0 before the spread, 1 after it, 2 after the forward transform, 3
+// before the gather) with the stream the k space runs on, which it may
+// replace for the rest of the k space.
 int pb_run_kspace(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells,
                   const pb_box &box, double a, double d, int ns, int nt, double *u, int accumulate,
-                  int precision, const std::function<int()> *fork = nullptr, int fork_at = 0);
+                  int precision, const std::function<int(hipStream_t &)> *fork = nullptr, int fork_at = 0);
+// The spread of this work takes the balanced form (count, chunked items,
+// reduction: large inputs), else one workgroup per region.
+bool pb_spread_split(const pb_context *c, const pb_kgrid &g, int ns);
 // The k table goes to c->dense_tmp (built on the device on stream s).
 int pb_run_dense(pb_context *c, hipStream_t s, double *dA, int lda, const double *y, int nt, const double *x,
                  int ns, const pb_kparams &kp, const pb_box &box, const int *lo_real, const int *hi_real,

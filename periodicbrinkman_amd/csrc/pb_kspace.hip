@@ -1874,11 +1874,6 @@ long long spread_multiplicity(const pb_cells &cells, const pb_kgrid &g)
 // region: beside the pair kernel, the three extra launches cost some 20-50 us
 // (measured at 1e5 sources, 128^3, P 12: 0.668 against 0.618 ms per step),
 // which only pays off against long spreading.
-bool spread_split(const pb_context *c, const pb_kgrid &g, int ns)
-{
-    return !c->arena && (double)std::max(ns, 0) * g.P[0] * g.P[1] * g.P[2] >= 1e9;
-}
-
 template <typename T>
 int spread_balanced(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells, int ns,
                     typename cplx<T>::type4 *grid4)
@@ -1887,7 +1882,7 @@ int spread_balanced(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_ce
     const int nreg = ((g.n[0] + 7) / 8) * ((g.n[1] + 7) / 8) * ((g.n[2] + 7) / 8);
     const int *cstart = (const int *)c->cell_start.ptr;
     const double4 *spos = (const double4 *)c->spos.ptr, *sfrc = (const double4 *)c->sfrc.ptr;
-    if (!spread_split(c, g, ns)) {
+    if (!pb_spread_split(c, g, ns)) {
         hipLaunchKernelGGL(k_spread_regions<T>, dim3((unsigned)nreg), dim3(64 * kSpreadWaves), 0, s, cstart, spos,
                            sfrc, cells, g, grid4);
         return PB_OK;
@@ -1929,7 +1924,7 @@ int spread_balanced(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_ce
 template <typename T>
 int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells,
                  const pb_box &box, double a, double d, int ns, int nt, double *u, int accumulate,
-                 const std::function<int()> *fork, int fork_at)
+                 const std::function<int(hipStream_t &)> *fork, int fork_at)
 {
     using C = typename cplx<T>::type;
     using C4 = typename cplx<T>::type4;
@@ -1949,14 +1944,11 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
     if ((st = upload_twiddles<T>(c, s))) return st;
     const kmult km = pb_make_kmult(g, box, a, d);
     void *grid4 = c->grid.ptr, *spec = c->spec.ptr;
-    // the caller's fork (the pair kernel on its own stream) at point fork_at:
-    // 0 before the spread, 1 after it, 2 after the forward transform, 3 before
-    // the gather; not before the balanced spread of large work, whose item
-    // workgroups (as many as fit the device) beside the pair kernel were
-    // slower (1e6 clustered sources, 256^3, P 16: 45.2-45.8 against 44.4-44.8
-    // ms per call with the fork after it)
-    if (fork_at == 0 && spread_split(c, g, ns)) fork_at = 1;
-    auto fork_here = [&](int at) { return fork && fork_at == at ? (*fork)() : PB_OK; };
+    // the caller's fork (the pair kernel beside the k space) at point
+    // fork_at: 0 before the spread, 1 after it, 2 after the forward
+    // transform, 3 before the gather; it may move the rest of the k space to
+    // another stream (s)
+    auto fork_here = [&](int at) { return fork && fork_at == at ? (*fork)(s) : PB_OK; };
     if ((st = fork_here(0))) return st;
     if ((st = spread_balanced<T>(c, s, g, cells, ns, (C4 *)grid4))) return st;
     if ((st = fork_here(1))) return st;
@@ -1985,11 +1977,16 @@ int run_kspace_t(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells
 
 int pb_run_kspace(pb_context *c, hipStream_t s, const pb_kgrid &g, const pb_cells &cells,
                   const pb_box &box, double a, double d, int ns, int nt, double *u, int accumulate,
-                  int precision, const std::function<int()> *fork, int fork_at)
+                  int precision, const std::function<int(hipStream_t &)> *fork, int fork_at)
 {
     if (precision == PB_PREC_MIXED)
         return run_kspace_t<float>(c, s, g, cells, box, a, d, ns, nt, u, accumulate, fork, fork_at);
     return run_kspace_t<double>(c, s, g, cells, box, a, d, ns, nt, u, accumulate, fork, fork_at);
+}
+
+bool pb_spread_split(const pb_context *c, const pb_kgrid &g, int ns)
+{
+    return !c->arena && (double)std::max(ns, 0) * g.P[0] * g.P[1] * g.P[2] >= 1e9;
 }
 
 pb_kmult pb_make_kmult(const pb_kgrid &g, const pb_box &box, double a, double d)
