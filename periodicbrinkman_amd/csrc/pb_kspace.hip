@@ -1114,13 +1114,21 @@ void launch_interp(hipStream_t s, int nt, const pb_kgrid &g, const void *tpos, c
 // conflicts) are transformed by Stockham stages of radix 8, 4, 2 (largest
 // first).  A stage reads all its butterfly inputs into registers, waits for
 // the workgroup, and writes the outputs in place, so one LDS buffer suffices.
-// Element budget per workgroup: E complex values
-// (FP32 per pass, measured on MI355X at 128^3: z passes 2048, y 4096, x with
-// the multiply 3072; FP64 2048 throughout.)
+// Element budget per workgroup: E complex values (FP32 per pass, measured on
+// MI355X at 128^3: z passes 2048, y 4096, x with the multiply 3072; FP64 2048
+// throughout).  A single system's FP32 passes on axes of <= 128 points take
+// half (z 1024, y 2048, x 1536): they run beside the pair kernel, and more,
+// smaller workgroups interleave with its (the headline step 0.479 -> 0.471
+// ms); the grouped passes and longer axes keep the full budget (the 64 sets
+// of the multi-RHS call and the 256^3 clustered system measured slower with
+// the half).
 template <typename T, int PASS> struct pbudget { static constexpr int E = 2048; };
 template <> struct pbudget<float, 0> { static constexpr int E = 2048; };
 template <> struct pbudget<float, 1> { static constexpr int E = 4096; };
 template <> struct pbudget<float, 2> { static constexpr int E = 3072; };
+template <typename T, int PASS, int N> struct sbudget {
+    static constexpr int E = sizeof(T) == 4 && N <= 128 ? pbudget<T, PASS>::E / 2 : pbudget<T, PASS>::E;
+};
 
 template <typename C> __device__ __forceinline__ C cmul(C a, C b)
 {
@@ -1258,14 +1266,14 @@ __device__ __forceinline__ void load_twiddles(C *tw, const C *__restrict__ gtw)
 // of the real grid R (T4: fx, fy, fz) are transformed as the three complex
 // pencils fx_a + i fy_a, fx_b + i fy_b, fz_a + i fz_b; the half spectra
 // (kz = 0 .. N/2) of the six real pencils are written to S.
-template <typename T, int N>
+template <typename T, int N, int E_ = pbudget<T, 0>::E>
 __device__ __forceinline__ void fft_z_r2c(const int blk, const typename cplx<T>::type4 *__restrict__ R4,
                                           typename cplx<T>::type *__restrict__ S, int n0, int n1,
                                           const typename cplx<T>::type *__restrict__ gtw)
 {
     using C = typename cplx<T>::type;
     constexpr int NP = N + 1, H = N / 2 + 1;
-    constexpr int PP = pbudget<T, 0>::E / (3 * N) > 0 ? pbudget<T, 0>::E / (3 * N) : 1;
+    constexpr int PP = E_ / (3 * N) > 0 ? E_ / (3 * N) : 1;
     constexpr int E = 3 * PP * N;
     __shared__ C buf[3 * PP * NP];
     __shared__ C tw[N];
@@ -1323,13 +1331,13 @@ __device__ __forceinline__ void fft_z_r2c(const int blk, const typename cplx<T>:
 
 // y pass (forward or inverse) on S: pencils along i1 of length N = n1 for
 // every (field, i0, kz), B consecutive (field, i0, kz) per workgroup.
-template <typename T, int N>
+template <typename T, int N, int E_ = pbudget<T, 1>::E>
 __device__ __forceinline__ void fft_y(const int blk, typename cplx<T>::type *__restrict__ S, int n0, int h2,
                                       int inverse, const typename cplx<T>::type *__restrict__ gtw)
 {
     using C = typename cplx<T>::type;
     constexpr int NP = N + 1;
-    constexpr int B = pbudget<T, 1>::E / N > 0 ? pbudget<T, 1>::E / N : 1;
+    constexpr int B = E_ / N > 0 ? E_ / N : 1;
     constexpr int E = B * N;
     __shared__ C buf[B * NP];
     __shared__ C tw[N];
@@ -1380,13 +1388,13 @@ using kmult = pb_kmult;
 // components negated: averaging over their sign keeps the spectrum Hermitian,
 // which equals the real part of the full complex transform (the
 // reference's convention).  k = 0 is set to zero.
-template <typename T, int N>
+template <typename T, int N, int E_ = pbudget<T, 2>::E>
 __device__ __forceinline__ void fft_x_mult(const int blk, typename cplx<T>::type *__restrict__ S, int n1, int h2,
                                            const kmult &km, const typename cplx<T>::type *__restrict__ gtw)
 {
     using C = typename cplx<T>::type;
     constexpr int NP = N + 1;
-    constexpr int B = pbudget<T, 2>::E / (3 * N) > 0 ? pbudget<T, 2>::E / (3 * N) : 1;
+    constexpr int B = E_ / (3 * N) > 0 ? E_ / (3 * N) : 1;
     constexpr int E = 3 * B * N;
     __shared__ C buf[3 * B * NP];
     __shared__ C tw[N];
@@ -1455,14 +1463,14 @@ __device__ __forceinline__ void fft_x_mult(const int blk, typename cplx<T>::type
 // Inverse z pass, complex to real: the half spectra of pencils a = (i0, i1)
 // and b = (i0, i1 + 1) are extended by U(N - k) = conj U(k) and transformed
 // as ux_a + i uy_a, ux_b + i uy_b, uz_a + i uz_b into the real grid (T4).
-template <typename T, int N>
+template <typename T, int N, int E_ = pbudget<T, 0>::E>
 __device__ __forceinline__ void fft_z_c2r(const int blk, const typename cplx<T>::type *__restrict__ S,
                                           trip<T> *__restrict__ U3, int n0, int n1,
                                           const typename cplx<T>::type *__restrict__ gtw)
 {
     using C = typename cplx<T>::type;
     constexpr int NP = N + 1, H = N / 2 + 1;
-    constexpr int PP = pbudget<T, 0>::E / (3 * N) > 0 ? pbudget<T, 0>::E / (3 * N) : 1;
+    constexpr int PP = E_ / (3 * N) > 0 ? E_ / (3 * N) : 1;
     constexpr int E = 3 * PP * N;
     __shared__ C buf[3 * PP * NP];
     __shared__ C tw[N];
@@ -1687,21 +1695,21 @@ __global__ __launch_bounds__(256) void k_fft_z_r2c(const typename cplx<T>::type4
                                                    typename cplx<T>::type *__restrict__ S, int n0,
                                                    int n1, const typename cplx<T>::type *__restrict__ gtw)
 {
-    fft_z_r2c<T, N>((int)blockIdx.x, R4, S, n0, n1, gtw);
+    fft_z_r2c<T, N, sbudget<T, 0, N>::E>((int)blockIdx.x, R4, S, n0, n1, gtw);
 }
 
 template <typename T, int N>
 __global__ __launch_bounds__(256) void k_fft_y(typename cplx<T>::type *__restrict__ S, int n0, int h2,
                                                int inverse, const typename cplx<T>::type *__restrict__ gtw)
 {
-    fft_y<T, N>((int)blockIdx.x, S, n0, h2, inverse, gtw);
+    fft_y<T, N, sbudget<T, 1, N>::E>((int)blockIdx.x, S, n0, h2, inverse, gtw);
 }
 
 template <typename T, int N>
 __global__ __launch_bounds__(256) void k_fft_x_mult(typename cplx<T>::type *__restrict__ S, int n1, int h2,
                                                     kmult km, const typename cplx<T>::type *__restrict__ gtw)
 {
-    fft_x_mult<T, N>((int)blockIdx.x, S, n1, h2, km, gtw);
+    fft_x_mult<T, N, sbudget<T, 2, N>::E>((int)blockIdx.x, S, n1, h2, km, gtw);
 }
 
 template <typename T, int N>
@@ -1709,7 +1717,7 @@ __global__ __launch_bounds__(256) void k_fft_z_c2r(const typename cplx<T>::type 
                                                    trip<T> *__restrict__ U3, int n0,
                                                    int n1, const typename cplx<T>::type *__restrict__ gtw)
 {
-    fft_z_c2r<T, N>((int)blockIdx.x, S, U3, n0, n1, gtw);
+    fft_z_c2r<T, N, sbudget<T, 0, N>::E>((int)blockIdx.x, S, U3, n0, n1, gtw);
 }
 
 // Grouped FFT pass PASS (0 z forward, 1 y forward, 2 x with the multiply,
@@ -1759,7 +1767,7 @@ void launch_axis(hipStream_t s, int pass, const pb_kgrid &g, void *grid4, void *
                                    (C *)spec, g.n[0], t);
         }
     } else if (pass == 0 || pass == 4) { // z
-        constexpr int PP = pbudget<T, 0>::E / (3 * N) > 0 ? pbudget<T, 0>::E / (3 * N) : 1;
+        constexpr int PP = sbudget<T, 0, N>::E / (3 * N) > 0 ? sbudget<T, 0, N>::E / (3 * N) : 1;
         const int npair = g.n[0] * (g.n[1] / 2);
         dim3 gd((npair + PP - 1) / PP);
         if (pass == 0)
@@ -1769,12 +1777,12 @@ void launch_axis(hipStream_t s, int pass, const pb_kgrid &g, void *grid4, void *
             hipLaunchKernelGGL((k_fft_z_c2r<T, N>), gd, dim3(256), 0, s, (const C *)spec, (trip<T> *)grid4,
                                g.n[0], g.n[1], t);
     } else if (pass == 1 || pass == 3) { // y
-        constexpr int B = pbudget<T, 1>::E / N > 0 ? pbudget<T, 1>::E / N : 1;
+        constexpr int B = sbudget<T, 1, N>::E / N > 0 ? sbudget<T, 1, N>::E / N : 1;
         const long long npen = 3LL * g.n[0] * h2;
         hipLaunchKernelGGL((k_fft_y<T, N>), dim3((unsigned)((npen + B - 1) / B)), dim3(256), 0, s, (C *)spec,
                            g.n[0], h2, pass == 3 ? 1 : 0, t);
     } else { // x with the multiply
-        constexpr int B = pbudget<T, 2>::E / (3 * N) > 0 ? pbudget<T, 2>::E / (3 * N) : 1;
+        constexpr int B = sbudget<T, 2, N>::E / (3 * N) > 0 ? sbudget<T, 2, N>::E / (3 * N) : 1;
         const long long npen = (long long)g.n[1] * h2;
         hipLaunchKernelGGL((k_fft_x_mult<T, N>), dim3((unsigned)((npen + B - 1) / B)), dim3(256), 0, s,
                            (C *)spec, g.n[1], h2, km, t);
