@@ -355,6 +355,7 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
         // stream.
         const bool split = pb_spread_split(c, g, ns);
         const bool fp64_table = p->precision != PB_PREC_MIXED && ntab > 0;
+        const double rounds = pb_pair_rounds(p->precision, ntab);
         const std::function<int(hipStream_t &)> fork = [&](hipStream_t &ks) -> int {
             int q;
             if ((q = pb_hip_check(c, hipEventRecord(c->pair_event[0], s), "fork"))) return q;
@@ -364,9 +365,10 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
             forked = true;
             if (split) {
                 ks = c->pair_stream;
-                return pb_run_real(c, s, cells, kp, ntab, tdeg, box, ns, nt, u, true, far32_of(p, ntab));
+                return pb_run_real(c, s, cells, kp, ntab, tdeg, box, ns, nt, u, true, far32_of(p, ntab), rounds);
             }
-            q = pb_run_real(c, c->pair_stream, cells, kp, ntab, tdeg, box, ns, nt, u, true, far32_of(p, ntab));
+            q = pb_run_real(c, c->pair_stream, cells, kp, ntab, tdeg, box, ns, nt, u, true, far32_of(p, ntab),
+                            rounds);
             jst = pb_hip_check(c, hipEventRecord(c->pair_event[1], c->pair_stream), "join");
             return q ? q : jst;
         };
@@ -384,7 +386,8 @@ int enqueue_velocity(pb_context *c, hipStream_t s, const pb_params *p, const pb_
         acc = true;
     }
     if (do_real &&
-        (st = pb_run_real(c, s, cells, kp, ntab, tdeg, box, ns, nt, u, acc, far32_of(p, ntab))))
+        (st = pb_run_real(c, s, cells, kp, ntab, tdeg, box, ns, nt, u, acc, far32_of(p, ntab),
+                           pb_pair_rounds(p->precision, ntab))))
         return st;
     return PB_OK;
 }
@@ -544,8 +547,10 @@ size_t noalloc_bytes(const pb_context *na, const pb_cells &cells, const pb_kgrid
     const size_t cb = precision == PB_PREC_MIXED ? 8 : 16;                                 // complex bytes
     b += sl((size_t)g.ng * 2 * cb) + sl(3 * (size_t)g.n[0] * g.n[1] * (g.n[2] / 2 + 1) * cb); // grid, spectrum
     if (nt > 0) { // pair-kernel statistics (pb_run_real's launch size)
-        const double tiles_est = std::max(1.0, nt / 48.0);
-        const int nsplit = (int)std::min(16.0, std::max(1.0, std::round(2.25 * 16 * na->cu_count / tiles_est)));
+        // (the most wavefronts per tile of any form)
+        const int nsplit = std::max(pb_pair_split(na->cu_count, nt, pb_pair_rounds(PB_PREC_FP64, 0)),
+                                    std::max(pb_pair_split(na->cu_count, nt, pb_pair_rounds(PB_PREC_FP64, 1)),
+                                             pb_pair_split(na->cu_count, nt, pb_pair_rounds(PB_PREC_MIXED, 1))));
         const size_t blocks = ((size_t)mt * nsplit + pb_pair_waves - 1) / pb_pair_waves;
         b += sl(sizeof(uint4) * blocks * pb_pair_waves);
     }

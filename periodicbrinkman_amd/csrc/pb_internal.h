@@ -390,11 +390,31 @@ bool pb_bin_fused_ok(const pb_cells &cells);
 // FFT twiddles of a precision (uploaded once per context)
 int pb_twiddles(pb_context *c, hipStream_t s, int precision, const void **tw);
 
+// Wavefronts per pair-kernel tile: about `rounds` rounds of resident
+// wavefronts (16 per compute unit) in total over the tiles (~48 targets each);
+// each extra wavefront repeats the tile's set-up.
+inline int pb_pair_split(int cu_count, int nt, double rounds)
+{
+    const double tiles_est = std::max(1.0, nt / 48.0);
+    return (int)std::min(16.0, std::max(1.0, std::round(rounds * 16 * cu_count / tiles_est)));
+}
+// The rounds for a form, measured beside the k space at 1e5 -> 1e5, 128^3,
+// P 12 (wavefronts per tile 2, 3, 4, 6): the mixed form with the table 1.1
+// (its pair kernel the shorter path; fewer, longer wavefronts leave the
+// compute units to the k space: 0.461-0.464 against 0.469-0.470 ms per step
+// at 2.25), the FP64 table form 2.25 (1.065 ms; 1.071-1.075 at 1.1 or 3),
+// the closed forms 3 (the longer path: 1.075-1.080 against 1.111 ms).
+inline double pb_pair_rounds(int precision, size_t table_doubles)
+{
+    if (table_doubles == 0) return 3.0;
+    return precision == PB_PREC_MIXED ? 1.1 : 2.25;
+}
 // far32: pairs beyond rfar in FP32 (table copy when table_doubles > 0, else
-// the FP32 closed form); otherwise every pair in FP64.
+// the FP32 closed form); otherwise every pair in FP64.  rounds: as in
+// pb_pair_split.
 int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp_dev,
                 size_t table_doubles, int table_deg, const pb_box &box, int ns, int nt, double *u, bool accumulate,
-                bool far32);
+                bool far32, double rounds);
 // accumulate: 0 overwrite u, 1 add to u, 2 add with atomics (another kernel
 // may be adding to u at the same time).  fork, when given, is called at point
 // fork_at (0 before the spread, 1 after it, 2 after the forward transform, 3

@@ -612,7 +612,7 @@ void pb_zero_u(hipStream_t s, int n, double *u)
 
 int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kparams *kp,
                 size_t table_doubles, int table_deg, const pb_box &box, int ns, int nt, double *u, bool accumulate,
-                bool far32)
+                bool far32, double rounds)
 {
     (void)ns;
     int st;
@@ -622,14 +622,7 @@ int pb_run_real(pb_context *c, hipStream_t s, const pb_cells &cells, const pb_kp
     if (!accumulate) pb_zero_u(s, nt, u);
     const int max_tiles = c->max_tiles; // tiles built with the target binning
     const int *ntiles = (const int *)c->qoff.ptr + cells.nkey_t / 16;
-    // wavefronts per tile: about two rounds of resident wavefronts (16 per CU)
-    // in total; each extra wavefront repeats the tile set-up
-    const double tiles_est = std::max(1.0, nt / 48.0);
-#ifndef PB_SPLIT_ROUNDS
-#define PB_SPLIT_ROUNDS 2.25
-#endif
-    const int nsplit =
-        (int)std::min(16.0, std::max(1.0, std::round(PB_SPLIT_ROUNDS * 16 * c->cu_count / tiles_est)));
+    const int nsplit = pb_pair_split(c->cu_count, nt, rounds);
     const unsigned blocks = (unsigned)(((long long)max_tiles * nsplit + kWaves - 1) / kWaves);
     if ((st = pb_reserve(c, &c->counters, sizeof(uint4) * (size_t)blocks * kWaves))) return st;
     c->stat_items = (long long)blocks * kWaves;
