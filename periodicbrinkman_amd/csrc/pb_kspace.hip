@@ -872,6 +872,21 @@ __global__ __launch_bounds__(256) void k_spread_reduce(const int *__restrict__ t
 }
 
 // ---- interpolation (gather) --------------------------------------------------
+// In FP32 the inverse transform leaves the velocity grid with the y rows
+// paired: point (x, y, z) at ((x n1/2 + y/2) n2 + z) 2 + (y & 1) (n1 is a
+// power of two, so even), the order the last pass produces its pencil pairs
+// in.  A window's z runs of two neighbouring y rows then share cache lines (one
+// 288-byte run instead of two 144-byte ones at P = 12).  The FP64 grid keeps
+// the plain (x, y, z) order, which its forms run faster with.  The offset is a
+// sum of one term per axis: this returns the axis-a term for index j.
+template <typename T> __device__ __forceinline__ int pb_grid_offset(int a, int j, int n1, int n2)
+{
+    if constexpr (sizeof(T) == 4)
+        return a == 0 ? j * n1 * n2 : (a == 1 ? (j >> 1) * 2 * n2 + (j & 1) : 2 * j);
+    else
+        return a == 0 ? j * n1 * n2 : (a == 1 ? j * n2 : j);
+}
+
 // u(y) = h^3 (pi eta)^{-3/2} sum_window e^{-|x_g - y|^2/eta} G(x_g), G = (ux,
 // uy, uz) per grid point.  One wavefront per target (targets in their sorted,
 // spatially compact order, each XCD taking one contiguous range).  The 3P
@@ -920,8 +935,7 @@ __device__ __forceinline__ void interp_wave(const int i0, int nt, const double4 
             wts[wave][w][a][p] = (T)__expf(-(float)(dx * dx) * (float)(1.0 / g.eta));
         int j = f0 + p;
         j = j < 0 ? j + g.n[a] : (j >= g.n[a] ? j - g.n[a] : j);
-        off[wave][w][a][p] = (unsigned)sizeof(trip<T>) *
-                             (unsigned)(a == 0 ? j * g.n[1] * g.n[2] : (a == 1 ? j * g.n[2] : j));
+        off[wave][w][a][p] = (unsigned)sizeof(trip<T>) * (unsigned)pb_grid_offset<T>(a, j, g.n[1], g.n[2]);
     }
     __builtin_amdgcn_wave_barrier();
     T acc[TW][3];
@@ -1040,7 +1054,8 @@ __device__ __forceinline__ void interp_any_target(const int i, int nt, const dou
                 const double dxz = (double)(f0[2] + pz) * g.h[2] - yc[2];
                 const double wt = wxy * exp(-dxz * dxz / g.eta);
                 const int jz = ((f0[2] + pz) % g.n[2] + g.n[2]) % g.n[2];
-                const trip<T> v = U3[((size_t)jx * g.n[1] + jy) * g.n[2] + jz];
+                const trip<T> v = U3[(size_t)pb_grid_offset<T>(0, jx, g.n[1], g.n[2]) + pb_grid_offset<T>(1, jy, g.n[1], g.n[2]) +
+                                     pb_grid_offset<T>(2, jz, g.n[1], g.n[2])];
                 a0 += wt * (double)v.x;
                 a1 += wt * (double)v.y;
                 a2 += wt * (double)v.z;
@@ -1524,7 +1539,8 @@ __device__ __forceinline__ void fft_z_c2r(const int blk, const typename cplx<T>:
         const int pr = pair0 + pp;
         if (pr >= npair) continue;
         const C a = buf[(3 * pp + side) * NP + z], c3 = buf[(3 * pp + 2) * NP + z];
-        U3[(size_t)(2 * pr + side) * N + z] = trip<T>{a.x, a.y, side == 0 ? c3.x : c3.y};
+        const size_t o = sizeof(T) == 4 ? ((size_t)pr * N + z) * 2 + side : (size_t)(2 * pr + side) * N + z;
+        U3[o] = trip<T>{a.x, a.y, side == 0 ? c3.x : c3.y}; // the layout of pb_grid_offset
     }
 }
 
@@ -1673,7 +1689,7 @@ __device__ __forceinline__ void fft_yz_inv(const int i0, const typename cplx<T>:
     for (int idx = threadIdx.x; idx < N * N; idx += 256) {
         const int i1 = idx / N, z = idx % N, pp = i1 >> 1, side = i1 & 1;
         const C a = buf[(3 * pp + side) * NP + z], c3 = buf[(3 * pp + 2) * NP + z];
-        P3[idx] = trip<T>{a.x, a.y, side == 0 ? c3.x : c3.y};
+        P3[sizeof(T) == 4 ? (pp * N + z) * 2 + side : idx] = trip<T>{a.x, a.y, side == 0 ? c3.x : c3.y};
     }
 }
 
